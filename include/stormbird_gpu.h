@@ -1,0 +1,149 @@
+/*
+ * stormbird_gpu.h — C ABI of the MI355X (gfx950) dynamic lifting-line library.
+ *
+ * Drop-in for Stormbird's `Simulation` (stormbird 0.9.0, Rust) on its dynamic
+ * wake path: the same JSON setup string, the same step semantics and the same
+ * result fields, with the wake state resident in GPU memory.  The convention
+ * follows the reference's own C interface (interfaces/cstormbird): opaque
+ * handle built from a NUL-terminated JSON string (NULL on error), explicit
+ * destroy, caller-owned output arrays, negative integer error codes, no
+ * panics across the ABI.  Vectors cross the ABI as double[3*n] (x,y,z per
+ * point), the layout of Rust's `[f64; 3]` / `SpatialVector`.
+ *
+ * Error codes: 0 ok; -1 null argument; -2 zero capacity; -3 size mismatch;
+ * -4 unsupported or invalid request; -5 GPU runtime error.
+ * Non-convergence of the fixed-point solve is not an error (as in the
+ * reference): it shows as iterations == max_iterations_per_time_step.
+ *
+ * Threading: one handle is used by one thread at a time; distinct handles may
+ * run concurrently (each owns its own HIP stream).
+ */
+#ifndef STORMBIRD_GPU_H
+#define STORMBIRD_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sb_sim sb_sim;
+
+/* Per-step outputs. Any array pointer may be NULL (that output is skipped).
+ * Mirrors SimulationResult (stormbird/src/common_utils/results/simulation.rs:27-40):
+ * force_input.{circulation_strength, angles_of_attack, velocity},
+ * solver_input_ctrl_points_velocity, ctrl_points, sectional_forces.*,
+ * integrated_forces[*].*, integrated_moments[*].*, iterations, residual. */
+typedef struct sb_step_result {
+    double time;
+    long iterations;
+    double residual;
+    double* circulation_strength;  /* [N]   */
+    double* angles_of_attack;      /* [N]   */
+    double* ctrl_points_velocity;  /* [3N]  force_input.velocity            */
+    double* solver_input_velocity; /* [3N]  solver_input_ctrl_points_velocity */
+    double* ctrl_points;           /* [3N]  */
+    double* sectional_forces;      /* [6][3N] circulatory, viscous_lift, sectional_drag,
+                                      added_mass, gyroscopic, total */
+    double* integrated_forces;     /* [6][3W] same order */
+    double* integrated_moments;    /* [6][3W] same order */
+} sb_step_result;
+
+/* Arithmetic of the Biot-Savart sums.
+ * SB_MODE_EXACT: the reference's operation order (no fused multiply-add,
+ *   IEEE division and square root, ascending panel order per target); results
+ *   equal the f64 CPU restatement bit for bit.
+ * SB_MODE_FAST: fused multiply-add and reciprocal square root in the pair
+ *   kernels; per-step deviation from the f64 restatement ~1e-13 relative.
+ * SB_MODE_FP32_KAHAN: pair terms in fp32 with compensated (Kahan) per-target
+ *   accumulation; solver, geometry and wake convection stay f64. */
+enum { SB_MODE_EXACT = 0, SB_MODE_FAST = 1, SB_MODE_FP32_KAHAN = 2 };
+
+/* Message of the last error on this thread. */
+const char* sb_last_error(void);
+
+/* Replaces Simulation::new_from_string (stormbird/src/lifting_line/simulation.rs:33-39).
+ * Returns NULL on a parse error or an unsupported setup (see sb_last_error). */
+sb_sim* sb_sim_create(const char* setup_json);
+/* Same, on GPU `device` with a Biot-Savart arithmetic mode (SB_MODE_*). */
+sb_sim* sb_sim_create_ex(const char* setup_json, int device, int mode);
+/* Mirrors complete_sail_model_drop (interfaces/cstormbird/src/lifting_line/complete_sail_model.rs:59-66). */
+void sb_sim_destroy(sb_sim* sim);
+
+/* Sizes: LineForceModel::nr_span_lines / nr_wings (line_force_model/data_access.rs:13-20),
+ * WakeIndices::nr_points / nr_panels (dynamic_wake/settings.rs:34-42). */
+long sb_sim_nr_ctrl_points(const sb_sim* sim);
+long sb_sim_nr_wings(const sb_sim* sim);
+long sb_sim_nr_span_points(const sb_sim* sim);
+long sb_sim_nr_wake_points(const sb_sim* sim);
+long sb_sim_nr_panels(const sb_sim* sim);
+long sb_sim_nr_freestream_points(const sb_sim* sim);
+
+/* Replaces Simulation::get_freestream_velocity_points (simulation.rs:45-63):
+ * ctrl points, then all wake points. Returns the count written (<= cap). */
+long sb_sim_get_freestream_points(const sb_sim* sim, double* xyz, size_t cap);
+
+/* Replaces Simulation::do_step (simulation.rs:107-203). `freestream_xyz` holds
+ * the freestream at every point of sb_sim_get_freestream_points, host memory. */
+int sb_sim_do_step(sb_sim* sim, double time, double time_step, const double* freestream_xyz,
+                   size_t n_points, sb_step_result* out);
+/* Same with `freestream_xyz` already in device memory (HBM-resident input). */
+int sb_sim_do_step_device(sb_sim* sim, double time, double time_step,
+                          const double* freestream_xyz_device, size_t n_points,
+                          sb_step_result* out);
+
+/* Replaces Simulation::induced_velocities (simulation.rs:207-216). */
+int sb_sim_induced_velocities(sb_sim* sim, const double* points_xyz, size_t n, double* out_xyz);
+
+/* The warm-start hook: Simulation::previous_circulation_strength (simulation.rs:26,165,189). */
+int sb_sim_get_previous_circulation(const sb_sim* sim, double* out, size_t n);
+int sb_sim_set_previous_circulation(sb_sim* sim, const double* gamma, size_t n);
+
+/* LineForceModel::set_translation_and_rotation_with_finite_difference_for_the_velocity
+ * (line_force_model/data_update.rs:188-205). */
+int sb_sim_set_translation_and_rotation_fd(sb_sim* sim, double time_step,
+                                           const double* translation3, const double* rotation3);
+/* LineForceModel::set_local_wing_angles (data_update.rs:171-177). */
+int sb_sim_set_local_wing_angles(sb_sim* sim, const double* angles, size_t n);
+
+/* DynamicWake public fields points / velocity_at_points / strengths /
+ * number_of_time_steps_completed (dynamic_wake/mod.rs:41-64) and
+ * Simulation::first_time_step_completed (simulation.rs:27). */
+int sb_sim_get_wake_state(const sb_sim* sim, double* points, double* velocities,
+                          double* strengths, long* steps_completed, int* first_step_done);
+int sb_sim_set_wake_state(sb_sim* sim, const double* points, const double* velocities,
+                          const double* strengths, long steps_completed, int first_step_done);
+/* Panel records [nr_panels][22]: 4 corners, center, normal, far-field length^2,
+ * viscous core length, area/(4 pi), strength (singularity_elements/panel.rs:16-23). */
+int sb_sim_get_panels(const sb_sim* sim, double* records);
+/* FrozenWake::fixed_velocities [3N] and variable_velocity_factors [N][N][3]
+ * row-major (wake/frozen_wake.rs:31-47). */
+int sb_sim_get_frozen_wake(const sb_sim* sim, double* fixed, double* factors);
+
+/* Per-kernel timing of the last step (milliseconds, HIP events on the
+ * handle's stream). names: "wake_update", "fixed", "factors", "solve",
+ * "forces", "self_induction".  Returns <0 for an unknown name. */
+double sb_sim_last_kernel_ms(const sb_sim* sim, const char* name);
+/* Pair evaluations with nonzero panel strength in the last step:
+ * [0] self-induction, [1] fixed velocities, [2] unit-strength factor pairs. */
+int sb_sim_last_pair_counts(const sb_sim* sim, double* counts3);
+
+/* Standalone Biot-Savart sum (kernel-level entry for parity tests): induced
+ * velocity at `n_targets` points from `n_panels` panels given as records
+ * [n_panels][22] (layout of sb_sim_get_panels), symmetry 0 none, 1 X, 2 Y,
+ * 3 Z. Host arrays; mirrors DynamicWake::induced_velocities
+ * (dynamic_wake/induced_velocity_calc.rs:18-26, 111-126). */
+int sb_wake_induced_velocities(const double* panel_records, size_t n_panels,
+                               const double* targets_xyz, size_t n_targets, int symmetry,
+                               int mode, double* out_xyz);
+
+/* Device-side elementary functions, for the host/device bit-identity test:
+ * fn 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log. */
+int sb_debug_libm_device(int fn, const double* x, const double* y, double* out, size_t n);
+/* Same functions evaluated by the library's host code. */
+int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

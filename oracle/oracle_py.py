@@ -1,0 +1,215 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper of oracle/liboracle.so, the CPU f64 restatement of Stormbird's
+dynamic lifting-line path.  Imported only by tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.
+Its Simulation class exposes the same methods as stormbird_amd.Simulation so
+that parity tests drive both the same way.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from stormbird_amd.results import StepBuffers, StepResultC
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+# "glibc": elementary functions from the host libm (what the Rust build uses on Linux);
+# "pm": the portable libm shared with the device code (bit-for-bit trajectory comparison).
+_LIB_PATHS = {"glibc": os.path.join(_HERE, "liboracle.so"), "pm": os.path.join(_HERE, "liboracle_pm.so")}
+_libs = {}
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib(variant: str = "glibc"):
+    if variant not in _libs:
+        path = _LIB_PATHS[variant]
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_sim_create.restype = vp
+        L.orc_sim_create.argtypes = [ctypes.c_char_p]
+        L.orc_sim_destroy.argtypes = [vp]
+        L.orc_sim_clone.restype = vp
+        L.orc_sim_clone.argtypes = [vp]
+        for f in ("nr_ctrl_points", "nr_wings", "nr_span_points", "nr_wake_points", "nr_panels",
+                  "nr_freestream_points"):
+            fn = getattr(L, "orc_sim_" + f)
+            fn.restype = ctypes.c_long
+            fn.argtypes = [vp]
+        L.orc_sim_get_freestream_points.restype = ctypes.c_long
+        L.orc_sim_get_freestream_points.argtypes = [vp, _dp, ctypes.c_size_t]
+        L.orc_sim_do_step.restype = ctypes.c_int
+        L.orc_sim_do_step.argtypes = [vp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t,
+                                      ctypes.POINTER(StepResultC)]
+        L.orc_sim_induced_velocities.argtypes = [vp, _dp, ctypes.c_size_t, _dp]
+        L.orc_sim_get_previous_circulation.argtypes = [vp, _dp, ctypes.c_size_t]
+        L.orc_sim_set_previous_circulation.argtypes = [vp, _dp, ctypes.c_size_t]
+        L.orc_sim_set_translation_and_rotation_fd.argtypes = [vp, ctypes.c_double, _dp, _dp]
+        L.orc_sim_set_local_wing_angles.argtypes = [vp, _dp, ctypes.c_size_t]
+        L.orc_sim_get_wake_state.argtypes = [vp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long),
+                                             ctypes.POINTER(ctypes.c_int)]
+        L.orc_sim_set_wake_state.argtypes = [vp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_int]
+        L.orc_sim_update_panels.argtypes = [vp]
+        L.orc_sim_get_panels.argtypes = [vp, _dp]
+        L.orc_sim_get_frozen_wake.argtypes = [vp, _dp, _dp]
+        L.orc_segment_unit_velocity.argtypes = [_dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_panel_unit_velocity.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp,
+                                              ctypes.c_int, _dp]
+        L.orc_foil_cl.restype = ctypes.c_double
+        L.orc_foil_cl.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_int]
+        L.orc_sigmoid.restype = ctypes.c_double
+        L.orc_sigmoid.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_libm_variant.restype = ctypes.c_int
+        L.orc_libm_eval.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
+        _libs[variant] = L
+    return _libs[variant]
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+class Simulation:
+    """CPU oracle simulation; same interface as stormbird_amd.Simulation."""
+
+    def __init__(self, setup_string: str, variant: str = "glibc", _handle=None):
+        self._v = variant
+        L = lib(variant)
+        self._h = _handle if _handle is not None else L.orc_sim_create(setup_string.encode())
+        if not self._h:
+            raise ValueError(L.orc_last_error().decode())
+        self.nr_ctrl_points = L.orc_sim_nr_ctrl_points(self._h)
+        self.nr_wings = L.orc_sim_nr_wings(self._h)
+        self.nr_span_points = L.orc_sim_nr_span_points(self._h)
+        self.nr_wake_points = L.orc_sim_nr_wake_points(self._h)
+        self.nr_panels = L.orc_sim_nr_panels(self._h)
+        self._buf = StepBuffers(self.nr_ctrl_points, self.nr_wings)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib(self._v).orc_sim_destroy(self._h)
+            self._h = None
+
+    def clone(self) -> "Simulation":
+        return Simulation("", self._v, _handle=lib(self._v).orc_sim_clone(self._h))
+
+    def nr_freestream_points(self) -> int:
+        return self.nr_ctrl_points + self.nr_wake_points
+
+    def get_freestream_velocity_points(self) -> np.ndarray:
+        n = self.nr_freestream_points()
+        out = np.zeros((n, 3))
+        lib(self._v).orc_sim_get_freestream_points(self._h, _p(out), n)
+        return out
+
+    def do_step(self, *, time: float, time_step: float, freestream_velocity):
+        fs = _f64(freestream_velocity, (-1, 3))
+        rc = lib(self._v).orc_sim_do_step(self._h, time, time_step, _p(fs), fs.shape[0],
+                                   ctypes.byref(self._buf.c))
+        if rc != 0:
+            raise RuntimeError(f"orc_sim_do_step failed ({rc}): {lib(self._v).orc_last_error().decode()}")
+        return self._buf.snapshot()
+
+    def induced_velocities(self, points) -> np.ndarray:
+        p = _f64(points, (-1, 3))
+        out = np.zeros_like(p)
+        lib(self._v).orc_sim_induced_velocities(self._h, _p(p), p.shape[0], _p(out))
+        return out
+
+    def get_previous_circulation(self) -> np.ndarray:
+        out = np.zeros(self.nr_ctrl_points)
+        lib(self._v).orc_sim_get_previous_circulation(self._h, _p(out), out.size)
+        return out
+
+    def set_previous_circulation(self, g):
+        g = _f64(g)
+        if lib(self._v).orc_sim_set_previous_circulation(self._h, _p(g), g.size) != 0:
+            raise ValueError("wrong length")
+
+    def set_translation_and_rotation_with_finite_difference_for_the_velocity(self, time_step,
+                                                                             translation, rotation):
+        t, r = _f64(translation), _f64(rotation)
+        lib(self._v).orc_sim_set_translation_and_rotation_fd(self._h, time_step, _p(t), _p(r))
+
+    def set_local_wing_angles(self, angles):
+        a = _f64(angles)
+        if lib(self._v).orc_sim_set_local_wing_angles(self._h, _p(a), a.size) != 0:
+            raise ValueError("wrong number of wing angles")
+
+    def get_wake_state(self):
+        pts = np.zeros((self.nr_wake_points, 3))
+        vel = np.zeros((self.nr_wake_points, 3))
+        st = np.zeros(self.nr_panels)
+        steps = ctypes.c_long()
+        first = ctypes.c_int()
+        lib(self._v).orc_sim_get_wake_state(self._h, _p(pts), _p(vel), _p(st), ctypes.byref(steps),
+                                     ctypes.byref(first))
+        return {"points": pts, "velocities": vel, "strengths": st, "steps_completed": steps.value,
+                "first_step_done": bool(first.value)}
+
+    def set_wake_state(self, points, velocities, strengths, steps_completed, first_step_done=True):
+        pts, vel, st = _f64(points), _f64(velocities), _f64(strengths)
+        lib(self._v).orc_sim_set_wake_state(self._h, _p(pts), _p(vel), _p(st), int(steps_completed),
+                                     1 if first_step_done else 0)
+
+    def get_panels(self) -> np.ndarray:
+        out = np.zeros((self.nr_panels, 22))
+        lib(self._v).orc_sim_get_panels(self._h, _p(out))
+        return out
+
+    def get_frozen_wake(self):
+        n = self.nr_ctrl_points
+        fixed = np.zeros((n, 3))
+        fac = np.zeros((n, n, 3))
+        lib(self._v).orc_sim_get_frozen_wake(self._h, _p(fixed), _p(fac))
+        return fixed, fac
+
+
+def segment_unit_velocity(p0, p1, x, core):
+    a, b, c = _f64(p0), _f64(p1), _f64(x)
+    out = np.zeros(3)
+    lib().orc_segment_unit_velocity(_p(a), _p(b), _p(c), core, _p(out))
+    return out
+
+
+def panel_unit_velocity(corners, x, far_ratio=5.0, core=0.0, mode=0):
+    """mode 0: near/far switch, 1: vortex ring, 2: point doublet."""
+    c, xx = _f64(corners, (12,)), _f64(x)
+    out = np.zeros(3)
+    lib().orc_panel_unit_velocity(_p(c), far_ratio, core, _p(xx), mode, _p(out))
+    return out
+
+
+def foil_value(foil_json: str, alpha: float, which: int = 0) -> float:
+    """which: 0 CL, 1 CL pre-stall with drop-off, 2 CD, 3 amount of stall, 4 post-stall CL."""
+    return lib().orc_foil_cl(foil_json.encode(), alpha, which)
+
+
+def sigmoid(x, x0, r):
+    return lib().orc_sigmoid(x, x0, r)
+
+
+def libm_eval(fn: int, x, y=None, variant: str = "glibc") -> np.ndarray:
+    """fn: 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log."""
+    x = _f64(x)
+    y = _f64(y) if y is not None else x
+    out = np.zeros_like(x)
+    lib(variant).orc_libm_eval(fn, _p(x), _p(y), _p(out), x.size)
+    return out

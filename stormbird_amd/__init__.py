@@ -1,0 +1,257 @@
+"""stormbird_amd — Stormbird's dynamic lifting-line step on an AMD Instinct MI355X.
+
+``Simulation`` mirrors the Python interface of Stormbird's ``Simulation``
+(interfaces/pystormbird/src/lifting_line/simulation.rs:16-165):
+construct from a SimulationBuilder JSON string, ``get_freestream_velocity_points``,
+``do_step(time=, time_step=, freestream_velocity=)`` returning a
+``SimulationResult``, ``induced_velocities(points)``, rigid-body and wing-angle
+setters, and the ``previous_circulation_strength`` warm-start hook.
+
+All work runs in libstormbird_amd.so (HIP kernels for gfx950) behind the C ABI
+of include/stormbird_gpu.h.  There is no CPU fallback: if the library is not
+built or no GPU is present, constructing a Simulation raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .results import StepBuffers, StepResultC, SimulationResult  # noqa: F401
+
+__all__ = ["Simulation", "SimulationResult", "MODE_EXACT", "MODE_FAST", "library_path", "lib",
+           "wake_induced_velocities"]
+
+MODE_EXACT = 0
+MODE_FAST = 1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.path.join(_HERE, "libstormbird_amd.so")
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    """Loads libstormbird_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(library_path):
+            raise RuntimeError(f"{library_path} is missing: build it with `python -m stormbird_amd.build`")
+        L = ctypes.CDLL(library_path)
+        vp = ctypes.c_void_p
+        L.sb_last_error.restype = ctypes.c_char_p
+        L.sb_sim_create.restype = vp
+        L.sb_sim_create.argtypes = [ctypes.c_char_p]
+        L.sb_sim_create_ex.restype = vp
+        L.sb_sim_create_ex.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
+        L.sb_sim_destroy.argtypes = [vp]
+        for f in ("nr_ctrl_points", "nr_wings", "nr_span_points", "nr_wake_points", "nr_panels",
+                  "nr_freestream_points"):
+            fn = getattr(L, "sb_sim_" + f)
+            fn.restype = ctypes.c_long
+            fn.argtypes = [vp]
+        L.sb_sim_get_freestream_points.restype = ctypes.c_long
+        L.sb_sim_get_freestream_points.argtypes = [vp, _dp, ctypes.c_size_t]
+        step_args = [vp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_size_t, ctypes.POINTER(StepResultC)]
+        L.sb_sim_do_step.argtypes = step_args
+        L.sb_sim_do_step_device.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.c_size_t, ctypes.POINTER(StepResultC)]
+        L.sb_sim_induced_velocities.argtypes = [vp, _dp, ctypes.c_size_t, _dp]
+        L.sb_sim_get_previous_circulation.argtypes = [vp, _dp, ctypes.c_size_t]
+        L.sb_sim_set_previous_circulation.argtypes = [vp, _dp, ctypes.c_size_t]
+        L.sb_sim_set_translation_and_rotation_fd.argtypes = [vp, ctypes.c_double, _dp, _dp]
+        L.sb_sim_set_local_wing_angles.argtypes = [vp, _dp, ctypes.c_size_t]
+        L.sb_sim_get_wake_state.argtypes = [vp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long),
+                                            ctypes.POINTER(ctypes.c_int)]
+        L.sb_sim_set_wake_state.argtypes = [vp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_int]
+        L.sb_sim_get_panels.argtypes = [vp, _dp]
+        L.sb_sim_get_frozen_wake.argtypes = [vp, _dp, _dp]
+        L.sb_sim_last_kernel_ms.restype = ctypes.c_double
+        L.sb_sim_last_kernel_ms.argtypes = [vp, ctypes.c_char_p]
+        L.sb_sim_last_pair_counts.argtypes = [vp, _dp]
+        L.sb_wake_induced_velocities.argtypes = [_dp, ctypes.c_size_t, _dp, ctypes.c_size_t, ctypes.c_int,
+                                                 ctypes.c_int, _dp]
+        L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
+        L.sb_debug_libm_host.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+def _err(what: str, rc: int):
+    raise RuntimeError(f"{what} failed ({rc}): {lib().sb_last_error().decode()}")
+
+
+class Simulation:
+    """GPU lifting-line simulation with Stormbird's Simulation interface."""
+
+    def __init__(self, setup_string: str, *, device: int = 0, mode: int = MODE_EXACT):
+        L = lib()
+        self._h = L.sb_sim_create_ex(setup_string.encode(), device, mode)
+        if not self._h:
+            raise ValueError(L.sb_last_error().decode())
+        self.mode = mode
+        self.nr_ctrl_points = L.sb_sim_nr_ctrl_points(self._h)
+        self.nr_wings = L.sb_sim_nr_wings(self._h)
+        self.nr_span_points = L.sb_sim_nr_span_points(self._h)
+        self.nr_wake_points = L.sb_sim_nr_wake_points(self._h)
+        self.nr_panels = L.sb_sim_nr_panels(self._h)
+        self._buf = StepBuffers(self.nr_ctrl_points, self.nr_wings)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().sb_sim_destroy(self._h)
+            self._h = None
+
+    def nr_freestream_points(self) -> int:
+        return self.nr_ctrl_points + self.nr_wake_points
+
+    def get_freestream_velocity_points(self) -> np.ndarray:
+        n = self.nr_freestream_points()
+        out = np.zeros((n, 3))
+        rc = lib().sb_sim_get_freestream_points(self._h, _p(out), n)
+        if rc < 0:
+            _err("get_freestream_velocity_points", rc)
+        return out
+
+    def do_step(self, *, time: float, time_step: float, freestream_velocity):
+        fs = _f64(freestream_velocity, (-1, 3))
+        rc = lib().sb_sim_do_step(self._h, time, time_step, _p(fs), fs.shape[0], ctypes.byref(self._buf.c))
+        if rc != 0:
+            _err("do_step", rc)
+        return self._buf.snapshot()
+
+    def do_step_device(self, *, time: float, time_step: float, freestream_ptr: int, n_points: int,
+                       want_result: bool = True):
+        """do_step with the freestream already in device memory (e.g. a torch tensor's data_ptr())."""
+        out = ctypes.byref(self._buf.c) if want_result else None
+        rc = lib().sb_sim_do_step_device(self._h, time, time_step, ctypes.c_void_p(freestream_ptr), n_points, out)
+        if rc != 0:
+            _err("do_step_device", rc)
+        return self._buf.snapshot() if want_result else None
+
+    def induced_velocities(self, points) -> np.ndarray:
+        p = _f64(points, (-1, 3))
+        out = np.zeros_like(p)
+        rc = lib().sb_sim_induced_velocities(self._h, _p(p), p.shape[0], _p(out))
+        if rc != 0:
+            _err("induced_velocities", rc)
+        return out
+
+    @property
+    def previous_circulation_strength(self) -> np.ndarray:
+        out = np.zeros(self.nr_ctrl_points)
+        rc = lib().sb_sim_get_previous_circulation(self._h, _p(out), out.size)
+        if rc != 0:
+            _err("get_previous_circulation", rc)
+        return out
+
+    @previous_circulation_strength.setter
+    def previous_circulation_strength(self, g):
+        g = _f64(g)
+        rc = lib().sb_sim_set_previous_circulation(self._h, _p(g), g.size)
+        if rc != 0:
+            _err("set_previous_circulation", rc)
+
+    def get_previous_circulation(self) -> np.ndarray:
+        return self.previous_circulation_strength
+
+    def set_previous_circulation(self, g):
+        self.previous_circulation_strength = g
+
+    def reset_previous_circulation_strength(self):
+        self.previous_circulation_strength = np.zeros(self.nr_ctrl_points)
+
+    def set_translation_and_rotation_with_finite_difference_for_the_velocity(self, time_step, translation,
+                                                                             rotation):
+        t, r = _f64(translation), _f64(rotation)
+        rc = lib().sb_sim_set_translation_and_rotation_fd(self._h, time_step, _p(t), _p(r))
+        if rc != 0:
+            _err("set_translation_and_rotation", rc)
+
+    def set_local_wing_angles(self, angles):
+        a = _f64(angles)
+        rc = lib().sb_sim_set_local_wing_angles(self._h, _p(a), a.size)
+        if rc != 0:
+            raise ValueError("The number of wing angles must match the number of wings in the simulation.")
+
+    def get_wake_state(self):
+        pts = np.zeros((self.nr_wake_points, 3))
+        vel = np.zeros((self.nr_wake_points, 3))
+        st = np.zeros(self.nr_panels)
+        steps = ctypes.c_long()
+        first = ctypes.c_int()
+        rc = lib().sb_sim_get_wake_state(self._h, _p(pts), _p(vel), _p(st), ctypes.byref(steps), ctypes.byref(first))
+        if rc != 0:
+            _err("get_wake_state", rc)
+        return {"points": pts, "velocities": vel, "strengths": st, "steps_completed": steps.value,
+                "first_step_done": bool(first.value)}
+
+    def set_wake_state(self, points, velocities, strengths, steps_completed, first_step_done=True):
+        pts, vel, st = _f64(points), _f64(velocities), _f64(strengths)
+        rc = lib().sb_sim_set_wake_state(self._h, _p(pts), _p(vel), _p(st), int(steps_completed),
+                                         1 if first_step_done else 0)
+        if rc != 0:
+            _err("set_wake_state", rc)
+
+    def get_panels(self) -> np.ndarray:
+        out = np.zeros((self.nr_panels, 22))
+        rc = lib().sb_sim_get_panels(self._h, _p(out))
+        if rc != 0:
+            _err("get_panels", rc)
+        return out
+
+    def get_frozen_wake(self):
+        n = self.nr_ctrl_points
+        fixed = np.zeros((n, 3))
+        fac = np.zeros((n, n, 3))
+        rc = lib().sb_sim_get_frozen_wake(self._h, _p(fixed), _p(fac))
+        if rc != 0:
+            _err("get_frozen_wake", rc)
+        return fixed, fac
+
+    def last_kernel_ms(self, name: str) -> float:
+        return lib().sb_sim_last_kernel_ms(self._h, name.encode())
+
+    def last_pair_counts(self) -> np.ndarray:
+        c = np.zeros(3)
+        lib().sb_sim_last_pair_counts(self._h, _p(c))
+        return c
+
+
+def wake_induced_velocities(panel_records, targets, symmetry: int = 0, mode: int = MODE_EXACT) -> np.ndarray:
+    """Induced velocity at `targets` from panel records [n][22] (kernel-level entry)."""
+    r = _f64(panel_records, (-1, 22))
+    t = _f64(targets, (-1, 3))
+    out = np.zeros_like(t)
+    rc = lib().sb_wake_induced_velocities(_p(r), r.shape[0], _p(t), t.shape[0], symmetry, mode, _p(out))
+    if rc != 0:
+        _err("wake_induced_velocities", rc)
+    return out
+
+
+def libm_device(fn: int, x, y=None) -> np.ndarray:
+    x = _f64(x)
+    y = _f64(y) if y is not None else x
+    out = np.zeros_like(x)
+    rc = lib().sb_debug_libm_device(fn, _p(x), _p(y), _p(out), x.size)
+    if rc != 0:
+        _err("libm_device", rc)
+    return out
+
+
+def libm_host(fn: int, x, y=None) -> np.ndarray:
+    x = _f64(x)
+    y = _f64(y) if y is not None else x
+    out = np.zeros_like(x)
+    lib().sb_debug_libm_host(fn, _p(x), _p(y), _p(out), x.size)
+    return out

@@ -1,0 +1,111 @@
+// Device-side data layout of one lifting-line instance and the kernel entry
+// points that operate on it (implemented in sb_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sb_common.h"
+
+namespace sb {
+
+// Packed per-step geometry (uploaded from the host after the rigid-body update).
+// Offsets are in doubles from the start of the geometry buffer.
+struct GeoLayout {
+    int ctrl;          // [N][3] ctrl points (global)
+    int rel_g;         // [N][3] span line vectors (global)
+    int dir_g;         // [N][3] normalized span line vectors (global)
+    int chord_g;       // [N][3] chord vectors (global)
+    int chord_len;     // [N]
+    int local_len;     // [N] length of the local span line
+    int rel_local;     // [N][3] span line vectors (local)
+    int rel_out;       // [N][3] span line vectors in the output coordinate system
+    int dir_out;       // [N][3] their directions
+    int ctrl_out;      // [N][3] ctrl points in the output coordinate system
+    int chord_out;     // [N][3] chord vectors in the output coordinate system
+    int is_virtual;    // [N] 1.0 for virtual strips
+    int span_pts;      // [P][3] span points (global)
+    int chord_span;    // [P][3] chord vectors at span points
+    int rbm;           // [18]: translation, rotation, v_lin, v_ang, (unused 6)
+    int body_axes;     // [9]: rotated unit x, y, z axes (body-fixed output)
+    int total;
+};
+
+// Instance descriptor passed by value to every kernel.
+struct Inst {
+    int N, W, P, M;     // strips, wings, span points, wake rows (panels per strip)
+    int nz_rows;        // leading panel rows that may hold nonzero strength
+    int symmetry;       // 0 none, 1 X, 2 Y, 3 Z
+    int neglect_self;   // skip same-wing pairs in the frozen wake
+    int first_dir;      // 0 chord, 1 freestream, 2 actual velocity
+    int steps_completed;
+    int output_body;
+    int viscous_lift_in_circulation;
+    int vc_mode;        // velocity correction: 0 none, 1 max ratio, 2 fixed magnitude
+    int max_iterations;
+    long end_index;     // wake points [0, end_index) receive induced velocities
+    double far_ratio, first_rel, last_rel, damping, rep_chord;
+    double solver_damping, residual_tol, strength_tol, vc_ratio, correction_factor;
+    double density;
+
+    double* pts;         // current wake points [P*(M+1)][3]
+    double* pts_next;    // next wake points (double buffer)
+    double* vel;         // velocity at wake points [P*(M+1)][3]
+    double* str;         // current strengths [N*M]
+    double* str_next;    // next strengths (double buffer)
+    const double* core;  // viscous core length [N*M]
+    const double* corners;  // optional explicit corners [N*M][12] (standalone panel sets)
+    PanelRec* rec;       // far-field panel records [N*M]
+    const int* col;      // [N] span-point column of strip i's first corner (i + wing index)
+    const int* wing_of;  // [N]
+    const int* span_src; // [P][2] ctrl-point indices mapped to span point j
+    const FoilParams* foil;  // [W]
+    const double* geo;   // packed geometry (GeoLayout)
+    GeoLayout gl;
+    const double* fs;    // freestream [N + P*(M+1)][3]
+    double* felt;        // felt ctrl velocity [N][3]
+    double* felt_span;   // felt span-point velocity [P][3]
+    double* fixed;       // fixed induced velocity at ctrl points [N][3]
+    double* F;           // unit-strength factors [N(cols j)][N(rows i)][3]
+    double* prev_gamma;  // [N] previous circulation (initial guess)
+    double* hist;        // flow-derivative history [2][N][3]
+    double* scal;        // scalars: [0..2] mean ctrl freestream, [3] fd count
+    double* out;         // packed step output (OutLayout)
+};
+
+// Packed step output, offsets in doubles.
+struct OutLayout {
+    SB_HD static long gamma(int) { return 0; }
+    SB_HD static long alpha(int N) { return N; }
+    SB_HD static long u(int N) { return 2L * N; }
+    SB_HD static long felt(int N) { return 5L * N; }
+    SB_HD static long ctrl(int N) { return 8L * N; }
+    SB_HD static long sect(int N) { return 11L * N; }                 // [6][N][3]
+    SB_HD static long integ_f(int N) { return 29L * N; }              // [6][W][3]
+    SB_HD static long integ_m(int N, int W) { return 29L * N + 18L * W; }
+    SB_HD static long iters(int N, int W) { return 29L * N + 36L * W; }
+    SB_HD static long residual(int N, int W) { return 29L * N + 36L * W + 1; }
+    SB_HD static long total(int N, int W) { return 29L * N + 36L * W + 2; }
+};
+
+enum Mode { MODE_EXACT = 0, MODE_FAST = 1, MODE_FP32_KAHAN = 2 };
+
+// Kernel launchers (sb_kernels.hip). All enqueue on `st` and return hipError_t.
+hipError_t launch_prepare(const Inst& d, int first_step, hipStream_t st);
+hipError_t launch_init_wake(const Inst& d, double dt, int use_mean_velocity, const double* velocity3,
+                            hipStream_t st);
+hipError_t launch_wake_step(const Inst& d, double dt, hipStream_t st);
+hipError_t launch_panels(const Inst& d, const double* pts, int shift_strengths, hipStream_t st);
+hipError_t launch_fixed(const Inst& d, int mode, hipStream_t st);
+hipError_t launch_factors(const Inst& d, hipStream_t st);
+hipError_t launch_solve(const Inst& d, hipStream_t st);
+hipError_t launch_set_row0(const Inst& d, hipStream_t st);
+hipError_t launch_velocity_copy(const Inst& d, hipStream_t st);
+hipError_t launch_self_induction(const Inst& d, int mode, hipStream_t st);
+hipError_t launch_forces(const Inst& d, double dt, hipStream_t st);
+hipError_t launch_flow_update(const Inst& d, hipStream_t st);
+// Induced velocity at arbitrary targets from panels [0, n_panels): out = sum.
+hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_targets, long n_panels,
+                                  int mode, double* out, hipStream_t st);
+hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st);
+
+}  // namespace sb

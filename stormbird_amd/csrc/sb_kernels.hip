@@ -1,0 +1,801 @@
+// gfx950 kernels of the dynamic lifting-line step.
+//
+// Reference (stormbird 0.9.0):
+//   singularity elements  lifting_line/singularity_elements/{vortex_line,panel,symmetry_condition}.rs
+//   wake update           lifting_line/wake/dynamic_wake/{initialization,update_data}.rs
+//   induced velocities    lifting_line/wake/dynamic_wake/induced_velocity_calc.rs
+//   frozen wake           lifting_line/wake/frozen_wake.rs
+//   solver                lifting_line/solvers/{simple_iterative,velocity_corrections}.rs
+//   sectional model       line_force_model/force_calculations.rs, section_models/foil.rs
+//   forces                common_utils/{forces_and_moments,flow_derivatives}.rs
+//
+// Compiled with -ffp-contract=off: the EXACT kernels evaluate every expression
+// in the reference's order with IEEE division and square root, so results are
+// bit-identical to the f64 CPU restatement.  The FAST pair kernels use fma and
+// a refined reciprocal square root instead.
+#include <hip/hip_runtime.h>
+
+#include "sb_device.h"
+
+namespace sb {
+
+constexpr int WAVE = 64;
+
+// ---------------------------------------------------------------------------
+// Singularity elements (exact: reference operation order)
+// ---------------------------------------------------------------------------
+
+// vortex_line.rs:59-84 + 90-106 ; rc.powi(4) = (rc*rc)*(rc*rc)
+__device__ inline double viscous_core_term(V3 l0, V3 l1, V3 x, double rc, double r1sq, double r2sq) {
+    V3 rel_line = l1 - l0;
+    V3 rel_pt = x - l0;
+    double line_len_sq = len2(rel_line);
+    double line_len = sqrt(line_len_sq);
+    V3 dir = rel_line / line_len;
+    double par = dot(rel_pt, dir);
+    double d2;
+    if (par < 0.0) d2 = r1sq;
+    else if (par > line_len) d2 = r2sq;
+    else d2 = len2(rel_pt) - par * par;
+    double rc2 = rc * rc;
+    double den = sqrt(rc2 * rc2 + d2 * d2);
+    return den > 0.0 ? d2 / den : 0.0;
+}
+
+// vortex_line.rs:18-55
+__device__ inline V3 segment_unit(V3 l0, V3 l1, V3 x, double rc) {
+    V3 r1 = x - l0;
+    V3 r2 = x - l1;
+    double r1sq = len2(r1), r2sq = len2(r2);
+    double r1l = sqrt(r1sq), r2l = sqrt(r2sq);
+    double r1r2 = r1l * r2l;
+    double den = r1r2 * (r1r2 + dot(r1, r2));
+    if (fabs(den) > CLOSENESS_ERROR) {
+        double core = rc == 0.0 ? 1.0 : viscous_core_term(l0, l1, x, rc, r1sq, r2sq);
+        double k = (r1l + r2l) / den;
+        return (cross(r1, r2) * core) * (k * FOUR_PI_INVERSE);
+    }
+    return V3{0.0, 0.0, 0.0};
+}
+
+// Corners of panel k (dynamic_wake/mod.rs:82-103): stream row s, strip i,
+// columns col[i] and col[i]+1 of rows s and s+1.
+__device__ inline void panel_corners(const Inst& d, int k, V3 c[4]) {
+    if (d.corners) {
+        const double* q = d.corners + 12L * k;
+        c[0] = load3(q);
+        c[1] = load3(q + 3);
+        c[2] = load3(q + 6);
+        c[3] = load3(q + 9);
+        return;
+    }
+    int s = k / d.N;
+    int i = k - s * d.N;
+    long a = (long)s * d.P + d.col[i];
+    long b = a + d.P;
+    c[0] = load3(d.pts + 3 * a);
+    c[1] = load3(d.pts + 3 * (a + 1));
+    c[2] = load3(d.pts + 3 * (b + 1));
+    c[3] = load3(d.pts + 3 * b);
+}
+
+// panel.rs:99-118 (vortex ring) — used for near pairs only
+__device__ inline V3 panel_rings(const Inst& d, int k, V3 x) {
+    V3 c[4];
+    panel_corners(d, k, c);
+    double rc = d.core[k];
+    V3 u{0.0, 0.0, 0.0};
+    u = u + segment_unit(c[0], c[1], x, rc);
+    u = u + segment_unit(c[1], c[2], x, rc);
+    u = u + segment_unit(c[2], c[3], x, rc);
+    u = u + segment_unit(c[3], c[0], x, rc);
+    return u;
+}
+
+template <int MODE>
+__device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
+    // panel.rs:53-76: near/far switch on |x - c|^2 <= far^2
+    double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
+    if (MODE == MODE_EXACT) {
+        double d2 = tx * tx + ty * ty + tz * tz;
+        if (d2 <= r.far2) return panel_rings(d, k, x);
+        // panel.rs:82-96 point doublet
+        double d5 = d2 * d2 * sqrt(d2);
+        double nh = tx * r.nx + ty * r.ny + tz * r.nz;
+        double h3 = 3.0 * nh;
+        V3 num{tx * h3 - r.nx * d2, ty * h3 - r.ny * d2, tz * h3 - r.nz * d2};
+        return V3{(num.x * r.area_term) / d5, (num.y * r.area_term) / d5, (num.z * r.area_term) / d5};
+    } else {
+        double d2 = fma(tx, tx, fma(ty, ty, tz * tz));
+        if (d2 <= r.far2) return panel_rings(d, k, x);
+        double y = __builtin_amdgcn_rsq(d2);
+        // one Newton step on 1/sqrt(d2): y <- y (3 - d2 y^2) / 2
+        double e = fma(-d2 * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+        double y2 = y * y;
+        double inv5 = (y2 * y2) * y;
+        double nh = fma(tx, r.nx, fma(ty, r.ny, tz * r.nz));
+        double h3 = 3.0 * nh;
+        double s = r.area_term * inv5;
+        return V3{fma(tx, h3, -r.nx * d2) * s, fma(ty, h3, -r.ny * d2) * s, fma(tz, h3, -r.nz * d2) * s};
+    }
+}
+
+// induced_velocity_calc.rs:182-198 with symmetry_condition.rs:27-43
+// The mirror flips one coordinate (multiplying by -1 is exact) and the
+// correction adds s*u_m component-wise, which equals the reference's
+// u -/+ u_m bit for bit (a - b is a + (-b) in IEEE arithmetic).  Both points
+// go through one evaluation site: with two separately inlined near-field
+// evaluations the gfx950 compiler returned the first result for both.
+template <int MODE>
+__device__ inline V3 unit_velocity_sym(const Inst& d, const PanelRec& r, int k, V3 x) {
+    int nq = d.symmetry == 0 ? 1 : 2;
+    V3 s{d.symmetry == 1 ? -1.0 : 1.0, d.symmetry == 2 ? -1.0 : 1.0, d.symmetry == 3 ? -1.0 : 1.0};
+    V3 out{0.0, 0.0, 0.0};
+#pragma nounroll
+    for (int q = 0; q < nq; ++q) {
+        V3 p = q == 0 ? x : V3{x.x * s.x, x.y * s.y, x.z * s.z};
+        V3 v = panel_unit<MODE>(d, r, k, p);
+        out = q == 0 ? v : V3{out.x + s.x * v.x, out.y + s.y * v.y, out.z + s.z * v.z};
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Biot-Savart sums
+// ---------------------------------------------------------------------------
+constexpr int TILE = 256;  // panel records per LDS tile (18 KiB)
+
+// One target per lane; panels streamed through LDS in ascending index order,
+// so each target's fold  acc = acc + strength * u  runs in the reference's
+// order (induced_velocity_calc.rs:111-126).  out[t] += acc when `add` is set
+// (update_velocity_at_points, update_data.rs:125-141), else out[t] = acc.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restrict__ targets, long n_targets,
+                                                 long n_panels, double* __restrict__ out, int add) {
+    __shared__ PanelRec tile[TILE];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long tc = t < n_targets ? t : n_targets - 1;
+    V3 x = load3(targets + 3 * tc);
+    V3 acc{0.0, 0.0, 0.0};
+    for (long base = 0; base < n_panels; base += TILE) {
+        int cnt = (int)((n_panels - base) < TILE ? (n_panels - base) : TILE);
+        __syncthreads();
+        const double* src = reinterpret_cast<const double*>(d.rec + base);
+        double* dst = reinterpret_cast<double*>(tile);
+        for (int q = threadIdx.x; q < cnt * 9; q += blockDim.x) dst[q] = src[q];
+        __syncthreads();
+        for (int p = 0; p < cnt; ++p) {
+            PanelRec r = tile[p];
+            double g = r.strength;
+            if (g == 0.0) continue;
+            V3 u = unit_velocity_sym<MODE>(d, r, (int)(base + p), x);
+            acc = acc + u * g;
+        }
+    }
+    if (t < n_targets) {
+        if (add) {
+            V3 v = load3(out + 3 * t);
+            store3(out + 3 * t, v + acc);
+        } else {
+            store3(out + 3 * t, acc);
+        }
+    }
+}
+
+// One wave per target, lanes over panels: each lane evaluates one panel's
+// term, then the wave folds the 64 terms in ascending panel order.  Used when
+// there are few targets (the N control points): it spreads one target's work
+// over 64 lanes while keeping the sequential per-target summation order.
+// Panels of the target's own wing are skipped in neglect-self mode
+// (induced_velocity_calc.rs:128-150).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fold(Inst d, const double* __restrict__ targets, int n_targets,
+                                              int p_begin, int p_end, int neglect, double* __restrict__ out) {
+    __shared__ double terms[4][WAVE * 3];
+    __shared__ int valid[4][WAVE];
+    int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    int t = blockIdx.x * 4 + wave;
+    int tc = t < n_targets ? t : n_targets - 1;
+    V3 x = load3(targets + 3 * tc);
+    int wt = neglect ? d.wing_of[tc] : -1;
+    V3 acc{0.0, 0.0, 0.0};
+    for (int base = p_begin; base < p_end; base += WAVE) {
+        int k = base + lane;
+        V3 term{0.0, 0.0, 0.0};
+        int ok = 0;
+        if (k < p_end) {
+            ok = 1;
+            if (neglect && d.wing_of[k % d.N] == wt) {
+                ok = 0;
+            } else {
+                PanelRec r = d.rec[k];
+                if (r.strength != 0.0) term = unit_velocity_sym<MODE>(d, r, k, x) * r.strength;
+            }
+        }
+        terms[wave][3 * lane + 0] = term.x;
+        terms[wave][3 * lane + 1] = term.y;
+        terms[wave][3 * lane + 2] = term.z;
+        valid[wave][lane] = ok;
+        __syncthreads();
+        int cnt = p_end - base < WAVE ? p_end - base : WAVE;
+        for (int q = 0; q < cnt; ++q) {
+            if (valid[wave][q]) acc = acc + V3{terms[wave][3 * q], terms[wave][3 * q + 1], terms[wave][3 * q + 2]};
+        }
+        __syncthreads();
+    }
+    if (t < n_targets && lane == 0) store3(out + 3 * t, acc);
+}
+
+// frozen_wake.rs:198-261: F[j][i] = unit velocity of first-row panel j at ctrl point i.
+__global__ void k_factors(Inst d) {
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long NN = (long)d.N * d.N;
+    if (idx >= NN) return;
+    int j = (int)(idx / d.N);  // panel (column)
+    int i = (int)(idx - (long)j * d.N);  // ctrl point (row)
+    V3 u{0.0, 0.0, 0.0};
+    if (!(d.neglect_self && d.wing_of[i] == d.wing_of[j])) {
+        V3 x = load3(d.geo + d.gl.ctrl + 3 * i);
+        PanelRec r = d.rec[j];
+        u = unit_velocity_sym<MODE_EXACT>(d, r, j, x);
+    }
+    store3(d.F + 3 * idx, u);
+}
+
+// ---------------------------------------------------------------------------
+// Wake shedding and convection
+// ---------------------------------------------------------------------------
+
+// Felt ctrl-point velocity (force_calculations.rs:50-69), its span-point map
+// (value_mapping.rs:87-137) and, at the first step, the mean ctrl freestream
+// and flow-derivative history (simulation.rs:65-94).
+__global__ void k_prepare(Inst d, int first_step) {
+    int t = threadIdx.x;
+    const double* rb = d.geo + d.gl.rbm;
+    V3 trans = load3(rb), vlin = load3(rb + 6), vang = load3(rb + 9);
+    for (int i = t; i < d.N; i += blockDim.x) {
+        V3 c = load3(d.geo + d.gl.ctrl + 3 * i);
+        V3 motion = vlin + cross(vang, c - trans);
+        store3(d.felt + 3 * i, load3(d.fs + 3 * i) - motion);
+    }
+    __syncthreads();
+    for (int j = t; j < d.P; j += blockDim.x) {
+        int a = d.span_src[2 * j], b = d.span_src[2 * j + 1];
+        V3 v = a == b ? load3(d.felt + 3 * a) : (load3(d.felt + 3 * a) + load3(d.felt + 3 * b)) * 0.5;
+        store3(d.felt_span + 3 * j, v);
+    }
+    if (first_step) {
+        if (t == 0) {
+            V3 s{0.0, 0.0, 0.0};
+            for (int i = 0; i < d.N; ++i) s = s + load3(d.fs + 3 * i);
+            store3(d.scal, s / (double)d.N);
+            d.scal[3] = 0.0;
+        }
+        for (int i = t; i < d.N; i += blockDim.x) {
+            V3 v = load3(d.fs + 3 * i);
+            store3(d.hist + 3 * i, v);
+            store3(d.hist + 3 * (d.N + i), v);
+            d.prev_gamma[i] = 0.0;
+        }
+    }
+}
+
+// initialization.rs:12-41: M+1 sweeps of update_wake_points_before_solving
+// with a uniform velocity v and freestream v.  Each column evolves on its
+// own.  Without shape damping the sweeps converge to
+//   p1 = p0 + dir*first_len, p_s = p_{s-1} + dt*v (2 <= s < M),
+//   p_M = p_{M-1} + last_len * normalize(p_{M-1} - p_{M-2})
+// bit for bit (row s is final after sweep s); with damping the sweeps are
+// replayed literally.
+__global__ void k_init_wake(Inst d, double dt, int use_mean, double vx, double vy, double vz, int fill) {
+    long P = d.P, M = d.M;
+    long npts = P * (M + 1);
+    V3 v = use_mean ? load3(d.scal) : V3{vx, vy, vz};
+    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long q = g; q < npts; q += (long)gridDim.x * blockDim.x) store3(d.vel + 3 * q, v);
+    for (long q = g; q < (long)d.N * M; q += (long)gridDim.x * blockDim.x) d.str[q] = 0.0;
+    if (g >= P) return;
+    int j = (int)g;
+    double first_len = d.rep_chord * d.first_rel;
+    double last_len = d.last_rel * d.rep_chord;
+    V3 sp = load3(d.geo + d.gl.span_pts + 3 * j);
+    V3 dir = d.first_dir == 0 ? normalize(load3(d.geo + d.gl.chord_span + 3 * j)) : normalize(v);
+    double* col = d.pts;  // column j: row s at col + 3*(s*P + j)
+    auto at = [&](long s) { return col + 3 * (s * P + j); };
+    if (fill)  // DynamicWakeBuilder::build: every row starts at the span points (builder.rs:127-135)
+        for (long s = 0; s <= M; ++s) store3(at(s), sp);
+    if (d.damping <= 0.0) {
+        store3(at(0), sp);
+        V3 prev = sp + dir * first_len;
+        store3(at(1), prev);
+        for (long s = 2; s < M; ++s) {
+            prev = prev + v * dt;
+            store3(at(s), prev);
+        }
+        if (M > 1) {
+            V3 a = load3(at(M - 1)), b = load3(at(M - 2));
+            store3(at(M), a + (last_len * normalize(a - b)));
+        } else {
+            store3(at(1), sp + (last_len * normalize(v)));
+        }
+    } else {
+        double dm = d.damping;
+        for (long sweep = 0; sweep <= M; ++sweep) {
+            store3(at(0), sp);
+            for (long s = M; s >= 2; --s) {
+                V3 integ = load3(at(s - 1)) + v * dt;
+                store3(at(s), load3(at(s)) * dm + integ * (1.0 - dm));
+            }
+            V3 est = sp + dir * first_len;
+            store3(at(1), load3(at(1)) * dm + est * (1.0 - dm));
+            if (M > 1) {
+                V3 a = load3(at(M - 1)), b = load3(at(M - 2));
+                store3(at(M), a + (last_len * normalize(a - b)));
+            } else {
+                store3(at(1), sp + (last_len * normalize(v)));
+            }
+        }
+    }
+}
+
+// update_data.rs:136-150 + 200-252 + 295-316: rows 0..M of the next points.
+__global__ void k_wake_step(Inst d, double dt) {
+    long P = d.P, M = d.M;
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P * (M + 1)) return;
+    long s = q / P;
+    int j = (int)(q - s * P);
+    V3 sp = load3(d.geo + d.gl.span_pts + 3 * j);
+    V3 out;
+    if (s == 0) {
+        out = sp;
+    } else if (s == 1) {
+        V3 dir;
+        if (d.first_dir == 0) dir = normalize(load3(d.geo + d.gl.chord_span + 3 * j));
+        else if (d.first_dir == 2 && d.steps_completed > 2)
+            dir = normalize((load3(d.vel + 3 * j) + load3(d.vel + 3 * (P + j))) * 0.5);
+        else dir = normalize(load3(d.felt_span + 3 * j));
+        V3 est = sp + dir * (d.rep_chord * d.first_rel);
+        out = d.damping > 0.0 ? load3(d.pts + 3 * q) * d.damping + est * (1.0 - d.damping) : est;
+    } else {
+        V3 integ = load3(d.pts + 3 * (q - P)) + load3(d.vel + 3 * (q - P)) * dt;
+        out = d.damping > 0.0 ? load3(d.pts + 3 * q) * d.damping + integ * (1.0 - d.damping) : integ;
+    }
+    store3(d.pts_next + 3 * q, out);
+}
+
+// update_data.rs:262-292: last row from the two rows before it.
+__global__ void k_wake_last(Inst d) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d.P) return;
+    long P = d.P, M = d.M;
+    double last_len = d.last_rel * d.rep_chord;
+    if (M > 1) {
+        V3 a = load3(d.pts_next + 3 * ((M - 1) * P + j));
+        V3 b = load3(d.pts_next + 3 * ((M - 2) * P + j));
+        store3(d.pts_next + 3 * (M * P + j), a + (last_len * normalize(a - b)));
+    } else {
+        V3 a = load3(d.pts_next + 3 * j);
+        store3(d.pts_next + 3 * (P + j), a + (last_len * normalize(load3(d.felt_span + 3 * j))));
+    }
+}
+
+// update_data.rs:102-115 + panel.rs:26-50 + geometry_functions.rs:11-30, and
+// the strength shift of update_data.rs:319-328 (into str_next).
+__global__ void k_panels(Inst d, int shift) {
+    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long np = (long)d.N * d.M;
+    if (k >= np) return;
+    V3 c[4];
+    panel_corners(d, (int)k, c);
+    PanelRec r;
+    V3 center = 0.25 * (c[0] + c[1] + c[2] + c[3]);
+    double a = len(c[1] - c[0]);
+    double b = len(c[2] - c[1]);
+    double cc = len(c[3] - c[2]);
+    double dd = len(c[0] - c[3]);
+    double s = (a + b + cc + dd) * 0.5;
+    double area = sqrt((s - a) * (s - b) * (s - cc) * (s - dd));
+    V3 n = normalize(cross(c[2] - c[0], c[3] - c[1]));
+    double rep = fmax_(a, b);
+    double f = rep * d.far_ratio;
+    r.cx = center.x;
+    r.cy = center.y;
+    r.cz = center.z;
+    r.nx = n.x;
+    r.ny = n.y;
+    r.nz = n.z;
+    r.far2 = f * f;
+    r.area_term = area * FOUR_PI_INVERSE;
+    double g;
+    if (shift) {
+        g = k >= d.N ? d.str[k - d.N] : d.str[k];
+        d.str_next[k] = g;
+    } else {
+        g = d.str[k];
+    }
+    r.strength = g;
+    d.rec[k] = r;
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-point solve (simple_iterative.rs:86-212), one workgroup
+// ---------------------------------------------------------------------------
+struct Strip {
+    V3 felt, fixed, rel_g, dir_g, chord_g, rel_out;
+    double c, local_len;
+    FoilParams foil;
+};
+
+__device__ inline double strip_cl_pre(const Strip& s, double a) { return foil_cl_pre_dropoff(s.foil, a); }
+
+// force_calculations.rs:552-627 (one strip's |CL(Gamma) - CL(alpha)|)
+__device__ inline double residual_term(const Strip& s, double rho, double g, double a, V3 u) {
+    V3 F = len(u) == 0.0 ? V3{0.0, 0.0, 0.0} : (cross(u, s.rel_out) * g) * rho;
+    double sign = -1.0 * signum(g);
+    double area = s.c * s.local_len;
+    double ff = 0.5 * area * rho * len2(u);
+    double clg = ff == 0.0 ? 0.0 : len(F) * sign / ff;
+    return fabs(clg - strip_cl_pre(s, a));
+}
+
+// force_calculations.rs:271-292
+__device__ inline double circulation(const Strip& s, int viscous, double a, V3 u) {
+    double cl = strip_cl_pre(s, a);
+    if (viscous) cl += foil_cl_post_weighted(s.foil, a);
+    return -0.5 * s.c * len(u) * cl;
+}
+
+__global__ __launch_bounds__(1024) void k_solve(Inst d) {
+    extern __shared__ double sh[];
+    int N = d.N;
+    double* G = sh;             // [N] current circulation (broadcast for the matvec)
+    double* R = sh + N;         // [N] residual terms
+    double* wm = R + N;         // [16] per-wave maxima
+    double* ctl = wm + 16;      // [0] residual, [1] converged flag
+    int i = threadIdx.x;
+    bool act = i < N;
+    int ic = act ? i : N - 1;
+    const double* geo = d.geo;
+    Strip s;
+    s.felt = load3(d.felt + 3 * ic);
+    s.fixed = load3(d.fixed + 3 * ic);
+    s.rel_g = load3(geo + d.gl.rel_g + 3 * ic);
+    s.dir_g = load3(geo + d.gl.dir_g + 3 * ic);
+    s.chord_g = load3(geo + d.gl.chord_g + 3 * ic);
+    s.rel_out = load3(geo + d.gl.rel_out + 3 * ic);
+    s.c = geo[d.gl.chord_len + ic];
+    s.local_len = geo[d.gl.local_len + ic];
+    s.foil = d.foil[d.wing_of[ic]];
+    double rho = d.density;
+
+    double g = d.prev_gamma[ic];
+    if (fabs(d.correction_factor - 1.0) > MIN_POSITIVE) g *= d.correction_factor;
+    V3 u{0.0, 0.0, 0.0};
+    double a = 0.0;
+    if (act) R[i] = residual_term(s, rho, g, a, u);
+    __syncthreads();
+    if (i == 0) {
+        double sum = 0.0;
+        for (int q = 0; q < N; ++q) sum += R[q];
+        ctl[0] = sum / (double)N;
+        ctl[1] = 0.0;
+    }
+    __syncthreads();
+    int it = 0;
+    bool conv = false;
+    double est = 0.0;
+    while (it < d.max_iterations && !conv) {
+        it += 1;
+        if (act) G[i] = g;
+        __syncthreads();
+        double ad = 0.0;
+        if (act) {
+            // frozen_wake.rs:268-301
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            const double* Fi = d.F + 3 * i;
+            for (int j = 0; j < N; ++j) {
+                const double* f = Fi + 3L * N * j;
+                double gj = G[j];
+                sx += f[0] * gj;
+                sy += f[1] * gj;
+                sz += f[2] * gj;
+            }
+            V3 ind = s.fixed;
+            ind.x += sx;
+            ind.y += sy;
+            ind.z += sz;
+            // velocity_corrections.rs:18-100
+            if (d.vc_mode == 1) {
+                double um = len(ind), fm = len(s.felt);
+                double mx = d.vc_ratio * fm;
+                V3 cvel = um > mx ? normalize(ind) * mx : ind;
+                u = s.felt + cvel;
+            } else if (d.vc_mode == 2) {
+                V3 tot = s.felt + ind;
+                u = normalize(tot) * len(s.felt);
+            } else {
+                u = s.felt + ind;
+            }
+            u = u - project(u, s.rel_g);  // line_force_model/mod.rs:235-254
+            a = len(u) > MIN_POSITIVE ? signed_angle_between(s.chord_g, u, s.dir_g) : 0.0;
+            est = circulation(s, d.viscous_lift_in_circulation, a, u);
+            R[i] = residual_term(s, rho, g, a, u);
+            double diff = est - g;
+            ad = fabs(diff);
+            g += d.solver_damping * diff;
+        }
+        // max |est - g| over strips (order-free, exact)
+        double m = ad;
+        for (int off = WAVE / 2; off > 0; off >>= 1) m = fmax_(m, __shfl_xor(m, off, WAVE));
+        if ((i % WAVE) == 0) wm[i / WAVE] = m;
+        __syncthreads();
+        if (i == 0) {
+            double sum = 0.0;
+            for (int q = 0; q < N; ++q) sum += R[q];
+            double residual = sum / (double)N;
+            double mx = 0.0;
+            for (int w = 0; w < (int)((blockDim.x + WAVE - 1) / WAVE); ++w) mx = fmax_(mx, wm[w]);
+            ctl[0] = residual;
+            ctl[1] = (residual < d.residual_tol || mx < d.strength_tol) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        conv = ctl[1] != 0.0;
+    }
+    if (act) {
+        double gamma = circulation(s, d.viscous_lift_in_circulation, a, u);
+        double* o = d.out;
+        o[OutLayout::gamma(N) + i] = gamma;
+        o[OutLayout::alpha(N) + i] = len(u) > MIN_POSITIVE ? signed_angle_between(s.chord_g, u, s.dir_g) : 0.0;
+        store3(o + OutLayout::u(N) + 3 * i, u);
+        store3(o + OutLayout::felt(N) + 3 * i, s.felt);
+        // update_wing_strength (update_data.rs:146-150): first-row strengths
+        d.str[i] = gamma;
+        d.rec[i].strength = gamma;
+        d.prev_gamma[i] = gamma;
+    }
+    if (i == 0) {
+        d.out[OutLayout::iters(N, d.W)] = (double)it;
+        d.out[OutLayout::residual(N, d.W)] = ctl[0];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forces (force_calculations.rs:322-548, forces_and_moments.rs:163-245,
+// flow_derivatives.rs:31-62), one workgroup
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_forces(Inst d, double dt) {
+    int N = d.N, W = d.W;
+    const double* geo = d.geo;
+    double* o = d.out;
+    int i = threadIdx.x;
+    double count = d.scal[3];
+    if (i < N) {
+        double gamma = o[OutLayout::gamma(N) + i];
+        double alpha = o[OutLayout::alpha(N) + i];
+        V3 ug = load3(o + OutLayout::u(N) + 3 * i);
+        V3 acc{0.0, 0.0, 0.0};
+        if (count >= 2.0) {  // first_derivative_second_order_backward
+            V3 h0 = load3(d.hist + 3 * i), h1 = load3(d.hist + 3 * (N + i));
+            acc = (ug * 3.0 - h1 * 4.0 + h0) / (2.0 * dt);
+        }
+        V3 vel = ug;
+        if (d.output_body) {
+            const double* ax = geo + d.gl.body_axes;
+            V3 X = load3(ax), Y = load3(ax + 3), Z = load3(ax + 6);
+            vel = V3{dot(ug, X), dot(ug, Y), dot(ug, Z)};
+            acc = V3{dot(acc, X), dot(acc, Y), dot(acc, Z)};
+        }
+        const FoilParams& foil = d.foil[d.wing_of[i]];
+        double rho = d.density;
+        double c = geo[d.gl.chord_len + i];
+        double llen = geo[d.gl.local_len + i];
+        V3 rel_out = load3(geo + d.gl.rel_out + 3 * i);
+        V3 rel_local = load3(geo + d.gl.rel_local + 3 * i);
+        V3 F[6];
+        F[0] = len(vel) == 0.0 ? V3{0.0, 0.0, 0.0} : (cross(vel, rel_out) * gamma) * rho;
+        F[1] = V3{0.0, 0.0, 0.0};
+        if (!d.viscous_lift_in_circulation) {
+            double clv = foil_cl_post_weighted(foil, alpha);
+            V3 dir = normalize(cross(rel_local, vel));
+            double area = c * llen;
+            double ff = 0.5 * area * rho * len2(vel);
+            F[1] = dir * clv * ff;
+        }
+        {
+            double cd = foil_cd(foil, alpha);
+            V3 dir = normalize(vel);
+            double area = c * llen;
+            double ff = 0.5 * area * rho * len2(vel);
+            F[2] = dir * cd * ff;
+        }
+        {
+            V3 dir_out = load3(geo + d.gl.dir_out + 3 * i);
+            V3 chord_out = load3(geo + d.gl.chord_out + 3 * i);
+            double area = c * len(rel_out);
+            V3 ra = acc;
+            ra = ra - project(ra, dir_out);
+            ra = ra - project(ra, chord_out);
+            if (len(ra) == 0.0) F[3] = V3{0.0, 0.0, 0.0};
+            else {
+                double coef = foil.added_mass_factor * len(ra);
+                F[3] = (coef * rho * area) * normalize(ra);
+            }
+        }
+        F[4] = V3{0.0, 0.0, 0.0};
+        V3 tot = F[0];
+        tot = tot + F[1];
+        tot = tot + F[2];
+        tot = tot + F[3];
+        tot = tot + F[4];
+        F[5] = tot;
+        for (int k = 0; k < 6; ++k) store3(o + OutLayout::sect(N) + 3 * ((long)k * N + i), F[k]);
+        store3(o + OutLayout::ctrl(N) + 3 * i, load3(geo + d.gl.ctrl + 3 * i));
+        store3(o + OutLayout::u(N) + 3 * i, vel);  // force_input.velocity is in the output frame
+        // flow-derivative history update (global-frame velocity)
+        store3(d.hist + 3 * i, load3(d.hist + 3 * (N + i)));
+        store3(d.hist + 3 * (N + i), ug);
+    }
+    __syncthreads();
+    for (int idx = i; idx < 6 * W; idx += blockDim.x) {
+        int k = idx / W, w = idx - k * W;
+        int a = -1, b = -1;
+        for (int q = 0; q < N; ++q) {
+            if (d.wing_of[q] == w) {
+                if (a < 0) a = q;
+                b = q + 1;
+            }
+        }
+        V3 fs{0.0, 0.0, 0.0}, ms{0.0, 0.0, 0.0};
+        for (int q = a; a >= 0 && q < b; ++q) {
+            if (geo[d.gl.is_virtual + q] != 0.0) continue;
+            V3 f = load3(o + OutLayout::sect(N) + 3 * ((long)k * N + q));
+            fs = fs + f;
+            ms = ms + cross(load3(geo + d.gl.ctrl_out + 3 * q), f);
+        }
+        store3(o + OutLayout::integ_f(N) + 3 * ((long)k * W + w), fs);
+        store3(o + OutLayout::integ_m(N, W) + 3 * ((long)k * W + w), ms);
+    }
+    if (i == 0) d.scal[3] = count + 1.0;
+}
+
+__global__ void k_copy3(const double* __restrict__ src, double* __restrict__ dst, long n) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < 3 * n) dst[q] = src[q];
+}
+
+__global__ void k_libm(int fn, const double* x, const double* y, double* out, long n) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double r;
+    switch (fn) {
+        case 0: r = sbm::sin(x[q]); break;
+        case 1: r = sbm::cos(x[q]); break;
+        case 2: r = sbm::exp(x[q]); break;
+        case 3: r = sbm::acos(x[q]); break;
+        case 4: r = sbm::pow(x[q], y[q]); break;
+        default: r = sbm::log(x[q]); break;
+    }
+    out[q] = r;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_prepare(const Inst& d, int first_step, hipStream_t st) {
+    hipLaunchKernelGGL(k_prepare, dim3(1), dim3(1024), 0, st, d, first_step);
+    return hipGetLastError();
+}
+
+hipError_t launch_init_wake(const Inst& d, double dt, int use_mean, const double* v3p, hipStream_t st) {
+    long n = (long)d.P * (d.M + 1);
+    unsigned nb = blocks_for(n, 256);
+    double vx = v3p ? v3p[0] : 0.0, vy = v3p ? v3p[1] : 0.0, vz = v3p ? v3p[2] : 0.0;
+    int fill = use_mean ? 0 : 1;
+    hipLaunchKernelGGL(k_init_wake, dim3(nb), dim3(256), 0, st, d, dt, use_mean, vx, vy, vz, fill);
+    return hipGetLastError();
+}
+
+hipError_t launch_wake_step(const Inst& d, double dt, hipStream_t st) {
+    long n = (long)d.P * (d.M + 1);
+    hipLaunchKernelGGL(k_wake_step, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, dt);
+    hipLaunchKernelGGL(k_wake_last, dim3(blocks_for(d.P, 256)), dim3(256), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_panels(const Inst& d, const double*, int shift, hipStream_t st) {
+    long n = (long)d.N * d.M;
+    hipLaunchKernelGGL(k_panels, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, shift);
+    return hipGetLastError();
+}
+
+hipError_t launch_fixed(const Inst& d, int mode, hipStream_t st) {
+    int p_begin = d.N;
+    int p_end = d.N * d.nz_rows;
+    if (p_end < p_begin) p_end = p_begin;
+    const double* ctrl = d.geo + d.gl.ctrl;
+    unsigned nb = blocks_for(d.N, 4);
+    if (mode == MODE_FAST)
+        hipLaunchKernelGGL(k_fold<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, ctrl, d.N, p_begin, p_end,
+                           d.neglect_self, d.fixed);
+    else
+        hipLaunchKernelGGL(k_fold<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, ctrl, d.N, p_begin, p_end,
+                           d.neglect_self, d.fixed);
+    return hipGetLastError();
+}
+
+hipError_t launch_factors(const Inst& d, hipStream_t st) {
+    long n = (long)d.N * d.N;
+    hipLaunchKernelGGL(k_factors, dim3(blocks_for(n, 256)), dim3(256), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_solve(const Inst& d, hipStream_t st) {
+    int threads = ((d.N + WAVE - 1) / WAVE) * WAVE;
+    size_t shm = sizeof(double) * (2 * (size_t)d.N + 16 + 4);
+    hipLaunchKernelGGL(k_solve, dim3(1), dim3(threads), shm, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_row0(const Inst&, hipStream_t) { return hipSuccess; }
+
+hipError_t launch_velocity_copy(const Inst& d, hipStream_t st) {
+    long n = (long)d.P * (d.M + 1);
+    hipLaunchKernelGGL(k_copy3, dim3(blocks_for(3 * n, 256)), dim3(256), 0, st, d.fs + 3L * d.N, d.vel, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_self_induction(const Inst& d, int mode, hipStream_t st) {
+    long np = (long)d.P * (d.M + 1);
+    long end = d.end_index < np ? d.end_index : np;
+    long n_panels = (long)d.N * d.nz_rows;
+    if (end <= 0 || n_panels <= 0) return hipSuccess;
+    unsigned nb = blocks_for(end, 256);
+    if (mode == MODE_FAST)
+        hipLaunchKernelGGL(k_targets<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, d.pts, end, n_panels, d.vel, 1);
+    else
+        hipLaunchKernelGGL(k_targets<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, d.pts, end, n_panels, d.vel, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces(const Inst& d, double dt, hipStream_t st) {
+    int threads = ((d.N + WAVE - 1) / WAVE) * WAVE;
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(threads), 0, st, d, dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_flow_update(const Inst&, hipStream_t) { return hipSuccess; }
+
+hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode,
+                                  double* out, hipStream_t st) {
+    if (n_targets <= 0) return hipSuccess;
+    if (n_panels <= 0) return hipMemsetAsync(out, 0, sizeof(double) * 3 * n_targets, st);
+    if (n_targets < 2048) {
+        unsigned nb = blocks_for(n_targets, 4);
+        if (mode == MODE_FAST)
+            hipLaunchKernelGGL(k_fold<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, targets, (int)n_targets, 0,
+                               (int)n_panels, 0, out);
+        else
+            hipLaunchKernelGGL(k_fold<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, targets, (int)n_targets, 0,
+                               (int)n_panels, 0, out);
+    } else {
+        unsigned nb = blocks_for(n_targets, 256);
+        if (mode == MODE_FAST)
+            hipLaunchKernelGGL(k_targets<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, targets, n_targets, n_panels,
+                               out, 0);
+        else
+            hipLaunchKernelGGL(k_targets<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, targets, n_targets, n_panels,
+                               out, 0);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_libm, dim3(blocks_for(n, 256)), dim3(256), 0, st, fn, x, y, out, n);
+    return hipGetLastError();
+}
+
+}  // namespace sb

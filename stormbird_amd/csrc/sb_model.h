@@ -1,0 +1,90 @@
+// Host-side model of a lifting-line simulation: setup parsing, line-force-model
+// geometry and solver/wake settings.  The per-step O(N) geometry (rigid-body
+// transform of span lines and chord vectors) runs here and is uploaded; all
+// O(N*M) and O(N^2) work runs on the GPU.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "sb_common.h"
+
+namespace sb {
+
+enum class RotType { XYZ, YawPitchRoll };
+
+// stormath/src/rigid_body_motion.rs:22-103
+struct RigidBody {
+    V3 translation{0, 0, 0}, rotation{0, 0, 0}, velocity_linear{0, 0, 0}, velocity_angular{0, 0, 0};
+    RotType rot = RotType::XYZ;
+    V3 rotate(V3 v) const;
+    V3 transform_point(V3 p) const { return rotate(p) + translation; }
+    V3 transform_vector(V3 v) const { return rotate(v); }
+    V3 body_fixed(V3 v) const;
+    V3 velocity_at_point(V3 p) const { return velocity_linear + cross(velocity_angular, p - translation); }
+    void update_translation_fd(V3 t, double dt);
+    void update_rotation_fd(V3 r, double dt);
+};
+
+V3 rotate_around_axis(V3 v, double angle, V3 axis);
+
+struct SpanLine {
+    V3 start, end;
+    V3 rel() const { return end - start; }
+    double length() const { return len(rel()); }
+    V3 dir() const { return normalize(rel()); }
+    V3 ctrl() const { return 0.5 * (start + end); }
+};
+
+struct LineModel {
+    std::vector<SpanLine> local, global;
+    std::vector<V3> chord_local_not_rotated, chord_local, chord_global, chord_global_at_span;
+    std::vector<double> chord_len;
+    std::vector<char> is_virtual;
+    std::vector<FoilParams> foils;  // one per wing
+    std::vector<int> wing_start, wing_end, wing_of;
+    std::vector<double> wing_angles;
+    std::vector<V3> span_points, ctrl_points;
+    double density = 1.225;
+    bool output_body = false;
+    bool viscous_lift_in_circulation = false;
+    RigidBody rbm;
+
+    int n() const { return (int)local.size(); }
+    int nw() const { return (int)wing_start.size(); }
+    void update_global();  // data_update.rs:13-90
+    template <class T>
+    std::vector<T> span_values(const std::vector<T>& cv) const;  // value_mapping.rs:87-137
+};
+
+struct WakeConfig {
+    int M = 100;
+    double first_rel = 0.75, last_rel = 0.25, damping = 0.0, far_ratio = 5.0;
+    int first_dir = 0;  // 0 chord, 1 freestream, 2 actual velocity
+    int symmetry = 0;   // 0 none, 1 X, 2 Y, 3 Z
+    bool neglect_self = false;
+    long end_index = 0;
+    double rep_chord = 0.0;
+    double initial_relative_wake_length = 50.0;
+    std::vector<double> core;  // per panel, [M][N]
+};
+
+struct SolverConfig {
+    int max_iterations = 40;
+    double damping = 0.05, residual_tol = 1e-4, strength_tol = 1e-6;
+    int velocity_correction = 0;  // 0 none, 1 max ratio, 2 fixed magnitude
+    double velocity_correction_ratio = 0.0;
+    double correction_factor_initial = 1.0;
+};
+
+struct Setup {
+    LineModel lfm;
+    WakeConfig wake;
+    SolverConfig solver;
+};
+
+// Parses a SimulationBuilder JSON string (simulation_builder.rs:34-143). Throws
+// std::runtime_error for malformed input and for settings outside this path.
+Setup parse_setup(const char* json);
+
+}  // namespace sb

@@ -1,0 +1,699 @@
+// Host orchestration of the dynamic lifting-line step on one GPU, and the C ABI
+// declared in include/stormbird_gpu.h.
+//
+// Step sequence (stormbird/src/lifting_line/simulation.rs:107-203):
+//   host  : line-force-model geometry update (O(N)), packed and uploaded
+//   device: felt velocities -> [first step: wake initialisation] -> wake
+//           shedding/convection -> panel rebuild + strength shift -> fixed
+//           velocities + unit factors (frozen wake) -> fixed-point solve ->
+//           wake velocities (freestream + free-wake self-induction) -> forces
+//   host  : one copy of the packed results into the caller's arrays
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/stormbird_gpu.h"
+#include "sb_device.h"
+#include "sb_model.h"
+
+using namespace sb;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e), (int)e);
+}
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+const char* kTimerNames[] = {"wake_update", "fixed", "factors", "solve", "self_induction", "forces"};
+constexpr int kTimers = 6;
+
+}  // namespace
+
+struct sb_sim {
+    Setup setup;
+    int device = 0;
+    int mode = MODE_EXACT;
+    hipStream_t stream = nullptr;
+    int N = 0, W = 0, P = 0, M = 0;
+    long nr_points = 0, nr_panels = 0;
+    int nz_rows = 0;
+    long steps_completed = 0;
+    bool first_done = false;
+    GeoLayout gl{};
+
+    // device buffers
+    double *d_pts[2] = {nullptr, nullptr}, *d_vel = nullptr, *d_str[2] = {nullptr, nullptr};
+    double *d_core = nullptr, *d_geo = nullptr, *d_fs = nullptr, *d_felt = nullptr, *d_felt_span = nullptr;
+    double *d_fixed = nullptr, *d_F = nullptr, *d_prev = nullptr, *d_hist = nullptr, *d_scal = nullptr;
+    double *d_out = nullptr, *d_tmp = nullptr;
+    size_t tmp_cap = 0;
+    PanelRec* d_rec = nullptr;
+    int *d_col = nullptr, *d_wing = nullptr, *d_span_src = nullptr;
+    FoilParams* d_foil = nullptr;
+    int cur = 0, scur = 0;
+
+    std::vector<double> h_geo, h_out;
+    hipEvent_t ev[kTimers + 1] = {};
+    double pairs[3] = {0, 0, 0};
+    double timer_ms[kTimers] = {0};
+
+    ~sb_sim() {
+        (void)hipSetDevice(device);
+        void* bufs[] = {d_pts[0], d_pts[1], d_vel, d_str[0], d_str[1], d_core, d_geo, d_fs, d_felt,
+                        d_felt_span, d_fixed, d_F, d_prev, d_hist, d_scal, d_out, d_tmp, d_rec,
+                        d_col, d_wing, d_span_src, d_foil};
+        for (void* p : bufs)
+            if (p) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    Inst inst() const {
+        Inst d{};
+        const WakeConfig& w = setup.wake;
+        const SolverConfig& s = setup.solver;
+        d.N = N;
+        d.W = W;
+        d.P = P;
+        d.M = M;
+        d.nz_rows = nz_rows;
+        d.symmetry = w.symmetry;
+        d.neglect_self = w.neglect_self ? 1 : 0;
+        d.first_dir = w.first_dir;
+        d.steps_completed = (int)steps_completed;
+        d.output_body = setup.lfm.output_body ? 1 : 0;
+        d.viscous_lift_in_circulation = setup.lfm.viscous_lift_in_circulation ? 1 : 0;
+        d.vc_mode = s.velocity_correction;
+        d.max_iterations = s.max_iterations;
+        d.end_index = w.end_index;
+        d.far_ratio = w.far_ratio;
+        d.first_rel = w.first_rel;
+        d.last_rel = w.last_rel;
+        d.damping = w.damping;
+        d.rep_chord = w.rep_chord;
+        d.solver_damping = s.damping;
+        d.residual_tol = s.residual_tol;
+        d.strength_tol = s.strength_tol;
+        d.vc_ratio = s.velocity_correction_ratio;
+        d.correction_factor = s.correction_factor_initial;
+        d.density = setup.lfm.density;
+        d.pts = d_pts[cur];
+        d.pts_next = d_pts[cur ^ 1];
+        d.vel = d_vel;
+        d.str = d_str[scur];
+        d.str_next = d_str[scur ^ 1];
+        d.core = d_core;
+        d.corners = nullptr;
+        d.rec = d_rec;
+        d.col = d_col;
+        d.wing_of = d_wing;
+        d.span_src = d_span_src;
+        d.foil = d_foil;
+        d.geo = d_geo;
+        d.gl = gl;
+        d.fs = d_fs;
+        d.felt = d_felt;
+        d.felt_span = d_felt_span;
+        d.fixed = d_fixed;
+        d.F = d_F;
+        d.prev_gamma = d_prev;
+        d.hist = d_hist;
+        d.scal = d_scal;
+        d.out = d_out;
+        return d;
+    }
+
+    void layout() {
+        int o = 0;
+        auto take = [&](int n) { int r = o; o += n; return r; };
+        gl.ctrl = take(3 * N);
+        gl.rel_g = take(3 * N);
+        gl.dir_g = take(3 * N);
+        gl.chord_g = take(3 * N);
+        gl.chord_len = take(N);
+        gl.local_len = take(N);
+        gl.rel_local = take(3 * N);
+        gl.rel_out = take(3 * N);
+        gl.dir_out = take(3 * N);
+        gl.ctrl_out = take(3 * N);
+        gl.chord_out = take(3 * N);
+        gl.is_virtual = take(N);
+        gl.span_pts = take(3 * P);
+        gl.chord_span = take(3 * P);
+        gl.rbm = take(18);
+        gl.body_axes = take(9);
+        gl.total = o;
+    }
+
+    void pack_geometry() {
+        const LineModel& m = setup.lfm;
+        h_geo.assign(gl.total, 0.0);
+        double* g = h_geo.data();
+        auto put = [](double* p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; };
+        for (int i = 0; i < N; ++i) {
+            const SpanLine& G = m.global[i];
+            const SpanLine& L = m.local[i];
+            const SpanLine& O = m.output_body ? L : G;
+            put(g + gl.ctrl + 3 * i, m.ctrl_points[i]);
+            put(g + gl.rel_g + 3 * i, G.rel());
+            put(g + gl.dir_g + 3 * i, G.dir());
+            put(g + gl.chord_g + 3 * i, m.chord_global[i]);
+            g[gl.chord_len + i] = m.chord_len[i];
+            g[gl.local_len + i] = L.length();
+            put(g + gl.rel_local + 3 * i, L.rel());
+            put(g + gl.rel_out + 3 * i, O.rel());
+            put(g + gl.dir_out + 3 * i, O.dir());
+            put(g + gl.ctrl_out + 3 * i, O.ctrl());
+            put(g + gl.chord_out + 3 * i, m.output_body ? m.chord_local[i] : m.chord_global[i]);
+            g[gl.is_virtual + i] = m.is_virtual[i] ? 1.0 : 0.0;
+        }
+        for (int j = 0; j < P; ++j) {
+            put(g + gl.span_pts + 3 * j, m.span_points[j]);
+            put(g + gl.chord_span + 3 * j, m.chord_global_at_span[j]);
+        }
+        put(g + gl.rbm, m.rbm.translation);
+        put(g + gl.rbm + 3, m.rbm.rotation);
+        put(g + gl.rbm + 6, m.rbm.velocity_linear);
+        put(g + gl.rbm + 9, m.rbm.velocity_angular);
+        put(g + gl.body_axes, m.rbm.rotate(v3(1, 0, 0)));
+        put(g + gl.body_axes + 3, m.rbm.rotate(v3(0, 1, 0)));
+        put(g + gl.body_axes + 6, m.rbm.rotate(v3(0, 0, 1)));
+        check(hipMemcpyAsync(d_geo, g, sizeof(double) * gl.total, hipMemcpyHostToDevice, stream), "upload geometry");
+    }
+
+    void create(const char* json, int dev, int md) {
+        setup = parse_setup(json);
+        device = dev;
+        mode = md;
+        check(hipSetDevice(device), "hipSetDevice");
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        const LineModel& m = setup.lfm;
+        N = m.n();
+        W = m.nw();
+        P = (int)m.span_points.size();
+        M = setup.wake.M;
+        if (N > 1024) throw std::runtime_error("more than 1024 strips per simulation is not supported");
+        nr_points = (long)P * (M + 1);
+        nr_panels = (long)N * M;
+        layout();
+        d_pts[0] = dalloc<double>(3 * nr_points);
+        d_pts[1] = dalloc<double>(3 * nr_points);
+        d_vel = dalloc<double>(3 * nr_points);
+        d_str[0] = dalloc<double>(nr_panels);
+        d_str[1] = dalloc<double>(nr_panels);
+        d_core = dalloc<double>(nr_panels);
+        d_rec = dalloc<PanelRec>(nr_panels);
+        d_geo = dalloc<double>(gl.total);
+        d_fs = dalloc<double>(3 * (N + nr_points));
+        d_felt = dalloc<double>(3 * N);
+        d_felt_span = dalloc<double>(3 * P);
+        d_fixed = dalloc<double>(3 * N);
+        d_F = dalloc<double>(3L * N * N);
+        d_prev = dalloc<double>(N);
+        d_hist = dalloc<double>(6 * N);
+        d_scal = dalloc<double>(8);
+        d_out = dalloc<double>(OutLayout::total(N, W));
+        d_col = dalloc<int>(N);
+        d_wing = dalloc<int>(N);
+        d_span_src = dalloc<int>(2 * P);
+        d_foil = dalloc<FoilParams>(W);
+        std::vector<int> col(N), span_src(2 * P);
+        for (int i = 0; i < N; ++i) col[i] = i + m.wing_of[i];
+        int j = 0;
+        for (int w = 0; w < W; ++w) {  // value_mapping.rs:87-137 without extrapolation
+            span_src[2 * j] = span_src[2 * j + 1] = m.wing_start[w];
+            ++j;
+            for (int i = m.wing_start[w]; i < m.wing_end[w]; ++i, ++j) {
+                span_src[2 * j] = i;
+                span_src[2 * j + 1] = (i == m.wing_end[w] - 1) ? i : i + 1;
+            }
+        }
+        check(hipMemcpyAsync(d_col, col.data(), sizeof(int) * N, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_wing, m.wing_of.data(), sizeof(int) * N, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_span_src, span_src.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_foil, m.foils.data(), sizeof(FoilParams) * W, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_core, setup.wake.core.data(), sizeof(double) * nr_panels, hipMemcpyHostToDevice, stream),
+              "upload");
+        check(hipMemsetAsync(d_prev, 0, sizeof(double) * N, stream), "memset");
+        check(hipMemsetAsync(d_hist, 0, sizeof(double) * 6 * N, stream), "memset");
+        check(hipMemsetAsync(d_scal, 0, sizeof(double) * 8, stream), "memset");
+        check(hipMemsetAsync(d_str[1], 0, sizeof(double) * nr_panels, stream), "memset");
+        pack_geometry();
+        // DynamicWakeBuilder::build -> initialize_based_on_chord_length (initialization.rs:46-71)
+        V3 sum = v3(0, 0, 0);
+        for (const V3& c : m.chord_global_at_span) sum = sum + c;
+        V3 avg = sum / (double)m.chord_global_at_span.size();
+        double wl = setup.wake.initial_relative_wake_length * len(avg);
+        V3 vel = (wl / (double)M) * avg;
+        double v[3] = {vel.x, vel.y, vel.z};
+        Inst d = inst();
+        check(launch_init_wake(d, 1.0, 0, v, stream), "init wake");
+        check(launch_panels(d, d.pts, 0, stream), "panels");
+        h_out.assign(OutLayout::total(N, W), 0.0);
+        check(hipStreamSynchronize(stream), "create");
+    }
+
+    void upload_freestream(const double* fs, size_t n, bool on_device) {
+        size_t bytes = sizeof(double) * 3 * n;
+        check(hipMemcpyAsync(d_fs, fs, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
+              "upload freestream");
+    }
+
+    void step(double time, double dt, const double* fs, size_t n, bool fs_on_device, sb_step_result* out) {
+        check(hipSetDevice(device), "hipSetDevice");
+        if ((long)n != N + nr_points) throw std::invalid_argument("freestream length mismatch");
+        setup.lfm.update_global();
+        pack_geometry();
+        upload_freestream(fs, n, fs_on_device);
+        Inst d = inst();
+        bool first = !first_done;
+        check(hipEventRecord(ev[0], stream), "event");
+        check(launch_prepare(d, first ? 1 : 0, stream), "prepare");
+        if (first) {
+            check(launch_init_wake(d, dt, 1, nullptr, stream), "init wake");
+            first_done = true;
+            nz_rows = 0;
+        }
+        // update_before_solving (update_data.rs:28-42)
+        check(launch_wake_step(d, dt, stream), "wake step");
+        cur ^= 1;
+        d = inst();
+        check(launch_panels(d, d.pts, 1, stream), "panels");
+        scur ^= 1;
+        nz_rows = nz_rows == 0 ? 0 : (nz_rows + 1 < M ? nz_rows + 1 : M);
+        d = inst();
+        check(hipEventRecord(ev[1], stream), "event");
+        // FrozenWake::update_from_full_wake (frozen_wake.rs:189-261)
+        check(launch_fixed(d, mode, stream), "fixed velocities");
+        check(hipEventRecord(ev[2], stream), "event");
+        check(launch_factors(d, stream), "factors");
+        check(hipEventRecord(ev[3], stream), "event");
+        // SimpleIterative::solve, then update_wing_strength
+        check(launch_solve(d, stream), "solve");
+        nz_rows = nz_rows < 1 ? 1 : nz_rows;
+        d = inst();
+        check(hipEventRecord(ev[4], stream), "event");
+        // update_velocity_at_points (update_data.rs:125-141)
+        check(launch_velocity_copy(d, stream), "velocity copy");
+        long end = d.end_index < nr_points ? d.end_index : nr_points;
+        pairs[0] = 0;
+        if (end > 0 && steps_completed > 2) {
+            check(launch_self_induction(d, mode, stream), "self induction");
+            pairs[0] = (double)end * (double)N * (double)nz_rows;
+        }
+        steps_completed += 1;
+        pairs[1] = (double)N * (double)N * (double)(nz_rows > 1 ? nz_rows - 1 : 0);
+        pairs[2] = (double)N * (double)N;
+        check(hipEventRecord(ev[5], stream), "event");
+        check(launch_forces(d, dt, stream), "forces");
+        check(hipEventRecord(ev[6], stream), "event");
+        check(hipMemcpyAsync(h_out.data(), d_out, sizeof(double) * h_out.size(), hipMemcpyDeviceToHost, stream),
+              "download");
+        check(hipStreamSynchronize(stream), "step");
+        static const int order[kTimers][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
+        for (int k = 0; k < kTimers; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[order[k][0]], ev[order[k][1]]);
+            timer_ms[k] = ms;
+        }
+        write_result(time, out);
+    }
+
+    void write_result(double time, sb_step_result* o) const {
+        if (!o) return;
+        const double* h = h_out.data();
+        o->time = time;
+        o->iterations = (long)h[OutLayout::iters(N, W)];
+        o->residual = h[OutLayout::residual(N, W)];
+        auto cp = [](double* dst, const double* src, size_t n) {
+            if (dst) memcpy(dst, src, sizeof(double) * n);
+        };
+        cp(o->circulation_strength, h + OutLayout::gamma(N), N);
+        cp(o->angles_of_attack, h + OutLayout::alpha(N), N);
+        cp(o->ctrl_points_velocity, h + OutLayout::u(N), 3 * N);
+        cp(o->solver_input_velocity, h + OutLayout::felt(N), 3 * N);
+        cp(o->ctrl_points, h + OutLayout::ctrl(N), 3 * N);
+        cp(o->sectional_forces, h + OutLayout::sect(N), 18 * N);
+        cp(o->integrated_forces, h + OutLayout::integ_f(N), 18 * W);
+        cp(o->integrated_moments, h + OutLayout::integ_m(N, W), 18 * W);
+    }
+
+    double* tmp(size_t n) {
+        if (n > tmp_cap) {
+            if (d_tmp) (void)hipFree(d_tmp);
+            d_tmp = dalloc<double>(n);
+            tmp_cap = n;
+        }
+        return d_tmp;
+    }
+
+    void induced(const double* pts, size_t n, double* out) {
+        check(hipSetDevice(device), "hipSetDevice");
+        if (n == 0) return;
+        double* buf = tmp(6 * n);
+        check(hipMemcpyAsync(buf, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream), "upload points");
+        Inst d = inst();
+        check(launch_induced_targets(d, buf, (long)n, (long)N * nz_rows, mode, buf + 3 * n, stream), "induced");
+        check(hipMemcpyAsync(out, buf + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, stream), "download");
+        check(hipStreamSynchronize(stream), "induced");
+    }
+};
+
+// ----------------------------------------------------------------------------
+// C ABI
+// ----------------------------------------------------------------------------
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const HipError& e) {
+        g_error = e.what();
+        return -5;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return -3;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -4;
+    }
+}
+
+extern "C" {
+
+const char* sb_last_error(void) { return g_error.c_str(); }
+
+sb_sim* sb_sim_create_ex(const char* json, int device, int mode) {
+    if (!json) {
+        g_error = "null setup string";
+        return nullptr;
+    }
+    if (mode != MODE_EXACT && mode != MODE_FAST) {
+        g_error = "unsupported mode";
+        return nullptr;
+    }
+    auto sim = std::make_unique<sb_sim>();
+    try {
+        sim->create(json, device, mode);
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return nullptr;
+    }
+    return sim.release();
+}
+
+sb_sim* sb_sim_create(const char* json) { return sb_sim_create_ex(json, 0, MODE_EXACT); }
+
+void sb_sim_destroy(sb_sim* sim) { delete sim; }
+
+long sb_sim_nr_ctrl_points(const sb_sim* s) { return s ? s->N : -1; }
+long sb_sim_nr_wings(const sb_sim* s) { return s ? s->W : -1; }
+long sb_sim_nr_span_points(const sb_sim* s) { return s ? s->P : -1; }
+long sb_sim_nr_wake_points(const sb_sim* s) { return s ? s->nr_points : -1; }
+long sb_sim_nr_panels(const sb_sim* s) { return s ? s->nr_panels : -1; }
+long sb_sim_nr_freestream_points(const sb_sim* s) { return s ? s->N + s->nr_points : -1; }
+
+long sb_sim_get_freestream_points(const sb_sim* s, double* xyz, size_t cap) {
+    if (!s || !xyz) return -1;
+    if (cap == 0) return -2;
+    size_t total = (size_t)(s->N + s->nr_points);
+    size_t n = cap < total ? cap : total;
+    int rc = guarded([&] {
+        std::vector<double> tmp(3 * total);
+        const LineModel& m = s->setup.lfm;
+        for (int i = 0; i < s->N; ++i) {
+            tmp[3 * i] = m.ctrl_points[i].x;
+            tmp[3 * i + 1] = m.ctrl_points[i].y;
+            tmp[3 * i + 2] = m.ctrl_points[i].z;
+        }
+        check(hipSetDevice(s->device), "hipSetDevice");
+        check(hipMemcpy(tmp.data() + 3 * s->N, s->d_pts[s->cur], sizeof(double) * 3 * s->nr_points,
+                        hipMemcpyDeviceToHost),
+              "download points");
+        memcpy(xyz, tmp.data(), sizeof(double) * 3 * n);
+    });
+    return rc != 0 ? rc : (long)n;
+}
+
+int sb_sim_do_step(sb_sim* s, double time, double dt, const double* fs, size_t n, sb_step_result* out) {
+    if (!s || !fs) return -1;
+    return guarded([&] { s->step(time, dt, fs, n, false, out); });
+}
+
+int sb_sim_do_step_device(sb_sim* s, double time, double dt, const double* fs, size_t n, sb_step_result* out) {
+    if (!s || !fs) return -1;
+    return guarded([&] { s->step(time, dt, fs, n, true, out); });
+}
+
+int sb_sim_induced_velocities(sb_sim* s, const double* pts, size_t n, double* out) {
+    if (!s || ((!pts || !out) && n)) return -1;
+    return guarded([&] { s->induced(pts, n, out); });
+}
+
+int sb_sim_get_previous_circulation(const sb_sim* s, double* out, size_t n) {
+    if (!s || !out) return -1;
+    if ((long)n != s->N) return -3;
+    return guarded([&] {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        check(hipMemcpy(out, s->d_prev, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
+    });
+}
+
+int sb_sim_set_previous_circulation(sb_sim* s, const double* g, size_t n) {
+    if (!s || !g) return -1;
+    if ((long)n != s->N) return -3;
+    return guarded([&] {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        check(hipMemcpy(s->d_prev, g, sizeof(double) * n, hipMemcpyHostToDevice), "upload");
+    });
+}
+
+int sb_sim_set_translation_and_rotation_fd(sb_sim* s, double dt, const double* t, const double* r) {
+    if (!s || !t || !r) return -1;
+    s->setup.lfm.rbm.update_translation_fd(v3(t[0], t[1], t[2]), dt);
+    s->setup.lfm.rbm.update_rotation_fd(v3(r[0], r[1], r[2]), dt);
+    s->setup.lfm.update_global();
+    return 0;
+}
+
+int sb_sim_set_local_wing_angles(sb_sim* s, const double* a, size_t n) {
+    if (!s || !a) return -1;
+    if ((long)n != s->W) return -3;
+    for (size_t i = 0; i < n; ++i) s->setup.lfm.wing_angles[i] = a[i];
+    s->setup.lfm.update_global();
+    return 0;
+}
+
+int sb_sim_get_wake_state(const sb_sim* s, double* pts, double* vel, double* str, long* steps, int* first) {
+    if (!s) return -1;
+    int rc = guarded([&] {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        if (pts)
+            check(hipMemcpy(pts, s->d_pts[s->cur], sizeof(double) * 3 * s->nr_points, hipMemcpyDeviceToHost),
+                  "download");
+        if (vel) check(hipMemcpy(vel, s->d_vel, sizeof(double) * 3 * s->nr_points, hipMemcpyDeviceToHost), "download");
+        if (str) check(hipMemcpy(str, s->d_str[s->scur], sizeof(double) * s->nr_panels, hipMemcpyDeviceToHost), "download");
+    });
+    if (steps) *steps = s->steps_completed;
+    if (first) *first = s->first_done ? 1 : 0;
+    return rc;
+}
+
+int sb_sim_set_wake_state(sb_sim* s, const double* pts, const double* vel, const double* str, long steps,
+                          int first) {
+    if (!s) return -1;
+    return guarded([&] {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        if (pts)
+            check(hipMemcpy(s->d_pts[s->cur], pts, sizeof(double) * 3 * s->nr_points, hipMemcpyHostToDevice), "upload");
+        if (vel) check(hipMemcpy(s->d_vel, vel, sizeof(double) * 3 * s->nr_points, hipMemcpyHostToDevice), "upload");
+        if (str) {
+            check(hipMemcpy(s->d_str[s->scur], str, sizeof(double) * s->nr_panels, hipMemcpyHostToDevice), "upload");
+            int nz = 0;
+            for (long k = 0; k < s->nr_panels; ++k)
+                if (str[k] != 0.0) nz = (int)(k / s->N) + 1;
+            s->nz_rows = nz;
+        }
+        s->steps_completed = steps;
+        if (first && !s->first_done) {
+            s->first_done = true;
+            check(hipMemset(s->d_prev, 0, sizeof(double) * s->N), "memset");
+        }
+        Inst d = s->inst();
+        check(launch_panels(d, d.pts, 0, s->stream), "panels");
+        check(hipStreamSynchronize(s->stream), "sync");
+    });
+}
+
+int sb_sim_get_panels(const sb_sim* s, double* rec) {
+    if (!s || !rec) return -1;
+    return guarded([&] {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        std::vector<PanelRec> r(s->nr_panels);
+        std::vector<double> p(3 * s->nr_points);
+        std::vector<double> core(s->nr_panels);
+        check(hipMemcpy(r.data(), s->d_rec, sizeof(PanelRec) * s->nr_panels, hipMemcpyDeviceToHost), "download");
+        check(hipMemcpy(p.data(), s->d_pts[s->cur], sizeof(double) * p.size(), hipMemcpyDeviceToHost), "download");
+        check(hipMemcpy(core.data(), s->d_core, sizeof(double) * s->nr_panels, hipMemcpyDeviceToHost), "download");
+        const LineModel& m = s->setup.lfm;
+        for (long k = 0; k < s->nr_panels; ++k) {
+            long st = k / s->N, i = k % s->N;
+            long a = st * s->P + i + m.wing_of[i], b = a + s->P;
+            long idx[4] = {a, a + 1, b + 1, b};
+            double* o = rec + 22 * k;
+            for (int q = 0; q < 4; ++q)
+                for (int c = 0; c < 3; ++c) o[3 * q + c] = p[3 * idx[q] + c];
+            o[12] = r[k].cx;
+            o[13] = r[k].cy;
+            o[14] = r[k].cz;
+            o[15] = r[k].nx;
+            o[16] = r[k].ny;
+            o[17] = r[k].nz;
+            o[18] = r[k].far2;
+            o[19] = core[k];
+            o[20] = r[k].area_term;
+            o[21] = r[k].strength;
+        }
+    });
+}
+
+int sb_sim_get_frozen_wake(const sb_sim* s, double* fixed, double* factors) {
+    if (!s) return -1;
+    return guarded([&] {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        if (fixed) check(hipMemcpy(fixed, s->d_fixed, sizeof(double) * 3 * s->N, hipMemcpyDeviceToHost), "download");
+        if (factors) {
+            long N = s->N;
+            std::vector<double> F(3 * N * N);
+            check(hipMemcpy(F.data(), s->d_F, sizeof(double) * F.size(), hipMemcpyDeviceToHost), "download");
+            for (long j = 0; j < N; ++j)
+                for (long i = 0; i < N; ++i)
+                    for (int c = 0; c < 3; ++c) factors[3 * (i * N + j) + c] = F[3 * (j * N + i) + c];
+        }
+    });
+}
+
+double sb_sim_last_kernel_ms(const sb_sim* s, const char* name) {
+    if (!s || !name) return -1.0;
+    for (int k = 0; k < kTimers; ++k)
+        if (strcmp(name, kTimerNames[k]) == 0) return s->timer_ms[k];
+    return -2.0;
+}
+
+int sb_sim_last_pair_counts(const sb_sim* s, double* c) {
+    if (!s || !c) return -1;
+    for (int k = 0; k < 3; ++k) c[k] = s->pairs[k];
+    return 0;
+}
+
+int sb_wake_induced_velocities(const double* recs, size_t n_panels, const double* targets, size_t n_targets,
+                               int symmetry, int mode, double* out) {
+    if ((!recs && n_panels) || (!targets && n_targets) || (!out && n_targets)) return -1;
+    if (n_targets == 0) return 0;
+    if (symmetry < 0 || symmetry > 3 || (mode != MODE_EXACT && mode != MODE_FAST)) return -4;
+    if (n_panels > (size_t)0x7fffffff) return -4;
+    PanelRec* d_rec = nullptr;
+    double *d_corners = nullptr, *d_core = nullptr, *d_t = nullptr;
+    hipStream_t st = nullptr;
+    int rc = 0;
+    try {
+        std::vector<PanelRec> r(n_panels);
+        std::vector<double> corners(12 * n_panels), core(n_panels);
+        for (size_t k = 0; k < n_panels; ++k) {
+            const double* q = recs + 22 * k;
+            memcpy(&corners[12 * k], q, sizeof(double) * 12);
+            r[k] = PanelRec{q[12], q[13], q[14], q[15], q[16], q[17], q[18], q[20], q[21]};
+            core[k] = q[19];
+        }
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
+        d_rec = dalloc<PanelRec>(n_panels);
+        d_corners = dalloc<double>(12 * n_panels);
+        d_core = dalloc<double>(n_panels);
+        d_t = dalloc<double>(6 * n_targets);
+        check(hipMemcpyAsync(d_rec, r.data(), sizeof(PanelRec) * n_panels, hipMemcpyHostToDevice, st), "upload");
+        check(hipMemcpyAsync(d_corners, corners.data(), sizeof(double) * corners.size(), hipMemcpyHostToDevice, st),
+              "upload");
+        check(hipMemcpyAsync(d_core, core.data(), sizeof(double) * n_panels, hipMemcpyHostToDevice, st), "upload");
+        check(hipMemcpyAsync(d_t, targets, sizeof(double) * 3 * n_targets, hipMemcpyHostToDevice, st), "upload");
+        Inst d{};
+        d.N = (int)(n_panels ? n_panels : 1);
+        d.M = 1;
+        d.P = d.N + 1;
+        d.symmetry = symmetry;
+        d.rec = d_rec;
+        d.corners = d_corners;
+        d.core = d_core;
+        check(launch_induced_targets(d, d_t, (long)n_targets, (long)n_panels, mode, d_t + 3 * n_targets, st),
+              "induced");
+        check(hipMemcpyAsync(out, d_t + 3 * n_targets, sizeof(double) * 3 * n_targets, hipMemcpyDeviceToHost, st),
+              "download");
+        check(hipStreamSynchronize(st), "sync");
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        rc = -5;
+    }
+    if (d_rec) (void)hipFree(d_rec);
+    if (d_corners) (void)hipFree(d_corners);
+    if (d_core) (void)hipFree(d_core);
+    if (d_t) (void)hipFree(d_t);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
+
+int sb_debug_libm_device(int fn, const double* x, const double* y, double* out, size_t n) {
+    if (!x || !out || (fn == 4 && !y)) return -1;
+    if (n == 0) return 0;
+    double* d = nullptr;
+    int rc = 0;
+    try {
+        d = dalloc<double>(3 * n);
+        check(hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice), "upload");
+        if (y) check(hipMemcpy(d + n, y, sizeof(double) * n, hipMemcpyHostToDevice), "upload");
+        check(launch_libm(fn, d, d + n, d + 2 * n, (long)n, nullptr), "libm");
+        check(hipMemcpy(out, d + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        rc = -5;
+    }
+    if (d) (void)hipFree(d);
+    return rc;
+}
+
+int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n) {
+    if (!x || !out || (fn == 4 && !y)) return -1;
+    for (size_t i = 0; i < n; ++i) {
+        switch (fn) {
+            case 0: out[i] = sbm::sin(x[i]); break;
+            case 1: out[i] = sbm::cos(x[i]); break;
+            case 2: out[i] = sbm::exp(x[i]); break;
+            case 3: out[i] = sbm::acos(x[i]); break;
+            case 4: out[i] = sbm::pow(x[i], y[i]); break;
+            default: out[i] = sbm::log(x[i]); break;
+        }
+    }
+    return 0;
+}
+
+}  // extern "C"

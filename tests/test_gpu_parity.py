@@ -1,0 +1,301 @@
+"""GPU parity: libstormbird_amd (HIP, gfx950) against the CPU f64 oracle.
+
+Bar: EXACT mode equals the oracle built with the portable libm bit for bit,
+over whole trajectories (free wakes included); against the oracle built with
+the host libm (the Rust build's elementary functions) it stays within 1e-10
+relative on frozen-shape wakes and on single steps from one state.  FAST mode
+is held to 1e-10 relative per step from one state.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import stormbird_amd as sb
+from stormbird_amd import setups as S
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("circulation_strength", "angles_of_attack", "velocity", "sectional_forces", "integrated_forces",
+          "integrated_moments")
+
+
+def same(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def rel_dev(a, b):
+    a, b = np.asarray(a, dtype=float), np.asarray(b, dtype=float)
+    scale = max(np.max(np.abs(b)), 1e-300)
+    return float(np.max(np.abs(a - b)) / scale) if a.size else 0.0
+
+
+def compare_results(rg, ro, exact=True, tol=1e-10, where=""):
+    ag, ao = rg.as_arrays(), ro.as_arrays()
+    assert rg.iterations == ro.iterations, f"{where}: iterations {rg.iterations} != {ro.iterations}"
+    for f in FIELDS:
+        if exact:
+            assert same(ag[f], ao[f]), f"{where}: {f} differs, rel {rel_dev(ag[f], ao[f]):.3e}"
+        else:
+            assert rel_dev(ag[f], ao[f]) <= tol, f"{where}: {f} rel dev {rel_dev(ag[f], ao[f]):.3e}"
+    if exact:
+        assert rg.residual == ro.residual, where
+
+
+def step_both(gpu, orc, fs, steps, dt, exact=True, tol=1e-10, t0=0, wake_check_every=0):
+    rg = ro = None
+    for k in range(t0, t0 + steps):
+        rg = gpu.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+        ro = orc.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+        compare_results(rg, ro, exact=exact, tol=tol, where=f"step {k}")
+        if wake_check_every and (k % wake_check_every == 0):
+            wg, wo = gpu.get_wake_state(), orc.get_wake_state()
+            for key in ("points", "velocities", "strengths"):
+                if exact:
+                    assert same(wg[key], wo[key]), f"step {k}: wake {key} differs"
+                else:
+                    assert rel_dev(wg[key], wo[key]) <= tol
+    return rg, ro
+
+
+def test_libm_device_matches_host_bitwise(oracle):
+    rng = np.random.default_rng(7)
+    n = 200_000
+    cases = [(0, rng.uniform(-8, 8, n), None), (1, rng.uniform(-8, 8, n), None), (2, rng.uniform(-60, 60, n), None),
+             (3, rng.uniform(-1, 1, n), None), (4, rng.uniform(0, 1, n), rng.uniform(0.2, 3, n)),
+             (5, np.exp(rng.uniform(-30, 30, n)), None)]
+    for fn, x, y in cases:
+        d = sb.libm_device(fn, x, y)
+        h = sb.libm_host(fn, x, y)
+        o = oracle.libm_eval(fn, x, y, variant="pm")
+        assert same(d, h), f"fn {fn}: device and host differ"
+        assert same(h, o), f"fn {fn}: library host and oracle_pm differ"
+
+
+def test_standalone_biot_savart_bitwise(oracle):
+    # a developed free wake from the oracle: panel records and targets around it
+    js, case = S.four_sail_free_wake(nr_panels=40, nr_strips=6)
+    orc = oracle.Simulation(js, "pm")
+    fs = S.uniform_freestream(orc.nr_freestream_points(), case["velocity"])
+    for k in range(12):
+        orc.do_step(time=k * case["time_step"], time_step=case["time_step"], freestream_velocity=fs)
+    rec = orc.get_panels()
+    pts = orc.get_wake_state()["points"]
+    rng = np.random.default_rng(3)
+    targets = np.concatenate([pts, pts[:500] + rng.normal(0, 0.3, (500, 3))])
+    ref = orc.induced_velocities(targets)
+    for n_t in (37, targets.shape[0]):  # fold-per-target path and lane-per-target path
+        out = sb.wake_induced_velocities(rec, targets[:n_t], symmetry=0, mode=sb.MODE_EXACT)
+        assert same(out, ref[:n_t]), f"n={n_t}: rel {rel_dev(out, ref[:n_t]):.3e}"
+        fast = sb.wake_induced_velocities(rec, targets[:n_t], symmetry=0, mode=sb.MODE_FAST)
+        assert rel_dev(fast, ref[:n_t]) < 1e-12
+
+
+def test_single_wing_sail_200_steps_bitwise(oracle):
+    """Frozen-shape wake, 20 cosine strips, 200 rows (4,000 panels), 200 steps."""
+    js = S.single_wing_sail()
+    case = S.single_wing_sail_case()
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "pm")
+    assert same(gpu.get_freestream_velocity_points(), orc.get_freestream_velocity_points())
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    step_both(gpu, orc, fs, case["steps"], case["time_step"], wake_check_every=50)
+
+
+def test_single_wing_sail_vs_host_libm_oracle(oracle):
+    """Against the oracle with the host libm (the Rust build's sin/exp/acos).
+
+    The two differ by <= 1 ulp per elementary-function call.  Whenever the
+    solver exits after one undamped iteration (residual already below 1e-4),
+    the undamped map amplifies a spanwise sawtooth of Gamma about 4x per step,
+    so these ulp differences grow beyond 1e-10 after ~50 steps; that is a
+    property of the reference algorithm (it makes the Rust build disagree with
+    itself across libms), not of the port.  Checked here: the first 40 steps
+    of the trajectory, and one step from the GPU's own state at five points of
+    the 200-step run, each within 1e-10 relative.
+    """
+    js = S.single_wing_sail()
+    case = S.single_wing_sail_case()
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "glibc")
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    dt = case["time_step"]
+    worst_traj = worst_step = 0.0
+    for k in range(case["steps"]):
+        if k in (60, 100, 140, 180, 199):
+            w = gpu.get_wake_state()
+            sync = oracle.Simulation(js, "glibc")
+            sync.set_wake_state(w["points"], w["velocities"], w["strengths"], w["steps_completed"])
+            sync.set_previous_circulation(gpu.previous_circulation_strength)
+            ro1 = sync.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+        rg = gpu.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+        if k < 40:
+            ro = orc.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+            for f in FIELDS:
+                worst_traj = max(worst_traj, rel_dev(rg.as_arrays()[f], ro.as_arrays()[f]))
+        if k in (60, 100, 140, 180, 199):
+            for f in ("circulation_strength", "angles_of_attack", "velocity", "integrated_forces"):
+                worst_step = max(worst_step, rel_dev(rg.as_arrays()[f], ro1.as_arrays()[f]))
+    print(f"host-libm oracle: trajectory (40 steps) {worst_traj:.3e}, single steps {worst_step:.3e}")
+    assert worst_traj <= 1e-10
+    assert worst_step <= 1e-10
+
+
+def test_free_wake_trajectory_bitwise(oracle):
+    """Four sails, free wake with self-induction on every wake point, 70 steps."""
+    js, case = S.four_sail_free_wake(nr_panels=60, nr_strips=8)
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "pm")
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    step_both(gpu, orc, fs, 70, case["time_step"], wake_check_every=10)
+
+
+@pytest.mark.parametrize("variant", ["symmetry_z", "neglect_self", "damping", "first_dir_actual", "max_ratio",
+                                     "fixed_magnitude", "no_core", "sin_core", "body_output"])
+def test_setting_variants_bitwise(oracle, variant):
+    extra, solver, lfm_extra = {}, None, {}
+    if variant == "symmetry_z":
+        extra = {"symmetry_condition": "Z"}
+    elif variant == "neglect_self":
+        extra = {"neglect_self_induced_velocities": True}
+    elif variant == "damping":
+        extra = {"shape_damping_factor": 0.3}
+    elif variant == "first_dir_actual":
+        extra = {"first_wake_points_direction": "ActualVelocity"}
+    elif variant == "max_ratio":
+        solver = {"velocity_corrections": {"MaxInducedVelocityMagnitudeRatio": 0.05}}
+    elif variant == "fixed_magnitude":
+        solver = {"velocity_corrections": "FixedMagnitudeEqualToFreestream"}
+    elif variant == "no_core":
+        extra = {"viscous_core_length": "NoViscousCore"}
+    elif variant == "sin_core":
+        extra = {"viscous_core_length_evolution": {"SinIncrease": {"last_panel_value": {"Absolute": 2.0}}}}
+    elif variant == "body_output":
+        lfm_extra = {"output_coordinate_system": "Body", "rotation": {"x": 0.1, "y": 0.0, "z": 0.3}}
+    import json
+    js, case = S.four_sail_free_wake(nr_panels=30, nr_strips=6)
+    d = json.loads(js)
+    d["simulation_settings"]["Dynamic"]["wake"].update(extra)
+    if solver:
+        d["simulation_settings"]["Dynamic"]["solver"] = {"SimpleIterative": solver}
+    d["line_force_model"].update(lfm_extra)
+    js = json.dumps(d)
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "pm")
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    step_both(gpu, orc, fs, 25, case["time_step"], wake_check_every=8)
+
+
+def test_moving_wing_bitwise(oracle):
+    """Rigid-body roll and heave through the finite-difference velocity update."""
+    js, case = S.four_sail_free_wake(nr_panels=30, nr_strips=6)
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "pm")
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    dt = case["time_step"]
+    for k in range(25):
+        t = k * dt
+        tr = [0.0, 0.0, 0.3 * math.sin(0.4 * t)]
+        rot = [10.0 * math.pi / 180 * math.sin(2 * math.pi * t / 12.0), 0.0, 0.0]
+        for s in (gpu, orc):
+            s.set_translation_and_rotation_with_finite_difference_for_the_velocity(dt, tr, rot)
+        rg = gpu.do_step(time=t, time_step=dt, freestream_velocity=fs)
+        ro = orc.do_step(time=t, time_step=dt, freestream_velocity=fs)
+        compare_results(rg, ro, where=f"step {k}")
+
+
+def test_reference_steady_lift_dynamic_on_gpu():
+    """compare_models.rs:28-168, dynamic part: AR 5, cl0 1.2, 2 deg, 64 strips, 200 steps."""
+    aoa = 2.0 * math.pi / 180
+    js = S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=1.2, angle_of_attack=aoa, nr_strips=64)
+    sim = sb.Simulation(js)
+    vel = [1.2, 0.0, 0.0]
+    dt = 0.25 / 1.2
+    fs = S.uniform_freestream(sim.nr_freestream_points(), vel)
+    for k in range(200):
+        r = sim.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+    ff = 0.5 * 13.2 * 1.2 ** 2 * 5.0
+    cl = r.integrated_forces_sum()[1] / ff
+    cd = r.integrated_forces_sum()[0] / ff
+    cl_t = (1.2 + 2 * math.pi * aoa) / (1 + 2 / 5.0)
+    cd_t = cl_t ** 2 / (math.pi * 5.0)
+    assert abs(cl - cl_t) / cl_t < 0.07
+    assert abs(cd - cd_t) / cd_t < 0.06
+
+
+def test_reference_forward_motion_equals_inflow_on_gpu():
+    """dynamic_simulation.rs:23-114: a wing translating at -U in still air == fixed wing in U."""
+    js = S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=0.5, negative_span_orientation=True)
+    a, b = sb.Simulation(js), sb.Simulation(js)
+    u = 5.2
+    n = b.nr_freestream_points()
+    fs_in = S.uniform_freestream(n, [u, 0, 0])
+    fs_0 = np.zeros((n, 3))
+    ff = 0.5 * 5.0 * u ** 2 * 13.2
+    dt = 0.1
+    for i in range(1, 20):
+        t = i * dt
+        a.set_translation_and_rotation_with_finite_difference_for_the_velocity(dt, [-u * t, 0, 0], [0, 0, 0])
+        ra = a.do_step(time=t, time_step=dt, freestream_velocity=fs_0)
+        rb = b.do_step(time=t, time_step=dt, freestream_velocity=fs_in)
+        cla = ra.integrated_forces_sum()[1] / ff
+        clb = rb.integrated_forces_sum()[1] / ff
+        assert abs(cla - clb) / abs(clb) < 1e-5
+
+
+def test_developed_wake_one_step_fast_and_exact(oracle):
+    """One step from a synthesized developed free-wake state: EXACT bitwise, FAST <= 1e-10."""
+    js, case = S.four_sail_free_wake(nr_panels=400, nr_strips=20)
+    orc = oracle.Simulation(js, "pm")
+    st = orc.get_wake_state()
+    rng = np.random.default_rng(11)
+    u = np.asarray(case["velocity"])
+    dt = case["time_step"]
+    P = orc.nr_span_points
+    M = orc.nr_panels // orc.nr_ctrl_points
+    pts = st["points"].reshape(M + 1, P, 3).copy()
+    for s in range(1, M + 1):
+        pts[s] = pts[0] + u * dt * s + rng.normal(0, 0.02 * math.sqrt(s), (P, 3))
+    pts = pts.reshape(-1, 3)
+    gam = -(2.0 + rng.normal(0, 0.1, (M, orc.nr_ctrl_points))).ravel()
+    vel = np.tile(u, (pts.shape[0], 1))
+    for s in (orc,):
+        s.set_wake_state(pts, vel, gam, 50)
+    ex = sb.Simulation(js, mode=sb.MODE_EXACT)
+    fa = sb.Simulation(js, mode=sb.MODE_FAST)
+    for s in (ex, fa):
+        s.set_wake_state(pts, vel, gam, 50)
+    fs = S.uniform_freestream(orc.nr_freestream_points(), u)
+    ro = orc.do_step(time=50 * dt, time_step=dt, freestream_velocity=fs)
+    re = ex.do_step(time=50 * dt, time_step=dt, freestream_velocity=fs)
+    rf = fa.do_step(time=50 * dt, time_step=dt, freestream_velocity=fs)
+    compare_results(re, ro, where="exact")
+    wo, we, wf = orc.get_wake_state(), ex.get_wake_state(), fa.get_wake_state()
+    assert same(we["velocities"], wo["velocities"])
+    for f in FIELDS:
+        d = rel_dev(rf.as_arrays()[f], ro.as_arrays()[f])
+        assert d <= 1e-10, f"fast {f}: {d:.3e}"
+    assert rel_dev(wf["velocities"] - u, wo["velocities"] - u) <= 1e-10
+
+
+def test_induced_velocities_api_bitwise(oracle):
+    js, case = S.four_sail_free_wake(nr_panels=30, nr_strips=6)
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "pm")
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    step_both(gpu, orc, fs, 8, case["time_step"])
+    rng = np.random.default_rng(5)
+    pts = rng.uniform(-60, 60, (3000, 3))
+    for n in (0, 1, 50, 3000):
+        assert same(gpu.induced_velocities(pts[:n]), orc.induced_velocities(pts[:n]))
+
+
+def test_warm_start_hook_roundtrip():
+    js = S.single_wing_sail(nr_panels=20)
+    sim = sb.Simulation(js)
+    g = np.linspace(-3, -1, sim.nr_ctrl_points)
+    sim.previous_circulation_strength = g
+    assert same(sim.previous_circulation_strength, g)
+    with pytest.raises(RuntimeError):
+        sim.previous_circulation_strength = g[:-1]
