@@ -128,6 +128,11 @@ double sb_sim_last_kernel_ms(const sb_sim* sim, const char* name);
  * [0] self-induction, [1] fixed velocities, [2] unit-strength factor pairs. */
 int sb_sim_last_pair_counts(const sb_sim* sim, double* counts3);
 
+/* Near-field (vortex-ring branch) and total pair counts, with nonzero panel
+ * strength, of the free-wake self-induction sum at the current wake state
+ * (update_velocity_at_points, dynamic_wake/update_data.rs:125-141). */
+int sb_sim_count_self_induction_pairs(sb_sim* sim, double* near_total2);
+
 /* Standalone Biot-Savart sum (kernel-level entry for parity tests): induced
  * velocity at `n_targets` points from `n_panels` panels given as records
  * [n_panels][22] (layout of sb_sim_get_panels), symmetry 0 none, 1 X, 2 Y,

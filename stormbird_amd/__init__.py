@@ -70,6 +70,7 @@ def lib():
         L.sb_sim_last_kernel_ms.restype = ctypes.c_double
         L.sb_sim_last_kernel_ms.argtypes = [vp, ctypes.c_char_p]
         L.sb_sim_last_pair_counts.argtypes = [vp, _dp]
+        L.sb_sim_count_self_induction_pairs.argtypes = [vp, _dp]
         L.sb_wake_induced_velocities.argtypes = [_dp, ctypes.c_size_t, _dp, ctypes.c_size_t, ctypes.c_int,
                                                  ctypes.c_int, _dp]
         L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
@@ -221,6 +222,14 @@ class Simulation:
 
     def last_kernel_ms(self, name: str) -> float:
         return lib().sb_sim_last_kernel_ms(self._h, name.encode())
+
+    def count_self_induction_pairs(self):
+        """(near, total) pairs with nonzero strength of the self-induction sum at the current state."""
+        c = np.zeros(2)
+        rc = lib().sb_sim_count_self_induction_pairs(self._h, _p(c))
+        if rc != 0:
+            _err("count_self_induction_pairs", rc)
+        return float(c[0]), float(c[1])
 
     def last_pair_counts(self) -> np.ndarray:
         c = np.zeros(3)

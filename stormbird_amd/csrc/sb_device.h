@@ -106,6 +106,8 @@ hipError_t launch_flow_update(const Inst& d, hipStream_t st);
 // Induced velocity at arbitrary targets from panels [0, n_panels): out = sum.
 hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_targets, long n_panels,
                                   int mode, double* out, hipStream_t st);
+hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targets, long n_panels,
+                              double* partial, long* n_blocks, hipStream_t st);
 hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st);
 
 }  // namespace sb

@@ -183,6 +183,48 @@ __global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restric
     }
 }
 
+// Pair census for the roofline: per block, the number of (target, panel)
+// pairs with nonzero strength that take the near-field (vortex ring) branch
+// and the total; one partial per block, summed on the host.
+__global__ __launch_bounds__(256) void k_count_pairs(Inst d, const double* __restrict__ targets, long n_targets,
+                                                     long n_panels, double* __restrict__ partial) {
+    __shared__ PanelRec tile[TILE];
+    __shared__ double red[2][256];
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long tc = t < n_targets ? t : n_targets - 1;
+    V3 x = load3(targets + 3 * tc);
+    double near = 0.0, total = 0.0;
+    for (long base = 0; base < n_panels; base += TILE) {
+        int cnt = (int)((n_panels - base) < TILE ? (n_panels - base) : TILE);
+        __syncthreads();
+        const double* src = reinterpret_cast<const double*>(d.rec + base);
+        double* dst = reinterpret_cast<double*>(tile);
+        for (int q = threadIdx.x; q < cnt * 9; q += blockDim.x) dst[q] = src[q];
+        __syncthreads();
+        for (int p = 0; p < cnt; ++p) {
+            PanelRec r = tile[p];
+            if (r.strength == 0.0) continue;
+            double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
+            double d2 = tx * tx + ty * ty + tz * tz;
+            total += 1.0;
+            if (d2 <= r.far2) near += 1.0;
+        }
+    }
+    if (t >= n_targets) near = total = 0.0;
+    red[0][threadIdx.x] = near;
+    red[1][threadIdx.x] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int q = 0; q < (int)blockDim.x; ++q) {
+            a += red[0][q];
+            b += red[1][q];
+        }
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
 // One wave per target, lanes over panels: each lane evaluates one panel's
 // term, then the wave folds the 64 terms in ascending panel order.  Used when
 // there are few targets (the N control points): it spreads one target's work
@@ -790,6 +832,15 @@ hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_t
             hipLaunchKernelGGL(k_targets<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, targets, n_targets, n_panels,
                                out, 0);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targets, long n_panels,
+                              double* partial, long* n_blocks, hipStream_t st) {
+    *n_blocks = (long)blocks_for(n_targets, 256);
+    if (n_targets <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)*n_blocks), dim3(256), 0, st, d, targets, n_targets, n_panels,
+                       partial);
     return hipGetLastError();
 }
 

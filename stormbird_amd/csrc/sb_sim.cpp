@@ -314,9 +314,9 @@ struct sb_sim {
         check(launch_solve(d, stream), "solve");
         nz_rows = nz_rows < 1 ? 1 : nz_rows;
         d = inst();
-        check(hipEventRecord(ev[4], stream), "event");
         // update_velocity_at_points (update_data.rs:125-141)
         check(launch_velocity_copy(d, stream), "velocity copy");
+        check(hipEventRecord(ev[4], stream), "event");
         long end = d.end_index < nr_points ? d.end_index : nr_points;
         pairs[0] = 0;
         if (end > 0 && steps_completed > 2) {
@@ -358,6 +358,28 @@ struct sb_sim {
         cp(o->sectional_forces, h + OutLayout::sect(N), 18 * N);
         cp(o->integrated_forces, h + OutLayout::integ_f(N), 18 * W);
         cp(o->integrated_moments, h + OutLayout::integ_m(N, W), 18 * W);
+    }
+
+    // Near-field and total pair counts of the free-wake self-induction at the
+    // current state (the sum update_velocity_at_points would evaluate).
+    void count_pairs(double* near_total) {
+        check(hipSetDevice(device), "hipSetDevice");
+        long end = setup.wake.end_index < nr_points ? setup.wake.end_index : nr_points;
+        long blocks = 0;
+        double* buf = tmp(2 * ((end + 255) / 256) + 2);
+        Inst d = inst();
+        check(launch_count_pairs(d, d.pts, end, (long)N * nz_rows, buf, &blocks, stream), "count pairs");
+        std::vector<double> h(2 * blocks);
+        if (blocks)
+            check(hipMemcpyAsync(h.data(), buf, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, stream), "download");
+        check(hipStreamSynchronize(stream), "count pairs");
+        double a = 0.0, b = 0.0;
+        for (long q = 0; q < blocks; ++q) {
+            a += h[2 * q];
+            b += h[2 * q + 1];
+        }
+        near_total[0] = a;
+        near_total[1] = b;
     }
 
     double* tmp(size_t n) {
@@ -606,6 +628,11 @@ int sb_sim_last_pair_counts(const sb_sim* s, double* c) {
     if (!s || !c) return -1;
     for (int k = 0; k < 3; ++k) c[k] = s->pairs[k];
     return 0;
+}
+
+int sb_sim_count_self_induction_pairs(sb_sim* s, double* near_total) {
+    if (!s || !near_total) return -1;
+    return guarded([&] { s->count_pairs(near_total); });
 }
 
 int sb_wake_induced_velocities(const double* recs, size_t n_panels, const double* targets, size_t n_targets,
