@@ -143,7 +143,7 @@ SB_HD double foil_cd(const FoilParams& f, double a) {  // foil.rs:212-243
     return raw + during;
 }
 
-// Far-field record of one wake panel, the unit the pair kernels stream through LDS.
+// Record of one wake panel, the unit the pair kernels stream through LDS.
 // (panel.rs:16-23; the corners live in the wake point array and are gathered
 // only for near pairs.)
 struct PanelRec {
@@ -152,7 +152,8 @@ struct PanelRec {
     double far2;         // (max(|p1-p0|, |p2-p1|) * far_field_ratio)^2
     double area_term;    // area / (4 pi)
     double strength;     // circulation of the panel
+    double cbase;        // flat index of corner 0 in the wake point array (s*P + i + wing), or -1
 };
-static_assert(sizeof(PanelRec) == 72, "PanelRec layout");
+static_assert(sizeof(PanelRec) == 80, "PanelRec layout");
 
 }  // namespace sb

@@ -89,6 +89,16 @@ struct OutLayout {
 
 enum Mode { MODE_EXACT = 0, MODE_FAST = 1, MODE_FP32_KAHAN = 2 };
 
+// Scratch of the load-balanced (McNaughton) pair-sum launch, owned by one
+// simulation: partial sums of split target blocks and their publish flags.
+struct McBuffers {
+    double* partial = nullptr;    // [cap_jobs][256][3]
+    long long* flags = nullptr;   // [cap_jobs], initialised to -1
+    long cap_jobs = 0;
+    long long epoch = 0;          // incremented per launch; flags compare against it
+    int slots[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // resident workgroups per (mode, symmetry)
+};
+
 // Kernel launchers (sb_kernels.hip). All enqueue on `st` and return hipError_t.
 hipError_t launch_prepare(const Inst& d, int first_step, hipStream_t st);
 hipError_t launch_init_wake(const Inst& d, double dt, int use_mean_velocity, const double* velocity3,
@@ -100,12 +110,15 @@ hipError_t launch_factors(const Inst& d, hipStream_t st);
 hipError_t launch_solve(const Inst& d, hipStream_t st);
 hipError_t launch_set_row0(const Inst& d, hipStream_t st);
 hipError_t launch_velocity_copy(const Inst& d, hipStream_t st);
-hipError_t launch_self_induction(const Inst& d, int mode, hipStream_t st);
+hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStream_t st);
+// Induced velocity at `targets` from panels [0, n_panels): out (+)= sum.
+hipError_t launch_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode, double* out,
+                          int add, McBuffers* mc, hipStream_t st);
 hipError_t launch_forces(const Inst& d, double dt, hipStream_t st);
 hipError_t launch_flow_update(const Inst& d, hipStream_t st);
 // Induced velocity at arbitrary targets from panels [0, n_panels): out = sum.
 hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_targets, long n_panels,
-                                  int mode, double* out, hipStream_t st);
+                                  int mode, double* out, McBuffers* mc, hipStream_t st);
 hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targets, long n_panels,
                               double* partial, long* n_blocks, hipStream_t st);
 hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st);

@@ -58,9 +58,9 @@ __device__ inline V3 segment_unit(V3 l0, V3 l1, V3 x, double rc) {
     return V3{0.0, 0.0, 0.0};
 }
 
-// Corners of panel k (dynamic_wake/mod.rs:82-103): stream row s, strip i,
-// columns col[i] and col[i]+1 of rows s and s+1.
-__device__ inline void panel_corners(const Inst& d, int k, V3 c[4]) {
+// Corners of panel k (dynamic_wake/mod.rs:82-103): rows s and s+1, columns
+// i + wing(i) and +1; `cbase` = s*P + i + wing(i) is stored in the record.
+__device__ inline void panel_corners(const Inst& d, const PanelRec& r, int k, V3 c[4]) {
     if (d.corners) {
         const double* q = d.corners + 12L * k;
         c[0] = load3(q);
@@ -69,9 +69,7 @@ __device__ inline void panel_corners(const Inst& d, int k, V3 c[4]) {
         c[3] = load3(q + 9);
         return;
     }
-    int s = k / d.N;
-    int i = k - s * d.N;
-    long a = (long)s * d.P + d.col[i];
+    long a = (long)r.cbase;
     long b = a + d.P;
     c[0] = load3(d.pts + 3 * a);
     c[1] = load3(d.pts + 3 * (a + 1));
@@ -80,9 +78,9 @@ __device__ inline void panel_corners(const Inst& d, int k, V3 c[4]) {
 }
 
 // panel.rs:99-118 (vortex ring) — used for near pairs only
-__device__ inline V3 panel_rings(const Inst& d, int k, V3 x) {
+__device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x) {
     V3 c[4];
-    panel_corners(d, k, c);
+    panel_corners(d, r, k, c);
     double rc = d.core[k];
     V3 u{0.0, 0.0, 0.0};
     u = u + segment_unit(c[0], c[1], x, rc);
@@ -98,7 +96,7 @@ __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
     double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
     if (MODE == MODE_EXACT) {
         double d2 = tx * tx + ty * ty + tz * tz;
-        if (d2 <= r.far2) return panel_rings(d, k, x);
+        if (d2 <= r.far2) return panel_rings(d, r, k, x);
         // panel.rs:82-96 point doublet
         double d5 = d2 * d2 * sqrt(d2);
         double nh = tx * r.nx + ty * r.ny + tz * r.nz;
@@ -107,7 +105,7 @@ __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
         return V3{(num.x * r.area_term) / d5, (num.y * r.area_term) / d5, (num.z * r.area_term) / d5};
     } else {
         double d2 = fma(tx, tx, fma(ty, ty, tz * tz));
-        if (d2 <= r.far2) return panel_rings(d, k, x);
+        if (d2 <= r.far2) return panel_rings(d, r, k, x);
         double y = __builtin_amdgcn_rsq(d2);
         // one Newton step on 1/sqrt(d2): y <- y (3 - d2 y^2) / 2
         double e = fma(-d2 * y, y, 1.0);
@@ -144,41 +142,128 @@ __device__ inline V3 unit_velocity_sym(const Inst& d, const PanelRec& r, int k, 
 // ---------------------------------------------------------------------------
 // Biot-Savart sums
 // ---------------------------------------------------------------------------
-constexpr int TILE = 256;  // panel records per LDS tile (18 KiB)
+constexpr int TILE = 256;  // panel records per LDS tile (20 KiB)
+constexpr int RECD = (int)(sizeof(PanelRec) / sizeof(double));
 
-// One target per lane; panels streamed through LDS in ascending index order,
-// so each target's fold  acc = acc + strength * u  runs in the reference's
-// order (induced_velocity_calc.rs:111-126).  out[t] += acc when `add` is set
-// (update_velocity_at_points, update_data.rs:125-141), else out[t] = acc.
-template <int MODE>
+template <int MODE, int SYM>
+__device__ inline V3 pair_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
+    if (SYM) return unit_velocity_sym<MODE>(d, r, k, x);
+    return panel_unit<MODE>(d, r, k, x);
+}
+
+// Folds panels [p0, p1) into `acc` for this lane's target x, in ascending
+// panel order: acc = acc + strength * u  (induced_velocity_calc.rs:111-126;
+// a zero-strength panel adds nothing, :202-210).  Every thread of the block
+// calls it with the same range: records are staged through the shared tile,
+// and the next record is read while the current one is evaluated.
+template <int MODE, int SYM>
+__device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1) {
+    for (long base = p0; base < p1; base += TILE) {
+        int cnt = (int)((p1 - base) < TILE ? (p1 - base) : TILE);
+        __syncthreads();
+        const double* src = reinterpret_cast<const double*>(d.rec + base);
+        double* dst = reinterpret_cast<double*>(tile);
+        for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
+        __syncthreads();
+        PanelRec nxt = tile[0];
+        for (int p = 0; p < cnt; ++p) {
+            PanelRec r = nxt;
+            if (p + 1 < cnt) nxt = tile[p + 1];
+            if (r.strength == 0.0) continue;
+            V3 u = pair_unit<MODE, SYM>(d, r, (int)(base + p), x);
+            acc = acc + u * r.strength;
+        }
+    }
+    return acc;
+}
+
+// One target per lane, the whole panel range per block.  out[t] += acc when
+// `add` is set (update_velocity_at_points, update_data.rs:125-141), else
+// out[t] = acc.
+template <int MODE, int SYM>
 __global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restrict__ targets, long n_targets,
                                                  long n_panels, double* __restrict__ out, int add) {
     __shared__ PanelRec tile[TILE];
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long tc = t < n_targets ? t : n_targets - 1;
     V3 x = load3(targets + 3 * tc);
-    V3 acc{0.0, 0.0, 0.0};
-    for (long base = 0; base < n_panels; base += TILE) {
-        int cnt = (int)((n_panels - base) < TILE ? (n_panels - base) : TILE);
-        __syncthreads();
-        const double* src = reinterpret_cast<const double*>(d.rec + base);
-        double* dst = reinterpret_cast<double*>(tile);
-        for (int q = threadIdx.x; q < cnt * 9; q += blockDim.x) dst[q] = src[q];
-        __syncthreads();
-        for (int p = 0; p < cnt; ++p) {
-            PanelRec r = tile[p];
-            double g = r.strength;
-            if (g == 0.0) continue;
-            V3 u = unit_velocity_sym<MODE>(d, r, (int)(base + p), x);
-            acc = acc + u * g;
-        }
-    }
+    V3 acc = sweep<MODE, SYM>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels);
     if (t < n_targets) {
-        if (add) {
-            V3 v = load3(out + 3 * t);
-            store3(out + 3 * t, v + acc);
-        } else {
-            store3(out + 3 * t, acc);
+        if (add) store3(out + 3 * t, load3(out + 3 * t) + acc);
+        else store3(out + 3 * t, acc);
+    }
+}
+
+// Same sum with the work spread evenly over exactly the resident workgroups
+// (McNaughton's wrap-around rule).  Job j = targets [256 j, 256 j + 256) over
+// all L panels; the concatenated jobs are cut into equal ranges of C > L
+// panel units, one per workgroup.  A job cut between workgroups b and b+1 runs
+// its first panels at the start of b+1's range (saving the partial sums) and
+// its last panels at the end of b's range, continuing from those partial sums:
+// the per-target fold keeps its ascending order, so the result is bit-identical
+// to k_targets.  b reaches the job no earlier than b+1 finishes its part; if
+// the partial sums are not published within `spin_limit` polls, b recomputes
+// the first panels itself (same bits), so every workgroup terminates.
+struct McPlan {
+    long C, n_jobs, L;
+    double* partial;     // [n_jobs][256][3]
+    long long* flags;    // [n_jobs], == epoch once the partial sums are published
+    long long epoch;
+    int spin_limit;
+};
+
+template <int MODE, int SYM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_targets_mc(Inst d, const double* __restrict__ targets, long n_targets,
+                                                    double* __restrict__ out, int add, McPlan pl) {
+    __shared__ PanelRec tile[TILE];
+    __shared__ int ready;
+    const long L = pl.L;
+    const long g0 = (long)blockIdx.x * pl.C;
+    const long gend = pl.n_jobs * L;
+    const long g1 = g0 + pl.C < gend ? g0 + pl.C : gend;
+    for (long j = g0 / L; j * L < g1; ++j) {
+        const long js = j * L, je = js + L;
+        const long a = js > g0 ? js : g0;
+        const long b = je < g1 ? je : g1;
+        const bool head = a > js;  // first panels of job j (its last panels end the previous range)
+        const bool tail = b < je;  // last panels of job j (its first panels start the next range)
+        const long p0 = tail ? L - (b - a) : 0;
+        const long p1 = head ? (b - a) : L;
+        const long t = j * 256 + threadIdx.x;
+        const long tc = t < n_targets ? t : n_targets - 1;
+        V3 x = load3(targets + 3 * tc);
+        V3 acc{0.0, 0.0, 0.0};
+        double* part = pl.partial + 3 * t;
+        long start = p0;
+        if (tail) {
+            if (threadIdx.x == 0) {
+                int ok = 0;
+                for (int it = 0; it < pl.spin_limit; ++it) {
+                    if (__hip_atomic_load(pl.flags + j, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == pl.epoch) {
+                        ok = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(8);
+                }
+                ready = ok;
+            }
+            __syncthreads();
+            if (ready) {
+                __threadfence();
+                acc = load3(part);
+            } else {
+                start = 0;  // recompute the first panels here: same order, same bits
+            }
+        }
+        acc = sweep<MODE, SYM>(d, tile, x, acc, start, p1);
+        if (head) {
+            store3(part, acc);
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_store(pl.flags + j, pl.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (t < n_targets) {
+            if (add) store3(out + 3 * t, load3(out + 3 * t) + acc);
+            else store3(out + 3 * t, acc);
         }
     }
 }
@@ -199,7 +284,7 @@ __global__ __launch_bounds__(256) void k_count_pairs(Inst d, const double* __res
         __syncthreads();
         const double* src = reinterpret_cast<const double*>(d.rec + base);
         double* dst = reinterpret_cast<double*>(tile);
-        for (int q = threadIdx.x; q < cnt * 9; q += blockDim.x) dst[q] = src[q];
+        for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
         __syncthreads();
         for (int p = 0; p < cnt; ++p) {
             PanelRec r = tile[p];
@@ -429,9 +514,16 @@ __global__ void k_panels(Inst d, int shift) {
     long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long np = (long)d.N * d.M;
     if (k >= np) return;
+    long s_row = k / d.N;
+    int i = (int)(k - s_row * d.N);
+    long a0 = s_row * d.P + d.col[i];
     V3 c[4];
-    panel_corners(d, (int)k, c);
+    c[0] = load3(d.pts + 3 * a0);
+    c[1] = load3(d.pts + 3 * (a0 + 1));
+    c[2] = load3(d.pts + 3 * (a0 + d.P + 1));
+    c[3] = load3(d.pts + 3 * (a0 + d.P));
     PanelRec r;
+    r.cbase = (double)a0;
     V3 center = 0.25 * (c[0] + c[1] + c[2] + c[3]);
     double a = len(c[1] - c[0]);
     double b = len(c[2] - c[1]);
@@ -790,17 +882,57 @@ hipError_t launch_velocity_copy(const Inst& d, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_self_induction(const Inst& d, int mode, hipStream_t st) {
+template <int MODE, int SYM>
+static int mc_slots_for() {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_targets_mc<MODE, SYM>, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+template <int MODE, int SYM>
+static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_targets, long n_panels, double* out,
+                                   int add, McBuffers* mc, hipStream_t st) {
+    long n_jobs = (n_targets + 255) / 256;
+    int& slots = mc ? mc->slots[MODE][SYM] : *(int*)nullptr;
+    if (mc && slots == 0) slots = mc_slots_for<MODE, SYM>();
+    if (mc && slots > 0 && n_jobs > slots && mc->cap_jobs >= n_jobs && n_panels > 0) {
+        McPlan pl;
+        pl.L = n_panels;
+        pl.n_jobs = n_jobs;
+        pl.C = (n_jobs * n_panels + slots - 1) / slots;
+        pl.partial = mc->partial;
+        pl.flags = mc->flags;
+        pl.epoch = ++mc->epoch;
+        pl.spin_limit = 200000;
+        long grid = (n_jobs * n_panels + pl.C - 1) / pl.C;
+        hipLaunchKernelGGL((k_targets_mc<MODE, SYM>), dim3((unsigned)grid), dim3(256), 0, st, d, targets, n_targets,
+                           out, add, pl);
+    } else {
+        hipLaunchKernelGGL((k_targets<MODE, SYM>), dim3((unsigned)n_jobs), dim3(256), 0, st, d, targets, n_targets,
+                           n_panels, out, add);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode, double* out,
+                          int add, McBuffers* mc, hipStream_t st) {
+    if (n_targets <= 0) return hipSuccess;
+    bool sym = d.symmetry != 0;
+    if (mode == MODE_FAST)
+        return sym ? launch_targets_t<MODE_FAST, 1>(d, targets, n_targets, n_panels, out, add, mc, st)
+                   : launch_targets_t<MODE_FAST, 0>(d, targets, n_targets, n_panels, out, add, mc, st);
+    return sym ? launch_targets_t<MODE_EXACT, 1>(d, targets, n_targets, n_panels, out, add, mc, st)
+               : launch_targets_t<MODE_EXACT, 0>(d, targets, n_targets, n_panels, out, add, mc, st);
+}
+
+hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStream_t st) {
     long np = (long)d.P * (d.M + 1);
     long end = d.end_index < np ? d.end_index : np;
     long n_panels = (long)d.N * d.nz_rows;
     if (end <= 0 || n_panels <= 0) return hipSuccess;
-    unsigned nb = blocks_for(end, 256);
-    if (mode == MODE_FAST)
-        hipLaunchKernelGGL(k_targets<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, d.pts, end, n_panels, d.vel, 1);
-    else
-        hipLaunchKernelGGL(k_targets<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, d.pts, end, n_panels, d.vel, 1);
-    return hipGetLastError();
+    return launch_targets(d, d.pts, end, n_panels, mode, d.vel, 1, mc, st);
 }
 
 hipError_t launch_forces(const Inst& d, double dt, hipStream_t st) {
@@ -812,7 +944,7 @@ hipError_t launch_forces(const Inst& d, double dt, hipStream_t st) {
 hipError_t launch_flow_update(const Inst&, hipStream_t) { return hipSuccess; }
 
 hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode,
-                                  double* out, hipStream_t st) {
+                                  double* out, McBuffers* mc, hipStream_t st) {
     if (n_targets <= 0) return hipSuccess;
     if (n_panels <= 0) return hipMemsetAsync(out, 0, sizeof(double) * 3 * n_targets, st);
     if (n_targets < 2048) {
@@ -823,16 +955,9 @@ hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_t
         else
             hipLaunchKernelGGL(k_fold<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, targets, (int)n_targets, 0,
                                (int)n_panels, 0, out);
-    } else {
-        unsigned nb = blocks_for(n_targets, 256);
-        if (mode == MODE_FAST)
-            hipLaunchKernelGGL(k_targets<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, targets, n_targets, n_panels,
-                               out, 0);
-        else
-            hipLaunchKernelGGL(k_targets<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, targets, n_targets, n_panels,
-                               out, 0);
+        return hipGetLastError();
     }
-    return hipGetLastError();
+    return launch_targets(d, targets, n_targets, n_panels, mode, out, 0, mc, st);
 }
 
 hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targets, long n_panels,

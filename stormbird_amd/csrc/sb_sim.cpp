@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -70,6 +71,8 @@ struct sb_sim {
     PanelRec* d_rec = nullptr;
     int *d_col = nullptr, *d_wing = nullptr, *d_span_src = nullptr;
     FoilParams* d_foil = nullptr;
+    McBuffers mc, tmp_mc;
+    bool balanced = true;
     int cur = 0, scur = 0;
 
     std::vector<double> h_geo, h_out;
@@ -81,7 +84,7 @@ struct sb_sim {
         (void)hipSetDevice(device);
         void* bufs[] = {d_pts[0], d_pts[1], d_vel, d_str[0], d_str[1], d_core, d_geo, d_fs, d_felt,
                         d_felt_span, d_fixed, d_F, d_prev, d_hist, d_scal, d_out, d_tmp, d_rec,
-                        d_col, d_wing, d_span_src, d_foil};
+                        d_col, d_wing, d_span_src, d_foil, mc.partial, mc.flags, tmp_mc.partial, tmp_mc.flags};
         for (void* p : bufs)
             if (p) (void)hipFree(p);
         for (auto& e : ev)
@@ -239,6 +242,16 @@ struct sb_sim {
         d_wing = dalloc<int>(N);
         d_span_src = dalloc<int>(2 * P);
         d_foil = dalloc<FoilParams>(W);
+        // SB_PAIR_SCHEDULE=plain keeps one workgroup per 256 targets for the pair
+        // sums (no load balancing); the default balances over the resident workgroups.
+        const char* sched = getenv("SB_PAIR_SCHEDULE");
+        balanced = !(sched && strcmp(sched, "plain") == 0);
+        mc.cap_jobs = balanced ? (nr_points + 255) / 256 : 0;
+        if (mc.cap_jobs > 0) {
+            mc.partial = dalloc<double>(mc.cap_jobs * 256 * 3);
+            mc.flags = dalloc<long long>(mc.cap_jobs);
+            check(hipMemsetAsync(mc.flags, 0xff, sizeof(long long) * mc.cap_jobs, stream), "memset");
+        }
         std::vector<int> col(N), span_src(2 * P);
         for (int i = 0; i < N; ++i) col[i] = i + m.wing_of[i];
         int j = 0;
@@ -320,7 +333,7 @@ struct sb_sim {
         long end = d.end_index < nr_points ? d.end_index : nr_points;
         pairs[0] = 0;
         if (end > 0 && steps_completed > 2) {
-            check(launch_self_induction(d, mode, stream), "self induction");
+            check(launch_self_induction(d, mode, &mc, stream), "self induction");
             pairs[0] = (double)end * (double)N * (double)nz_rows;
         }
         steps_completed += 1;
@@ -397,7 +410,20 @@ struct sb_sim {
         double* buf = tmp(6 * n);
         check(hipMemcpyAsync(buf, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream), "upload points");
         Inst d = inst();
-        check(launch_induced_targets(d, buf, (long)n, (long)N * nz_rows, mode, buf + 3 * n, stream), "induced");
+        McBuffers* m = (long)((n + 255) / 256) <= mc.cap_jobs ? &mc : nullptr;
+        if (!m && n > 2048 && balanced) {  // larger than the wake: scratch sized for n
+            if (!tmp_mc.partial || tmp_mc.cap_jobs < (long)((n + 255) / 256)) {
+                if (tmp_mc.partial) (void)hipFree(tmp_mc.partial);
+                if (tmp_mc.flags) (void)hipFree(tmp_mc.flags);
+                tmp_mc.cap_jobs = (long)((n + 255) / 256);
+                tmp_mc.partial = dalloc<double>(tmp_mc.cap_jobs * 256 * 3);
+                tmp_mc.flags = dalloc<long long>(tmp_mc.cap_jobs);
+                check(hipMemsetAsync(tmp_mc.flags, 0xff, sizeof(long long) * tmp_mc.cap_jobs, stream), "memset");
+                tmp_mc.epoch = 0;
+            }
+            m = &tmp_mc;
+        }
+        check(launch_induced_targets(d, buf, (long)n, (long)N * nz_rows, mode, buf + 3 * n, m, stream), "induced");
         check(hipMemcpyAsync(out, buf + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, stream), "download");
         check(hipStreamSynchronize(stream), "induced");
     }
@@ -651,7 +677,7 @@ int sb_wake_induced_velocities(const double* recs, size_t n_panels, const double
         for (size_t k = 0; k < n_panels; ++k) {
             const double* q = recs + 22 * k;
             memcpy(&corners[12 * k], q, sizeof(double) * 12);
-            r[k] = PanelRec{q[12], q[13], q[14], q[15], q[16], q[17], q[18], q[20], q[21]};
+            r[k] = PanelRec{q[12], q[13], q[14], q[15], q[16], q[17], q[18], q[20], q[21], -1.0};
             core[k] = q[19];
         }
         check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "stream");
@@ -672,7 +698,7 @@ int sb_wake_induced_velocities(const double* recs, size_t n_panels, const double
         d.rec = d_rec;
         d.corners = d_corners;
         d.core = d_core;
-        check(launch_induced_targets(d, d_t, (long)n_targets, (long)n_panels, mode, d_t + 3 * n_targets, st),
+        check(launch_induced_targets(d, d_t, (long)n_targets, (long)n_panels, mode, d_t + 3 * n_targets, nullptr, st),
               "induced");
         check(hipMemcpyAsync(out, d_t + 3 * n_targets, sizeof(double) * 3 * n_targets, hipMemcpyDeviceToHost, st),
               "download");

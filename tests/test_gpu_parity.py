@@ -299,3 +299,38 @@ def test_warm_start_hook_roundtrip():
     assert same(sim.previous_circulation_strength, g)
     with pytest.raises(RuntimeError):
         sim.previous_circulation_strength = g[:-1]
+
+
+def test_load_balanced_sum_bitwise_many_targets(oracle):
+    """300k targets (more 256-target blocks than resident workgroups): the wrap-around
+    schedule splits target blocks between workgroups; results stay bit-identical."""
+    js, case = S.four_sail_free_wake(nr_panels=40, nr_strips=8)
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "pm")
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    step_both(gpu, orc, fs, 10, case["time_step"])
+    rng = np.random.default_rng(12)
+    tg = rng.uniform(-80, 80, (300_000, 3))
+    assert same(gpu.induced_velocities(tg), orc.induced_velocities(tg))
+
+
+def test_full_size_free_wake_step_sampled_bitwise(oracle):
+    """One EXACT step at the benchmark size (328,164 wake points, 320,000 panels, balanced
+    schedule): the wake velocities at 3,000 sampled points equal freestream + the oracle's
+    induced velocity (at the GPU's post-step wake) bit for bit."""
+    import bench
+
+    js, case = S.four_sail_free_wake(nr_panels=2000, nr_strips=40)
+    gpu = sb.Simulation(js)
+    pts, vel, gam, prev, fs = bench.synthesize_developed_state(gpu, case)
+    gpu.set_wake_state(pts, vel, gam, 2000)
+    gpu.previous_circulation_strength = prev
+    dt = case["time_step"]
+    gpu.do_step(time=2000 * dt, time_step=dt, freestream_velocity=fs)
+    w = gpu.get_wake_state()
+    orc = oracle.Simulation(js, "pm")  # the GPU's post-step geometry and strengths
+    orc.set_wake_state(w["points"], w["velocities"], w["strengths"], w["steps_completed"])
+    idx = np.random.default_rng(3).choice(w["points"].shape[0], 3000, replace=False)
+    u = orc.induced_velocities(w["points"][idx])
+    expect = fs[gpu.nr_ctrl_points:][idx] + u
+    assert same(w["velocities"][idx], expect)
