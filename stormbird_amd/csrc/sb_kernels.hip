@@ -90,6 +90,47 @@ __device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x) 
     return u;
 }
 
+// The point-doublet quotient num / (d2 * d2 * sqrt(d2)) of panel.rs:86-95 with
+// IEEE square root and division, in fewer instructions.  gfx950 expands an
+// IEEE f64 sqrt into operand scaling, v_rsq_f64 and two Goldschmidt/Newton
+// corrections, then a zero/inf select; and each IEEE f64 division into operand
+// scaling (v_div_scale), a reciprocal refined by two Newton steps, a quotient
+// corrected by one fma residual step (Markstein), and v_div_fixup.  For
+// d2 in [2^-120, 2^120] (so the denominator lies in [2^-300, 2^300]) and
+// numerators with 2^-600 <= |n| <= 2^400, every scaling step, the select and
+// the fix-up return their input unchanged, so the sequences below, which keep
+// the rest of those expansions step for step and share the refined reciprocal
+// among the three numerators, return the same bits.  Other lanes (including a
+// zero numerator) take the plain IEEE operators.
+__device__ inline V3 doublet_quotients(double d2, V3 num) {
+    double mx = fmax(fmax(fabs(num.x), fabs(num.y)), fabs(num.z));
+    double mn = fmin(fmin(fabs(num.x), fabs(num.y)), fabs(num.z));
+    bool fast = d2 >= 0x1p-120 && d2 <= 0x1p120 && mx <= 0x1p400 && mn >= 0x1p-600;
+    if (fast) {
+        double y = __builtin_amdgcn_rsq(d2);
+        double h = d2 * y;
+        double g = y * 0.5;
+        double e = fma(-g, h, 0.5);
+        h = fma(h, e, h);
+        double rr = fma(-h, h, d2);
+        g = fma(g, e, g);
+        h = fma(rr, g, h);
+        rr = fma(-h, h, d2);
+        h = fma(rr, g, h);  // = sqrt(d2), correctly rounded
+        double den = d2 * d2 * h;
+        double v = __builtin_amdgcn_rcp(den);
+        double t = fma(-den, v, 1.0);
+        v = fma(v, t, v);
+        t = fma(-den, v, 1.0);
+        v = fma(v, t, v);
+        double qx = num.x * v, qy = num.y * v, qz = num.z * v;
+        double ex = fma(-den, qx, num.x), ey = fma(-den, qy, num.y), ez = fma(-den, qz, num.z);
+        return V3{fma(ex, v, qx), fma(ey, v, qy), fma(ez, v, qz)};
+    }
+    double d5 = d2 * d2 * sqrt(d2);
+    return V3{num.x / d5, num.y / d5, num.z / d5};
+}
+
 template <int MODE>
 __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
     // panel.rs:53-76: near/far switch on |x - c|^2 <= far^2
@@ -98,11 +139,11 @@ __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
         double d2 = tx * tx + ty * ty + tz * tz;
         if (d2 <= r.far2) return panel_rings(d, r, k, x);
         // panel.rs:82-96 point doublet
-        double d5 = d2 * d2 * sqrt(d2);
         double nh = tx * r.nx + ty * r.ny + tz * r.nz;
         double h3 = 3.0 * nh;
-        V3 num{tx * h3 - r.nx * d2, ty * h3 - r.ny * d2, tz * h3 - r.nz * d2};
-        return V3{(num.x * r.area_term) / d5, (num.y * r.area_term) / d5, (num.z * r.area_term) / d5};
+        V3 num{(tx * h3 - r.nx * d2) * r.area_term, (ty * h3 - r.ny * d2) * r.area_term,
+               (tz * h3 - r.nz * d2) * r.area_term};
+        return doublet_quotients(d2, num);
     } else {
         double d2 = fma(tx, tx, fma(ty, ty, tz * tz));
         if (d2 <= r.far2) return panel_rings(d, r, k, x);
@@ -165,13 +206,15 @@ __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0,
         double* dst = reinterpret_cast<double*>(tile);
         for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
         __syncthreads();
-        PanelRec nxt = tile[0];
-        for (int p = 0; p < cnt; ++p) {
-            PanelRec r = nxt;
-            if (p + 1 < cnt) nxt = tile[p + 1];
-            if (r.strength == 0.0) continue;
-            V3 u = pair_unit<MODE, SYM>(d, r, (int)(base + p), x);
-            acc = acc + u * r.strength;
+        // Two records in flight, in alternating registers: b is read while a
+        // is evaluated and the next a while b is evaluated.
+        PanelRec a = tile[0];
+        for (int p = 0; p < cnt; p += 2) {
+            PanelRec b = tile[p + 1 < cnt ? p + 1 : p];
+            if (a.strength != 0.0) acc = acc + pair_unit<MODE, SYM>(d, a, (int)(base + p), x) * a.strength;
+            if (p + 1 >= cnt) break;
+            a = tile[p + 2 < cnt ? p + 2 : p + 1];
+            if (b.strength != 0.0) acc = acc + pair_unit<MODE, SYM>(d, b, (int)(base + p + 1), x) * b.strength;
         }
     }
     return acc;
