@@ -99,13 +99,24 @@ struct McBuffers {
     int slots[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // resident workgroups per (mode, symmetry)
 };
 
+// Scratch of the two-pass few-target sum (k_terms + k_fold_terms): one chunk
+// of per-pair terms [3][chunk][ld] and the running sums [3][n_targets].
+struct FoldBuffers {
+    double* terms = nullptr;
+    long cap = 0;                 // doubles in `terms`
+    double* acc = nullptr;        // [3 * cap_targets]
+    long cap_targets = 0;
+};
+
 // Kernel launchers (sb_kernels.hip). All enqueue on `st` and return hipError_t.
 hipError_t launch_prepare(const Inst& d, int first_step, hipStream_t st);
 hipError_t launch_init_wake(const Inst& d, double dt, int use_mean_velocity, const double* velocity3,
                             hipStream_t st);
 hipError_t launch_wake_step(const Inst& d, double dt, hipStream_t st);
 hipError_t launch_panels(const Inst& d, const double* pts, int shift_strengths, hipStream_t st);
-hipError_t launch_fixed(const Inst& d, int mode, hipStream_t st);
+// Fixed velocities at the control points (rows >= 1).  Two-pass sum when `fb`
+// holds room for at least one 64-panel chunk, else one wave per target.
+hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st);
 hipError_t launch_factors(const Inst& d, hipStream_t st);
 hipError_t launch_solve(const Inst& d, hipStream_t st);
 hipError_t launch_set_row0(const Inst& d, hipStream_t st);

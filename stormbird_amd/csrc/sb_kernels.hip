@@ -397,6 +397,58 @@ __global__ __launch_bounds__(256) void k_fold(Inst d, const double* __restrict__
     if (t < n_targets && lane == 0) store3(out + 3 * t, acc);
 }
 
+// Few targets, many panels, in two passes over a chunk of panels
+// [p_begin, p_begin + cnt):
+//  k_terms      every (target, panel) term strength * u, one wave per 64
+//               targets x 16 panels, written to terms[c][q][t] (ld >= n_targets);
+//               a skipped panel (zero strength, or the target's own wing in
+//               neglect-self mode) writes +0;
+//  k_fold_terms one lane per (component, target) chain adds the chunk's terms
+//               in ascending panel order to the running sum acc[c][t].
+// Adding +0 leaves a sum unchanged unless the sum is -0, and a sum that starts
+// at +0 never becomes -0, so the result equals the reference's fold that
+// skips those panels (induced_velocity_calc.rs:111-150, 202-210).
+constexpr int TERM_PANELS = 16;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_terms(Inst d, const double* __restrict__ targets, int n_targets,
+                                               long p_begin, long cnt, int neglect, double* __restrict__ terms,
+                                               int ld) {
+    int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    int t = blockIdx.y * WAVE + lane;
+    int tc = t < n_targets ? t : n_targets - 1;
+    V3 x = load3(targets + 3 * tc);
+    int wt = neglect ? d.wing_of[tc] : -1;
+    long q0 = ((long)blockIdx.x * 4 + wave) * TERM_PANELS;
+    for (int j = 0; j < TERM_PANELS; ++j) {
+        long q = q0 + j;
+        if (q >= cnt) break;
+        int k = __builtin_amdgcn_readfirstlane((int)(p_begin + q));
+        PanelRec r = d.rec[k];
+        V3 term{0.0, 0.0, 0.0};
+        bool skip = r.strength == 0.0 || (neglect && d.wing_of[k % d.N] == wt);
+        if (!skip) term = unit_velocity_sym<MODE>(d, r, k, x) * r.strength;
+        if (t < n_targets) {
+            terms[(0 * cnt + q) * ld + t] = term.x;
+            terms[(1 * cnt + q) * ld + t] = term.y;
+            terms[(2 * cnt + q) * ld + t] = term.z;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_fold_terms(const double* __restrict__ terms, int ld, int n_targets, long cnt,
+                                                   int first, double* __restrict__ acc, double* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n_targets) return;
+    int c = i / n_targets, t = i % n_targets;
+    const double* src = terms + c * cnt * ld + t;
+    double s = first ? 0.0 : acc[i];
+#pragma unroll 8
+    for (long q = 0; q < cnt; ++q) s = s + src[q * ld];
+    acc[i] = s;
+    if (out) out[3 * t + c] = s;
+}
+
 // frozen_wake.rs:198-261: F[j][i] = unit velocity of first-row panel j at ctrl point i.
 __global__ void k_factors(Inst d) {
     long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -889,11 +941,31 @@ hipError_t launch_panels(const Inst& d, const double*, int shift, hipStream_t st
     return hipGetLastError();
 }
 
-hipError_t launch_fixed(const Inst& d, int mode, hipStream_t st) {
+hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st) {
     int p_begin = d.N;
     int p_end = d.N * d.nz_rows;
     if (p_end < p_begin) p_end = p_begin;
     const double* ctrl = d.geo + d.gl.ctrl;
+    const int ld = (d.N + WAVE - 1) / WAVE * WAVE;
+    const long chunk = fb && fb->cap_targets >= d.N ? fb->cap / (3L * ld) / (4 * TERM_PANELS) * (4 * TERM_PANELS) : 0;
+    if (chunk > 0 && p_end > p_begin) {
+        int first = 1;
+        for (long b = p_begin; b < p_end; b += chunk) {
+            long cnt = p_end - b < chunk ? p_end - b : chunk;
+            dim3 grid(blocks_for(cnt, 4 * TERM_PANELS), (unsigned)(ld / WAVE));
+            if (mode == MODE_FAST)
+                hipLaunchKernelGGL(k_terms<MODE_FAST>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt, d.neglect_self,
+                                   fb->terms, ld);
+            else
+                hipLaunchKernelGGL(k_terms<MODE_EXACT>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt, d.neglect_self,
+                                   fb->terms, ld);
+            bool last = b + cnt >= p_end;
+            hipLaunchKernelGGL(k_fold_terms, dim3(blocks_for(3L * d.N, 64)), dim3(64), 0, st, fb->terms, ld, d.N,
+                               cnt, first, fb->acc, last ? d.fixed : nullptr);
+            first = 0;
+        }
+        return hipGetLastError();
+    }
     unsigned nb = blocks_for(d.N, 4);
     if (mode == MODE_FAST)
         hipLaunchKernelGGL(k_fold<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, ctrl, d.N, p_begin, p_end,

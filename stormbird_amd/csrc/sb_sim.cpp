@@ -72,6 +72,7 @@ struct sb_sim {
     int *d_col = nullptr, *d_wing = nullptr, *d_span_src = nullptr;
     FoilParams* d_foil = nullptr;
     McBuffers mc, tmp_mc;
+    FoldBuffers fb;
     bool balanced = true;
     int cur = 0, scur = 0;
 
@@ -84,7 +85,8 @@ struct sb_sim {
         (void)hipSetDevice(device);
         void* bufs[] = {d_pts[0], d_pts[1], d_vel, d_str[0], d_str[1], d_core, d_geo, d_fs, d_felt,
                         d_felt_span, d_fixed, d_F, d_prev, d_hist, d_scal, d_out, d_tmp, d_rec,
-                        d_col, d_wing, d_span_src, d_foil, mc.partial, mc.flags, tmp_mc.partial, tmp_mc.flags};
+                        d_col, d_wing, d_span_src, d_foil, mc.partial, mc.flags, tmp_mc.partial, tmp_mc.flags,
+                        fb.terms, fb.acc};
         for (void* p : bufs)
             if (p) (void)hipFree(p);
         for (auto& e : ev)
@@ -252,6 +254,15 @@ struct sb_sim {
             mc.flags = dalloc<long long>(mc.cap_jobs);
             check(hipMemsetAsync(mc.flags, 0xff, sizeof(long long) * mc.cap_jobs, stream), "memset");
         }
+        // Fixed velocities: per-pair terms of up to 32 Mi doubles (256 MiB) per chunk.
+        {
+            long ld = (N + 63) / 64 * 64;
+            long need = 3L * ld * (long)N * (M > 1 ? M - 1 : 1);
+            fb.cap = need < (32L << 20) ? need : (32L << 20);
+            fb.terms = dalloc<double>(fb.cap);
+            fb.cap_targets = N;
+            fb.acc = dalloc<double>(3L * N);
+        }
         std::vector<int> col(N), span_src(2 * P);
         for (int i = 0; i < N; ++i) col[i] = i + m.wing_of[i];
         int j = 0;
@@ -319,7 +330,7 @@ struct sb_sim {
         d = inst();
         check(hipEventRecord(ev[1], stream), "event");
         // FrozenWake::update_from_full_wake (frozen_wake.rs:189-261)
-        check(launch_fixed(d, mode, stream), "fixed velocities");
+        check(launch_fixed(d, mode, &fb, stream), "fixed velocities");
         check(hipEventRecord(ev[2], stream), "event");
         check(launch_factors(d, stream), "factors");
         check(hipEventRecord(ev[3], stream), "event");
