@@ -148,6 +148,40 @@ int sb_debug_libm_device(int fn, const double* x, const double* y, double* out, 
 /* Same functions evaluated by the library's host code. */
 int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n);
 
+/* ---- Batched simulations ------------------------------------------------
+ * Many independent simulations (each built from its own setup string, with
+ * its own strips, wings and wake length) stepped together: one call advances
+ * every instance by one time step with the kernels of sb_sim_do_step launched
+ * once over all instances, the pair sums as persistent workgroups pulling
+ * (instance, target block) jobs from a queue, and, from the second step on,
+ * the step replayed from a captured HIP graph.  Each instance's results are
+ * bit-identical to stepping it alone with sb_sim_do_step (same mode).
+ * No counterpart in the reference, whose callers loop over Simulation values
+ * (interfaces/pystormbird/src/lifting_line/simulation.rs:131-149 per instance);
+ * it replaces such a loop. */
+typedef struct sb_batch sb_batch;
+
+/* n setup strings; NULL on error (sb_last_error). mode: SB_MODE_EXACT or SB_MODE_FAST. */
+sb_batch* sb_batch_create(const char* const* setup_jsons, long n, int device, int mode);
+void sb_batch_destroy(sb_batch* b);
+long sb_batch_size(const sb_batch* b);
+/* Borrowed handle of instance i (valid until sb_batch_destroy): the sb_sim_*
+ * queries and state setters apply to it; step it only through the batch. */
+sb_sim* sb_batch_instance(sb_batch* b, long i);
+/* offsets[i] = first freestream point of instance i in the concatenated
+ * freestream array (instance order, each instance's own point order);
+ * offsets[n] = total.  Returns n + 1, or -1 / -2. */
+long sb_batch_freestream_offsets(const sb_batch* b, long* offsets, size_t cap);
+/* One time step of every instance.  times/dts: [n] (times may be NULL);
+ * fs: the concatenated freestream, n_points points (host memory, or device
+ * memory for the _device variant); results: [n] or NULL. */
+int sb_batch_do_step(sb_batch* b, const double* times, const double* dts, const double* fs, size_t n_points,
+                     sb_step_result* results);
+int sb_batch_do_step_device(sb_batch* b, const double* times, const double* dts, const double* fs,
+                            size_t n_points, sb_step_result* results);
+/* Device time of the last batched step (ms, HIP events around the step). */
+double sb_batch_last_step_ms(const sb_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

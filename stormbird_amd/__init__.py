@@ -75,6 +75,19 @@ def lib():
                                                  ctypes.c_int, _dp]
         L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
         L.sb_debug_libm_host.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
+        L.sb_batch_create.restype = vp
+        L.sb_batch_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_long, ctypes.c_int, ctypes.c_int]
+        L.sb_batch_destroy.argtypes = [vp]
+        L.sb_batch_size.restype = ctypes.c_long
+        L.sb_batch_size.argtypes = [vp]
+        L.sb_batch_instance.restype = vp
+        L.sb_batch_instance.argtypes = [vp, ctypes.c_long]
+        L.sb_batch_freestream_offsets.restype = ctypes.c_long
+        L.sb_batch_freestream_offsets.argtypes = [vp, ctypes.POINTER(ctypes.c_long), ctypes.c_size_t]
+        L.sb_batch_do_step.argtypes = [vp, _dp, _dp, _dp, ctypes.c_size_t, vp]
+        L.sb_batch_do_step_device.argtypes = [vp, _dp, _dp, ctypes.c_void_p, ctypes.c_size_t, vp]
+        L.sb_batch_last_step_ms.restype = ctypes.c_double
+        L.sb_batch_last_step_ms.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -108,10 +121,26 @@ class Simulation:
         self.nr_panels = L.sb_sim_nr_panels(self._h)
         self._buf = StepBuffers(self.nr_ctrl_points, self.nr_wings)
 
+    @classmethod
+    def _borrowed(cls, handle, mode: int, owner):
+        """View of an instance owned by a Batch (not destroyed by this object)."""
+        self = cls.__new__(cls)
+        L = lib()
+        self._h = handle
+        self._owner = owner
+        self.mode = mode
+        self.nr_ctrl_points = L.sb_sim_nr_ctrl_points(handle)
+        self.nr_wings = L.sb_sim_nr_wings(handle)
+        self.nr_span_points = L.sb_sim_nr_span_points(handle)
+        self.nr_wake_points = L.sb_sim_nr_wake_points(handle)
+        self.nr_panels = L.sb_sim_nr_panels(handle)
+        self._buf = StepBuffers(self.nr_ctrl_points, self.nr_wings)
+        return self
+
     def __del__(self):
-        if getattr(self, "_h", None):
+        if getattr(self, "_h", None) and getattr(self, "_owner", None) is None:
             lib().sb_sim_destroy(self._h)
-            self._h = None
+        self._h = None
 
     def nr_freestream_points(self) -> int:
         return self.nr_ctrl_points + self.nr_wake_points
@@ -264,3 +293,78 @@ def libm_host(fn: int, x, y=None) -> np.ndarray:
     out = np.zeros_like(x)
     lib().sb_debug_libm_host(fn, _p(x), _p(y), _p(out), x.size)
     return out
+
+
+class Batch:
+    """Many simulations stepped together on one GPU (``sb_batch_*``).
+
+    ``instances[i]`` is a ``Simulation`` view of instance i: queries and state
+    setters apply to it; advance it only through ``Batch.do_step``.  Each
+    instance's results equal stepping it alone (same mode) bit for bit.
+    """
+
+    def __init__(self, setup_strings, *, device: int = 0, mode: int = MODE_EXACT):
+        L = lib()
+        enc = [s.encode() for s in setup_strings]
+        n = len(enc)
+        arr = (ctypes.c_char_p * n)(*enc)
+        self._h = L.sb_batch_create(arr, n, device, mode)
+        if not self._h:
+            raise ValueError(L.sb_last_error().decode())
+        self.mode = mode
+        self.n = n
+        self.instances = [Simulation._borrowed(L.sb_batch_instance(self._h, i), mode, self) for i in range(n)]
+        offs = (ctypes.c_long * (n + 1))()
+        if L.sb_batch_freestream_offsets(self._h, offs, n + 1) != n + 1:
+            _err("sb_batch_freestream_offsets", -1)
+        self.freestream_offsets = np.array(list(offs), dtype=np.int64)
+        self._res = (StepResultC * n)()
+        for i, sim in enumerate(self.instances):
+            self._res[i] = sim._buf.c
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            for sim in getattr(self, "instances", []):
+                sim._h = None
+            lib().sb_batch_destroy(self._h)
+            self._h = None
+
+    @property
+    def nr_freestream_points(self) -> int:
+        return int(self.freestream_offsets[-1])
+
+    def get_freestream_velocity_points(self) -> np.ndarray:
+        return np.concatenate([s.get_freestream_velocity_points() for s in self.instances])
+
+    def _arrays(self, times, time_steps):
+        dts = _f64(np.broadcast_to(np.asarray(time_steps, dtype=np.float64), (self.n,)).copy())
+        ts = _f64(np.broadcast_to(np.asarray(0.0 if times is None else times, dtype=np.float64), (self.n,)).copy())
+        return ts, dts
+
+    def _collect(self):
+        out = []
+        for i, s in enumerate(self.instances):
+            c = self._res[i]
+            s._buf.c.time, s._buf.c.iterations, s._buf.c.residual = c.time, c.iterations, c.residual
+            out.append(s._buf.snapshot())
+        return out
+
+    def do_step(self, *, times, time_steps, freestream_velocity, want_result: bool = True):
+        fs = _f64(freestream_velocity, (-1, 3))
+        ts, dts = self._arrays(times, time_steps)
+        res = ctypes.cast(self._res, ctypes.c_void_p) if want_result else None
+        rc = lib().sb_batch_do_step(self._h, _p(ts), _p(dts), _p(fs), fs.shape[0], res)
+        if rc != 0:
+            _err("sb_batch_do_step", rc)
+        return self._collect() if want_result else None
+
+    def do_step_device(self, *, times, time_steps, freestream_ptr: int, n_points: int, want_result: bool = True):
+        ts, dts = self._arrays(times, time_steps)
+        res = ctypes.cast(self._res, ctypes.c_void_p) if want_result else None
+        rc = lib().sb_batch_do_step_device(self._h, _p(ts), _p(dts), ctypes.c_void_p(freestream_ptr), n_points, res)
+        if rc != 0:
+            _err("sb_batch_do_step_device", rc)
+        return self._collect() if want_result else None
+
+    def last_step_ms(self) -> float:
+        return lib().sb_batch_last_step_ms(self._h)

@@ -1,0 +1,340 @@
+// Many simulations stepped together on one GPU: the batched form of
+// Simulation::do_step (stormbird/src/lifting_line/simulation.rs:107-203).
+//
+// Each instance keeps its own wake buffers (an sb_sim created from its own
+// setup string); the batch adds one slab per per-step transfer (geometry,
+// freestream, packed results), the instance descriptors of the four stages of
+// a step, and the job lists of the pair sums.  A step is one host update of
+// every instance's line-force model, three uploads, fourteen launches over all
+// instances and one download.  From the second step on, the launches and the
+// transfers between pinned staging buffers and the device are replayed from a
+// captured HIP graph, re-captured only when the launch shapes change (the
+// free-wake self-induction starting, or a different job count).
+//
+// Every instance computes exactly what its own single-instance step computes,
+// bit for bit: the kernel bodies are shared, and the batched pair sums keep
+// each target's ascending panel order.
+#include <algorithm>
+#include <numeric>
+
+#include "sb_host.h"
+
+namespace {
+
+template <class T>
+T* halloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    check(hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    return static_cast<T*>(p);
+}
+
+constexpr int kStages = 4;  // before the wake step, after it, after the strength shift, after the solve
+
+}  // namespace
+
+struct sb_batch {
+    std::vector<std::unique_ptr<sb_sim>> sims;
+    int device = 0, mode = MODE_EXACT, n = 0;
+    hipStream_t stream = nullptr;
+    std::vector<long> geo_off, fs_off, out_off;  // doubles
+    long geo_total = 0, fs_total = 0, out_total = 0;
+    double *d_geo = nullptr, *d_fs = nullptr, *d_out = nullptr;
+    double *h_geo = nullptr, *h_out = nullptr;
+    Inst *h_inst = nullptr, *d_inst = nullptr;  // [kStages][n]
+    long* d_off = nullptr;                      // [BK_KINDS][n + 1]
+    int2 *h_jobs = nullptr, *d_jobs = nullptr;  // fixed jobs then target jobs
+    long cap_fixed_jobs = 0, cap_target_jobs = 0;
+    unsigned* d_counters = nullptr;
+    BatchDev bd;
+    bool first_done = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long graph_sig[3] = {-1, -1, -1};
+    hipEvent_t ev[2] = {};
+    double last_ms = 0.0;
+    int use_graph = 1;
+
+    ~sb_batch() {
+        (void)hipSetDevice(device);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        void* dev[] = {d_geo, d_fs, d_out, d_inst, d_off, d_jobs, d_counters};
+        for (void* p : dev)
+            if (p) (void)hipFree(p);
+        void* host[] = {h_geo, h_out, h_inst, h_jobs};
+        for (void* p : host)
+            if (p) (void)hipHostFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        sims.clear();
+    }
+
+    void create(const char* const* jsons, long count, int dev, int md) {
+        if (count <= 0) throw std::invalid_argument("empty batch");
+        device = dev;
+        mode = md;
+        n = (int)count;
+        check(hipSetDevice(device), "hipSetDevice");
+        for (long i = 0; i < count; ++i) {
+            if (!jsons[i]) throw std::invalid_argument("null setup string in batch");
+            auto s = std::make_unique<sb_sim>();
+            s->create(jsons[i], dev, md);
+            if (s->N > 1024) throw std::invalid_argument("batched instances support at most 1024 strips");
+            sims.push_back(std::move(s));
+        }
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        geo_off.resize(n + 1);
+        fs_off.resize(n + 1);
+        out_off.resize(n + 1);
+        geo_off[0] = fs_off[0] = out_off[0] = 0;
+        int max_n = 1, sym = 0;
+        for (int i = 0; i < n; ++i) {
+            const sb_sim& s = *sims[i];
+            geo_off[i + 1] = geo_off[i] + s.gl.total;
+            fs_off[i + 1] = fs_off[i] + 3L * (s.N + s.nr_points);
+            out_off[i + 1] = out_off[i] + OutLayout::total(s.N, s.W);
+            max_n = std::max(max_n, s.N);
+            sym |= s.setup.wake.symmetry != 0 ? 1 : 0;
+        }
+        geo_total = geo_off[n];
+        fs_total = fs_off[n];
+        out_total = out_off[n];
+        d_geo = dalloc<double>(geo_total);
+        d_fs = dalloc<double>(fs_total);
+        d_out = dalloc<double>(out_total);
+        h_geo = halloc<double>(geo_total);
+        h_out = halloc<double>(out_total);
+        h_inst = halloc<Inst>((size_t)kStages * n);
+        d_inst = dalloc<Inst>((size_t)kStages * n);
+
+        // flat block offsets per kind (256-thread blocks)
+        std::vector<long> off((size_t)BK_KINDS * (n + 1), 0);
+        for (int k = 0; k < BK_KINDS; ++k) {
+            long* o = off.data() + (long)k * (n + 1);
+            for (int i = 0; i < n; ++i) {
+                const sb_sim& s = *sims[i];
+                long items = 0;
+                switch (k) {
+                    case BK_POINTS: items = (long)s.P * (s.M + 1); break;
+                    case BK_SPAN: items = s.P; break;
+                    case BK_PANELS: items = (long)s.N * s.M; break;
+                    case BK_FACTORS: items = (long)s.N * s.N; break;
+                    default: items = 3L * s.P * (s.M + 1); break;
+                }
+                o[i + 1] = o[i] + (items + 255) / 256;
+            }
+            bd.blocks[k] = o[n];
+        }
+        d_off = dalloc<long>(off.size());
+        check(hipMemcpyAsync(d_off, off.data(), sizeof(long) * off.size(), hipMemcpyHostToDevice, stream), "upload");
+
+        cap_fixed_jobs = 0;
+        cap_target_jobs = 0;
+        for (auto& s : sims) {
+            cap_fixed_jobs += (s->N + 63) / 64;
+            long np = (long)s->P * (s->M + 1);
+            long end = std::min<long>(s->setup.wake.end_index, np);
+            cap_target_jobs += (end + 255) / 256;
+        }
+        h_jobs = halloc<int2>(cap_fixed_jobs + cap_target_jobs);
+        d_jobs = dalloc<int2>(cap_fixed_jobs + cap_target_jobs);
+        d_counters = dalloc<unsigned>(2);
+        check(hipMemsetAsync(d_counters, 0, 2 * sizeof(unsigned), stream), "memset");
+
+        bd.n = n;
+        bd.off = d_off;
+        bd.max_threads = std::min(1024, ((max_n + 63) / 64) * 64);
+        bd.solve_shm = sizeof(double) * (2 * (size_t)max_n + 16 + 4);
+        bd.counters = d_counters;
+        bd.sym = sym;
+        bd.fixed_jobs = d_jobs;
+        bd.target_jobs = d_jobs + cap_fixed_jobs;
+        bd.fixed_grid = batch_resident_groups(0, mode, sym);
+        bd.target_grid = batch_resident_groups(1, mode, sym);
+        if (bd.fixed_grid <= 0 || bd.target_grid <= 0) throw HipError("occupancy query failed", -1);
+        const char* g = getenv("SB_BATCH_GRAPH");
+        use_graph = !(g && strcmp(g, "0") == 0);
+        check(hipStreamSynchronize(stream), "batch create");
+    }
+
+    // Descriptor of instance i at the current host state, with the batch slabs.
+    Inst stage_inst(int i, double dt) const {
+        const sb_sim& s = *sims[i];
+        Inst d = s.inst();
+        d.geo = d_geo + geo_off[i];
+        d.fs = d_fs + fs_off[i];
+        d.out = d_out + out_off[i];
+        d.dt = dt;
+        return d;
+    }
+
+    void step(const double* times, const double* dts, const double* fs, size_t n_fs, bool fs_on_device,
+              sb_step_result* results) {
+        check(hipSetDevice(device), "hipSetDevice");
+        if ((long)n_fs != fs_total / 3) throw std::invalid_argument("freestream length mismatch");
+        if (!dts) throw std::invalid_argument("null time steps");
+        for (int i = 0; i < n; ++i) {
+            if (sims[i]->first_done != first_done || sims[i]->steps_completed != sims[0]->steps_completed)
+                throw std::invalid_argument("batched instances must be stepped together");
+        }
+        const bool first = !first_done;
+        // host: line-force-model update and packed geometry of every instance
+        for (int i = 0; i < n; ++i) {
+            sims[i]->setup.lfm.update_global();
+            sims[i]->pack_geometry(h_geo + geo_off[i]);
+        }
+        // stage descriptors (the host state transitions of sb_sim::step)
+        for (int i = 0; i < n; ++i) {
+            sb_sim& s = *sims[i];
+            h_inst[0 * n + i] = stage_inst(i, dts[i]);
+            if (first) s.nz_rows = 0;
+            s.cur ^= 1;
+            h_inst[1 * n + i] = stage_inst(i, dts[i]);
+            s.scur ^= 1;
+            s.nz_rows = s.nz_rows == 0 ? 0 : (s.nz_rows + 1 < s.M ? s.nz_rows + 1 : s.M);
+            h_inst[2 * n + i] = stage_inst(i, dts[i]);
+            s.nz_rows = s.nz_rows < 1 ? 1 : s.nz_rows;
+            h_inst[3 * n + i] = stage_inst(i, dts[i]);
+        }
+        // pair-sum jobs: fixed velocities longest first; self-induction where it runs
+        std::vector<std::pair<long, int2>> fj;
+        fj.reserve(cap_fixed_jobs);
+        long nt = 0;
+        for (int i = 0; i < n; ++i) {
+            const Inst& d = h_inst[2 * n + i];
+            long len = (long)d.N * std::max(d.nz_rows - 1, 0);
+            for (int tb = 0; tb < (d.N + 63) / 64; ++tb) fj.push_back({len, int2{i, tb}});
+        }
+        std::stable_sort(fj.begin(), fj.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        for (size_t k = 0; k < fj.size(); ++k) h_jobs[k] = fj[k].second;
+        for (int i = 0; i < n; ++i) {
+            const sb_sim& s = *sims[i];
+            const Inst& d = h_inst[3 * n + i];
+            long np = (long)d.P * (d.M + 1);
+            long end = std::min<long>(d.end_index, np);
+            if (end > 0 && s.steps_completed > 2 && d.nz_rows > 0)
+                for (long tb = 0; tb < (end + 255) / 256; ++tb) h_jobs[cap_fixed_jobs + nt++] = int2{i, (int)tb};
+        }
+        bd.n_fixed_jobs = (int)fj.size();
+        bd.n_target_jobs = (int)nt;
+
+        // freestream into the slab (outside the graph: the caller's pointer may change)
+        check(hipMemcpyAsync(d_fs, fs, sizeof(double) * fs_total,
+                             fs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
+              "upload freestream");
+        check(hipEventRecord(ev[0], stream), "event");
+        long sig[3] = {first ? 1 : 0, bd.n_fixed_jobs, bd.n_target_jobs};
+        if (first || !use_graph) {
+            enqueue(first);
+        } else {
+            if (!exec || sig[0] != graph_sig[0] || sig[1] != graph_sig[1] || sig[2] != graph_sig[2]) {
+                if (exec) (void)hipGraphExecDestroy(exec);
+                if (graph) (void)hipGraphDestroy(graph);
+                exec = nullptr;
+                graph = nullptr;
+                check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                try {
+                    enqueue(false);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                check(hipStreamEndCapture(stream, &graph), "end capture");
+                check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+                std::copy(sig, sig + 3, graph_sig);
+            }
+            check(hipGraphLaunch(exec, stream), "graph launch");
+        }
+        check(hipEventRecord(ev[1], stream), "event");
+        check(hipStreamSynchronize(stream), "batch step");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        last_ms = ms;
+        if (first) first_done = true;
+        for (int i = 0; i < n; ++i) {
+            sb_sim& s = *sims[i];
+            s.first_done = true;
+            s.steps_completed += 1;
+            std::copy(h_out + out_off[i], h_out + out_off[i + 1], s.h_out.begin());
+            if (results) s.write_result(times ? times[i] : 0.0, results + i);
+        }
+    }
+
+    // The launches of one step (captured into the graph after the first step).
+    void enqueue(bool first) {
+        check(hipMemcpyAsync(d_geo, h_geo, sizeof(double) * geo_total, hipMemcpyHostToDevice, stream), "upload geometry");
+        check(hipMemcpyAsync(d_inst, h_inst, sizeof(Inst) * kStages * n, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_jobs, h_jobs, sizeof(int2) * (cap_fixed_jobs + cap_target_jobs), hipMemcpyHostToDevice,
+                             stream),
+              "upload jobs");
+        BatchDev b = bd;
+        b.inst = d_inst;
+        check(launch_batch_prepare(b, first ? 1 : 0, stream), "prepare");
+        if (first) check(launch_batch_init_wake(b, stream), "init wake");
+        check(launch_batch_wake_step(b, stream), "wake step");
+        b.inst = d_inst + 1L * n;
+        check(launch_batch_panels(b, 1, stream), "panels");
+        b.inst = d_inst + 2L * n;
+        check(launch_batch_fixed(b, mode, stream), "fixed velocities");
+        check(launch_batch_factors(b, stream), "factors");
+        check(launch_batch_solve(b, stream), "solve");
+        b.inst = d_inst + 3L * n;
+        check(launch_batch_velocity_copy(b, stream), "velocity copy");
+        check(launch_batch_self_induction(b, mode, stream), "self induction");
+        check(launch_batch_forces(b, stream), "forces");
+        check(hipMemcpyAsync(h_out, d_out, sizeof(double) * out_total, hipMemcpyDeviceToHost, stream), "download");
+    }
+};
+
+extern "C" {
+
+sb_batch* sb_batch_create(const char* const* setup_jsons, long n, int device, int mode) {
+    if (!setup_jsons) {
+        g_error = "null setup array";
+        return nullptr;
+    }
+    if (mode != MODE_EXACT && mode != MODE_FAST) {
+        g_error = "unsupported mode";
+        return nullptr;
+    }
+    auto b = std::make_unique<sb_batch>();
+    int rc = guarded([&] { b->create(setup_jsons, n, device, mode); });
+    return rc == 0 ? b.release() : nullptr;
+}
+
+void sb_batch_destroy(sb_batch* b) { delete b; }
+
+long sb_batch_size(const sb_batch* b) { return b ? b->n : -1; }
+
+sb_sim* sb_batch_instance(sb_batch* b, long i) {
+    if (!b || i < 0 || i >= b->n) return nullptr;
+    return b->sims[i].get();
+}
+
+long sb_batch_freestream_offsets(const sb_batch* b, long* offsets, size_t cap) {
+    if (!b || !offsets) return -1;
+    if (cap < (size_t)b->n + 1) return -2;
+    for (int i = 0; i <= b->n; ++i) offsets[i] = b->fs_off[i] / 3;
+    return b->n + 1;
+}
+
+int sb_batch_do_step(sb_batch* b, const double* times, const double* dts, const double* fs, size_t n_points,
+                     sb_step_result* results) {
+    if (!b || !fs) return -1;
+    return guarded([&] { b->step(times, dts, fs, n_points, false, results); });
+}
+
+int sb_batch_do_step_device(sb_batch* b, const double* times, const double* dts, const double* fs, size_t n_points,
+                            sb_step_result* results) {
+    if (!b || !fs) return -1;
+    return guarded([&] { b->step(times, dts, fs, n_points, true, results); });
+}
+
+double sb_batch_last_step_ms(const sb_batch* b) { return b ? b->last_ms : -1.0; }
+
+}  // extern "C"

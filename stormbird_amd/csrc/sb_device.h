@@ -46,6 +46,7 @@ struct Inst {
     double far_ratio, first_rel, last_rel, damping, rep_chord;
     double solver_damping, residual_tol, strength_tol, vc_ratio, correction_factor;
     double density;
+    double dt;           // time step (batched launches read it here)
 
     double* pts;         // current wake points [P*(M+1)][3]
     double* pts_next;    // next wake points (double buffer)
@@ -133,5 +134,42 @@ hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_t
 hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targets, long n_panels,
                               double* partial, long* n_blocks, hipStream_t st);
 hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// Many instances per launch (sb_batch).  Every instance runs the same kernel
+// sequence as the single-instance step; a flat grid is mapped back to
+// (instance, block of that instance) through per-instance block offsets, and
+// the pair sums run as persistent workgroups pulling jobs from a queue.
+// ---------------------------------------------------------------------------
+enum BatchKind { BK_POINTS = 0, BK_SPAN = 1, BK_PANELS = 2, BK_FACTORS = 3, BK_VEL = 4, BK_KINDS = 5 };
+
+struct BatchDev {
+    const Inst* inst = nullptr;   // device [n], the descriptors of the current stage
+    int n = 0;
+    const long* off = nullptr;    // device [BK_KINDS][n + 1]: first flat 256-thread block per instance
+    long blocks[BK_KINDS] = {};   // total flat blocks per kind
+    int max_threads = 64;         // one-workgroup kernels: max over instances of N rounded to 64
+    size_t solve_shm = 0;         // bytes of shared memory for the batched solve
+    const int2* fixed_jobs = nullptr;   // (instance, 64-target block), longest first
+    int n_fixed_jobs = 0;
+    const int2* target_jobs = nullptr;  // (instance, 256-target block) of the free-wake self-induction
+    int n_target_jobs = 0;
+    unsigned* counters = nullptr;       // device [2] job-queue heads, zero between launches
+    int fixed_grid = 0, target_grid = 0;  // persistent workgroups
+    int sym = 0;                  // 1 when any instance uses a symmetry plane
+};
+
+hipError_t launch_batch_prepare(const BatchDev& b, int first_step, hipStream_t st);
+hipError_t launch_batch_init_wake(const BatchDev& b, hipStream_t st);
+hipError_t launch_batch_wake_step(const BatchDev& b, hipStream_t st);
+hipError_t launch_batch_panels(const BatchDev& b, int shift, hipStream_t st);
+hipError_t launch_batch_fixed(const BatchDev& b, int mode, hipStream_t st);
+hipError_t launch_batch_factors(const BatchDev& b, hipStream_t st);
+hipError_t launch_batch_solve(const BatchDev& b, hipStream_t st);
+hipError_t launch_batch_velocity_copy(const BatchDev& b, hipStream_t st);
+hipError_t launch_batch_self_induction(const BatchDev& b, int mode, hipStream_t st);
+hipError_t launch_batch_forces(const BatchDev& b, hipStream_t st);
+// Resident workgroups of the persistent pair-sum kernels (0 fixed, 1 self-induction).
+int batch_resident_groups(int which, int mode, int sym);
 
 }  // namespace sb

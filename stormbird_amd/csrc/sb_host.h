@@ -1,0 +1,472 @@
+// Host-side state of one simulation on the GPU (struct sb_sim) and the helpers
+// shared by the single-instance and batched C ABI implementations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <algorithm>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/stormbird_gpu.h"
+#include "sb_device.h"
+#include "sb_model.h"
+
+namespace sb_host {
+
+using namespace sb;
+
+extern thread_local std::string g_error;
+
+
+
+struct HipError : std::runtime_error {
+    int code;
+    HipError(const std::string& m, int c) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e), (int)e);
+}
+
+template <class T>
+T* dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    check(hipMalloc(&p, n * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+inline const char* kTimerNames[] = {"wake_update", "fixed", "factors", "solve", "self_induction", "forces"};
+constexpr int kTimers = 6;
+
+
+}  // namespace sb_host
+
+using namespace sb;
+using namespace sb_host;
+
+struct sb_sim {
+    Setup setup;
+    int device = 0;
+    int mode = MODE_EXACT;
+    hipStream_t stream = nullptr;
+    int N = 0, W = 0, P = 0, M = 0;
+    long nr_points = 0, nr_panels = 0;
+    int nz_rows = 0;
+    long steps_completed = 0;
+    bool first_done = false;
+    GeoLayout gl{};
+
+    // device buffers
+    double *d_pts[2] = {nullptr, nullptr}, *d_vel = nullptr, *d_str[2] = {nullptr, nullptr};
+    double *d_core = nullptr, *d_geo = nullptr, *d_fs = nullptr, *d_felt = nullptr, *d_felt_span = nullptr;
+    double *d_fixed = nullptr, *d_F = nullptr, *d_prev = nullptr, *d_hist = nullptr, *d_scal = nullptr;
+    double *d_out = nullptr, *d_tmp = nullptr;
+    size_t tmp_cap = 0;
+    PanelRec* d_rec = nullptr;
+    int *d_col = nullptr, *d_wing = nullptr, *d_span_src = nullptr;
+    FoilParams* d_foil = nullptr;
+    McBuffers mc, tmp_mc;
+    FoldBuffers fb;
+    bool balanced = true;
+    int cur = 0, scur = 0;
+
+    std::vector<double> h_geo, h_out;
+    hipEvent_t ev[kTimers + 1] = {};
+    double pairs[3] = {0, 0, 0};
+    double timer_ms[kTimers] = {0};
+
+    ~sb_sim() {
+        (void)hipSetDevice(device);
+        void* bufs[] = {d_pts[0], d_pts[1], d_vel, d_str[0], d_str[1], d_core, d_geo, d_fs, d_felt,
+                        d_felt_span, d_fixed, d_F, d_prev, d_hist, d_scal, d_out, d_tmp, d_rec,
+                        d_col, d_wing, d_span_src, d_foil, mc.partial, mc.flags, tmp_mc.partial, tmp_mc.flags,
+                        fb.terms, fb.acc};
+        for (void* p : bufs)
+            if (p) (void)hipFree(p);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    Inst inst() const {
+        Inst d{};
+        const WakeConfig& w = setup.wake;
+        const SolverConfig& s = setup.solver;
+        d.N = N;
+        d.W = W;
+        d.P = P;
+        d.M = M;
+        d.nz_rows = nz_rows;
+        d.symmetry = w.symmetry;
+        d.neglect_self = w.neglect_self ? 1 : 0;
+        d.first_dir = w.first_dir;
+        d.steps_completed = (int)steps_completed;
+        d.output_body = setup.lfm.output_body ? 1 : 0;
+        d.viscous_lift_in_circulation = setup.lfm.viscous_lift_in_circulation ? 1 : 0;
+        d.vc_mode = s.velocity_correction;
+        d.max_iterations = s.max_iterations;
+        d.end_index = w.end_index;
+        d.far_ratio = w.far_ratio;
+        d.first_rel = w.first_rel;
+        d.last_rel = w.last_rel;
+        d.damping = w.damping;
+        d.rep_chord = w.rep_chord;
+        d.solver_damping = s.damping;
+        d.residual_tol = s.residual_tol;
+        d.strength_tol = s.strength_tol;
+        d.vc_ratio = s.velocity_correction_ratio;
+        d.correction_factor = s.correction_factor_initial;
+        d.density = setup.lfm.density;
+        d.pts = d_pts[cur];
+        d.pts_next = d_pts[cur ^ 1];
+        d.vel = d_vel;
+        d.str = d_str[scur];
+        d.str_next = d_str[scur ^ 1];
+        d.core = d_core;
+        d.corners = nullptr;
+        d.rec = d_rec;
+        d.col = d_col;
+        d.wing_of = d_wing;
+        d.span_src = d_span_src;
+        d.foil = d_foil;
+        d.geo = d_geo;
+        d.gl = gl;
+        d.fs = d_fs;
+        d.felt = d_felt;
+        d.felt_span = d_felt_span;
+        d.fixed = d_fixed;
+        d.F = d_F;
+        d.prev_gamma = d_prev;
+        d.hist = d_hist;
+        d.scal = d_scal;
+        d.out = d_out;
+        return d;
+    }
+
+    void layout() {
+        int o = 0;
+        auto take = [&](int n) { int r = o; o += n; return r; };
+        gl.ctrl = take(3 * N);
+        gl.rel_g = take(3 * N);
+        gl.dir_g = take(3 * N);
+        gl.chord_g = take(3 * N);
+        gl.chord_len = take(N);
+        gl.local_len = take(N);
+        gl.rel_local = take(3 * N);
+        gl.rel_out = take(3 * N);
+        gl.dir_out = take(3 * N);
+        gl.ctrl_out = take(3 * N);
+        gl.chord_out = take(3 * N);
+        gl.is_virtual = take(N);
+        gl.span_pts = take(3 * P);
+        gl.chord_span = take(3 * P);
+        gl.rbm = take(18);
+        gl.body_axes = take(9);
+        gl.total = o;
+    }
+
+    // Fixed velocities of the single-instance step: per-pair terms of up to
+    // 32 Mi doubles (256 MiB) per chunk, allocated at the first step.
+    void ensure_fold_buffers() {
+        if (fb.terms) return;
+        long ld = (N + 63) / 64 * 64;
+        long need = 3L * ld * (long)N * (M > 1 ? M - 1 : 1);
+        fb.cap = need < (32L << 20) ? need : (32L << 20);
+        fb.terms = dalloc<double>(fb.cap);
+        fb.cap_targets = N;
+        fb.acc = dalloc<double>(3L * N);
+    }
+
+    // Packs the geometry into `dst` (gl.total doubles), or into h_geo and
+    // uploads it to d_geo when dst is null.
+    void pack_geometry(double* dst = nullptr) {
+        const LineModel& m = setup.lfm;
+        if (!dst) h_geo.assign(gl.total, 0.0);
+        double* g = dst ? dst : h_geo.data();
+        if (dst) std::fill(dst, dst + gl.total, 0.0);
+        auto put = [](double* p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; };
+        for (int i = 0; i < N; ++i) {
+            const SpanLine& G = m.global[i];
+            const SpanLine& L = m.local[i];
+            const SpanLine& O = m.output_body ? L : G;
+            put(g + gl.ctrl + 3 * i, m.ctrl_points[i]);
+            put(g + gl.rel_g + 3 * i, G.rel());
+            put(g + gl.dir_g + 3 * i, G.dir());
+            put(g + gl.chord_g + 3 * i, m.chord_global[i]);
+            g[gl.chord_len + i] = m.chord_len[i];
+            g[gl.local_len + i] = L.length();
+            put(g + gl.rel_local + 3 * i, L.rel());
+            put(g + gl.rel_out + 3 * i, O.rel());
+            put(g + gl.dir_out + 3 * i, O.dir());
+            put(g + gl.ctrl_out + 3 * i, O.ctrl());
+            put(g + gl.chord_out + 3 * i, m.output_body ? m.chord_local[i] : m.chord_global[i]);
+            g[gl.is_virtual + i] = m.is_virtual[i] ? 1.0 : 0.0;
+        }
+        for (int j = 0; j < P; ++j) {
+            put(g + gl.span_pts + 3 * j, m.span_points[j]);
+            put(g + gl.chord_span + 3 * j, m.chord_global_at_span[j]);
+        }
+        put(g + gl.rbm, m.rbm.translation);
+        put(g + gl.rbm + 3, m.rbm.rotation);
+        put(g + gl.rbm + 6, m.rbm.velocity_linear);
+        put(g + gl.rbm + 9, m.rbm.velocity_angular);
+        put(g + gl.body_axes, m.rbm.rotate(v3(1, 0, 0)));
+        put(g + gl.body_axes + 3, m.rbm.rotate(v3(0, 1, 0)));
+        put(g + gl.body_axes + 6, m.rbm.rotate(v3(0, 0, 1)));
+        if (!dst)
+            check(hipMemcpyAsync(d_geo, g, sizeof(double) * gl.total, hipMemcpyHostToDevice, stream),
+                  "upload geometry");
+    }
+
+    void create(const char* json, int dev, int md) {
+        setup = parse_setup(json);
+        device = dev;
+        mode = md;
+        check(hipSetDevice(device), "hipSetDevice");
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        const LineModel& m = setup.lfm;
+        N = m.n();
+        W = m.nw();
+        P = (int)m.span_points.size();
+        M = setup.wake.M;
+        if (N > 1024) throw std::runtime_error("more than 1024 strips per simulation is not supported");
+        nr_points = (long)P * (M + 1);
+        nr_panels = (long)N * M;
+        layout();
+        d_pts[0] = dalloc<double>(3 * nr_points);
+        d_pts[1] = dalloc<double>(3 * nr_points);
+        d_vel = dalloc<double>(3 * nr_points);
+        d_str[0] = dalloc<double>(nr_panels);
+        d_str[1] = dalloc<double>(nr_panels);
+        d_core = dalloc<double>(nr_panels);
+        d_rec = dalloc<PanelRec>(nr_panels);
+        d_geo = dalloc<double>(gl.total);
+        d_fs = dalloc<double>(3 * (N + nr_points));
+        d_felt = dalloc<double>(3 * N);
+        d_felt_span = dalloc<double>(3 * P);
+        d_fixed = dalloc<double>(3 * N);
+        d_F = dalloc<double>(3L * N * N);
+        d_prev = dalloc<double>(N);
+        d_hist = dalloc<double>(6 * N);
+        d_scal = dalloc<double>(8);
+        d_out = dalloc<double>(OutLayout::total(N, W));
+        d_col = dalloc<int>(N);
+        d_wing = dalloc<int>(N);
+        d_span_src = dalloc<int>(2 * P);
+        d_foil = dalloc<FoilParams>(W);
+        // SB_PAIR_SCHEDULE=plain keeps one workgroup per 256 targets for the pair
+        // sums (no load balancing); the default balances over the resident workgroups.
+        const char* sched = getenv("SB_PAIR_SCHEDULE");
+        balanced = !(sched && strcmp(sched, "plain") == 0);
+        mc.cap_jobs = balanced ? (nr_points + 255) / 256 : 0;
+        if (mc.cap_jobs > 0) {
+            mc.partial = dalloc<double>(mc.cap_jobs * 256 * 3);
+            mc.flags = dalloc<long long>(mc.cap_jobs);
+            check(hipMemsetAsync(mc.flags, 0xff, sizeof(long long) * mc.cap_jobs, stream), "memset");
+        }
+        std::vector<int> col(N), span_src(2 * P);
+        for (int i = 0; i < N; ++i) col[i] = i + m.wing_of[i];
+        int j = 0;
+        for (int w = 0; w < W; ++w) {  // value_mapping.rs:87-137 without extrapolation
+            span_src[2 * j] = span_src[2 * j + 1] = m.wing_start[w];
+            ++j;
+            for (int i = m.wing_start[w]; i < m.wing_end[w]; ++i, ++j) {
+                span_src[2 * j] = i;
+                span_src[2 * j + 1] = (i == m.wing_end[w] - 1) ? i : i + 1;
+            }
+        }
+        check(hipMemcpyAsync(d_col, col.data(), sizeof(int) * N, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_wing, m.wing_of.data(), sizeof(int) * N, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_span_src, span_src.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_foil, m.foils.data(), sizeof(FoilParams) * W, hipMemcpyHostToDevice, stream), "upload");
+        check(hipMemcpyAsync(d_core, setup.wake.core.data(), sizeof(double) * nr_panels, hipMemcpyHostToDevice, stream),
+              "upload");
+        check(hipMemsetAsync(d_prev, 0, sizeof(double) * N, stream), "memset");
+        check(hipMemsetAsync(d_hist, 0, sizeof(double) * 6 * N, stream), "memset");
+        check(hipMemsetAsync(d_scal, 0, sizeof(double) * 8, stream), "memset");
+        check(hipMemsetAsync(d_str[1], 0, sizeof(double) * nr_panels, stream), "memset");
+        pack_geometry();
+        // DynamicWakeBuilder::build -> initialize_based_on_chord_length (initialization.rs:46-71)
+        V3 sum = v3(0, 0, 0);
+        for (const V3& c : m.chord_global_at_span) sum = sum + c;
+        V3 avg = sum / (double)m.chord_global_at_span.size();
+        double wl = setup.wake.initial_relative_wake_length * len(avg);
+        V3 vel = (wl / (double)M) * avg;
+        double v[3] = {vel.x, vel.y, vel.z};
+        Inst d = inst();
+        check(launch_init_wake(d, 1.0, 0, v, stream), "init wake");
+        check(launch_panels(d, d.pts, 0, stream), "panels");
+        h_out.assign(OutLayout::total(N, W), 0.0);
+        check(hipStreamSynchronize(stream), "create");
+    }
+
+    void upload_freestream(const double* fs, size_t n, bool on_device) {
+        size_t bytes = sizeof(double) * 3 * n;
+        check(hipMemcpyAsync(d_fs, fs, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
+              "upload freestream");
+    }
+
+    void step(double time, double dt, const double* fs, size_t n, bool fs_on_device, sb_step_result* out) {
+        check(hipSetDevice(device), "hipSetDevice");
+        if ((long)n != N + nr_points) throw std::invalid_argument("freestream length mismatch");
+        setup.lfm.update_global();
+        pack_geometry();
+        upload_freestream(fs, n, fs_on_device);
+        Inst d = inst();
+        bool first = !first_done;
+        check(hipEventRecord(ev[0], stream), "event");
+        check(launch_prepare(d, first ? 1 : 0, stream), "prepare");
+        if (first) {
+            check(launch_init_wake(d, dt, 1, nullptr, stream), "init wake");
+            first_done = true;
+            nz_rows = 0;
+        }
+        // update_before_solving (update_data.rs:28-42)
+        check(launch_wake_step(d, dt, stream), "wake step");
+        cur ^= 1;
+        d = inst();
+        check(launch_panels(d, d.pts, 1, stream), "panels");
+        scur ^= 1;
+        nz_rows = nz_rows == 0 ? 0 : (nz_rows + 1 < M ? nz_rows + 1 : M);
+        d = inst();
+        check(hipEventRecord(ev[1], stream), "event");
+        // FrozenWake::update_from_full_wake (frozen_wake.rs:189-261)
+        ensure_fold_buffers();
+        check(launch_fixed(d, mode, &fb, stream), "fixed velocities");
+        check(hipEventRecord(ev[2], stream), "event");
+        check(launch_factors(d, stream), "factors");
+        check(hipEventRecord(ev[3], stream), "event");
+        // SimpleIterative::solve, then update_wing_strength
+        check(launch_solve(d, stream), "solve");
+        nz_rows = nz_rows < 1 ? 1 : nz_rows;
+        d = inst();
+        // update_velocity_at_points (update_data.rs:125-141)
+        check(launch_velocity_copy(d, stream), "velocity copy");
+        check(hipEventRecord(ev[4], stream), "event");
+        long end = d.end_index < nr_points ? d.end_index : nr_points;
+        pairs[0] = 0;
+        if (end > 0 && steps_completed > 2) {
+            check(launch_self_induction(d, mode, &mc, stream), "self induction");
+            pairs[0] = (double)end * (double)N * (double)nz_rows;
+        }
+        steps_completed += 1;
+        pairs[1] = (double)N * (double)N * (double)(nz_rows > 1 ? nz_rows - 1 : 0);
+        pairs[2] = (double)N * (double)N;
+        check(hipEventRecord(ev[5], stream), "event");
+        check(launch_forces(d, dt, stream), "forces");
+        check(hipEventRecord(ev[6], stream), "event");
+        check(hipMemcpyAsync(h_out.data(), d_out, sizeof(double) * h_out.size(), hipMemcpyDeviceToHost, stream),
+              "download");
+        check(hipStreamSynchronize(stream), "step");
+        static const int order[kTimers][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
+        for (int k = 0; k < kTimers; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[order[k][0]], ev[order[k][1]]);
+            timer_ms[k] = ms;
+        }
+        write_result(time, out);
+    }
+
+    void write_result(double time, sb_step_result* o) const {
+        if (!o) return;
+        const double* h = h_out.data();
+        o->time = time;
+        o->iterations = (long)h[OutLayout::iters(N, W)];
+        o->residual = h[OutLayout::residual(N, W)];
+        auto cp = [](double* dst, const double* src, size_t n) {
+            if (dst) memcpy(dst, src, sizeof(double) * n);
+        };
+        cp(o->circulation_strength, h + OutLayout::gamma(N), N);
+        cp(o->angles_of_attack, h + OutLayout::alpha(N), N);
+        cp(o->ctrl_points_velocity, h + OutLayout::u(N), 3 * N);
+        cp(o->solver_input_velocity, h + OutLayout::felt(N), 3 * N);
+        cp(o->ctrl_points, h + OutLayout::ctrl(N), 3 * N);
+        cp(o->sectional_forces, h + OutLayout::sect(N), 18 * N);
+        cp(o->integrated_forces, h + OutLayout::integ_f(N), 18 * W);
+        cp(o->integrated_moments, h + OutLayout::integ_m(N, W), 18 * W);
+    }
+
+    // Near-field and total pair counts of the free-wake self-induction at the
+    // current state (the sum update_velocity_at_points would evaluate).
+    void count_pairs(double* near_total) {
+        check(hipSetDevice(device), "hipSetDevice");
+        long end = setup.wake.end_index < nr_points ? setup.wake.end_index : nr_points;
+        long blocks = 0;
+        double* buf = tmp(2 * ((end + 255) / 256) + 2);
+        Inst d = inst();
+        check(launch_count_pairs(d, d.pts, end, (long)N * nz_rows, buf, &blocks, stream), "count pairs");
+        std::vector<double> h(2 * blocks);
+        if (blocks)
+            check(hipMemcpyAsync(h.data(), buf, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, stream), "download");
+        check(hipStreamSynchronize(stream), "count pairs");
+        double a = 0.0, b = 0.0;
+        for (long q = 0; q < blocks; ++q) {
+            a += h[2 * q];
+            b += h[2 * q + 1];
+        }
+        near_total[0] = a;
+        near_total[1] = b;
+    }
+
+    double* tmp(size_t n) {
+        if (n > tmp_cap) {
+            if (d_tmp) (void)hipFree(d_tmp);
+            d_tmp = dalloc<double>(n);
+            tmp_cap = n;
+        }
+        return d_tmp;
+    }
+
+    void induced(const double* pts, size_t n, double* out) {
+        check(hipSetDevice(device), "hipSetDevice");
+        if (n == 0) return;
+        double* buf = tmp(6 * n);
+        check(hipMemcpyAsync(buf, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream), "upload points");
+        Inst d = inst();
+        McBuffers* m = (long)((n + 255) / 256) <= mc.cap_jobs ? &mc : nullptr;
+        if (!m && n > 2048 && balanced) {  // larger than the wake: scratch sized for n
+            if (!tmp_mc.partial || tmp_mc.cap_jobs < (long)((n + 255) / 256)) {
+                if (tmp_mc.partial) (void)hipFree(tmp_mc.partial);
+                if (tmp_mc.flags) (void)hipFree(tmp_mc.flags);
+                tmp_mc.cap_jobs = (long)((n + 255) / 256);
+                tmp_mc.partial = dalloc<double>(tmp_mc.cap_jobs * 256 * 3);
+                tmp_mc.flags = dalloc<long long>(tmp_mc.cap_jobs);
+                check(hipMemsetAsync(tmp_mc.flags, 0xff, sizeof(long long) * tmp_mc.cap_jobs, stream), "memset");
+                tmp_mc.epoch = 0;
+            }
+            m = &tmp_mc;
+        }
+        check(launch_induced_targets(d, buf, (long)n, (long)N * nz_rows, mode, buf + 3 * n, m, stream), "induced");
+        check(hipMemcpyAsync(out, buf + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, stream), "download");
+        check(hipStreamSynchronize(stream), "induced");
+    }
+};
+
+namespace sb_host {
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const HipError& e) {
+        g_error = e.what();
+        return -5;
+    } catch (const std::invalid_argument& e) {
+        g_error = e.what();
+        return -3;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -4;
+    }
+}
+
+}  // namespace sb_host

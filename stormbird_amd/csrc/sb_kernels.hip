@@ -450,8 +450,7 @@ __global__ __launch_bounds__(64) void k_fold_terms(const double* __restrict__ te
 }
 
 // frozen_wake.rs:198-261: F[j][i] = unit velocity of first-row panel j at ctrl point i.
-__global__ void k_factors(Inst d) {
-    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void factors_body(const Inst& d, long idx) {
     long NN = (long)d.N * d.N;
     if (idx >= NN) return;
     int j = (int)(idx / d.N);  // panel (column)
@@ -464,6 +463,7 @@ __global__ void k_factors(Inst d) {
     }
     store3(d.F + 3 * idx, u);
 }
+__global__ void k_factors(Inst d) { factors_body(d, (long)blockIdx.x * blockDim.x + threadIdx.x); }
 
 // ---------------------------------------------------------------------------
 // Wake shedding and convection
@@ -472,7 +472,7 @@ __global__ void k_factors(Inst d) {
 // Felt ctrl-point velocity (force_calculations.rs:50-69), its span-point map
 // (value_mapping.rs:87-137) and, at the first step, the mean ctrl freestream
 // and flow-derivative history (simulation.rs:65-94).
-__global__ void k_prepare(Inst d, int first_step) {
+__device__ inline void prepare_body(const Inst& d, int first_step) {
     int t = threadIdx.x;
     const double* rb = d.geo + d.gl.rbm;
     V3 trans = load3(rb), vlin = load3(rb + 6), vang = load3(rb + 9);
@@ -502,6 +502,7 @@ __global__ void k_prepare(Inst d, int first_step) {
         }
     }
 }
+__global__ void k_prepare(Inst d, int first_step) { prepare_body(d, first_step); }
 
 // initialization.rs:12-41: M+1 sweeps of update_wake_points_before_solving
 // with a uniform velocity v and freestream v.  Each column evolves on its
@@ -510,13 +511,12 @@ __global__ void k_prepare(Inst d, int first_step) {
 //   p_M = p_{M-1} + last_len * normalize(p_{M-1} - p_{M-2})
 // bit for bit (row s is final after sweep s); with damping the sweeps are
 // replayed literally.
-__global__ void k_init_wake(Inst d, double dt, int use_mean, double vx, double vy, double vz, int fill) {
+__device__ inline void init_wake_body(const Inst& d, double dt, int use_mean, V3 vin, int fill, long g, long stride) {
     long P = d.P, M = d.M;
     long npts = P * (M + 1);
-    V3 v = use_mean ? load3(d.scal) : V3{vx, vy, vz};
-    long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    for (long q = g; q < npts; q += (long)gridDim.x * blockDim.x) store3(d.vel + 3 * q, v);
-    for (long q = g; q < (long)d.N * M; q += (long)gridDim.x * blockDim.x) d.str[q] = 0.0;
+    V3 v = use_mean ? load3(d.scal) : vin;
+    for (long q = g; q < npts; q += stride) store3(d.vel + 3 * q, v);
+    for (long q = g; q < (long)d.N * M; q += stride) d.str[q] = 0.0;
     if (g >= P) return;
     int j = (int)g;
     double first_len = d.rep_chord * d.first_rel;
@@ -560,11 +560,14 @@ __global__ void k_init_wake(Inst d, double dt, int use_mean, double vx, double v
         }
     }
 }
+__global__ void k_init_wake(Inst d, double dt, int use_mean, double vx, double vy, double vz, int fill) {
+    init_wake_body(d, dt, use_mean, V3{vx, vy, vz}, fill, (long)blockIdx.x * blockDim.x + threadIdx.x,
+                   (long)gridDim.x * blockDim.x);
+}
 
 // update_data.rs:136-150 + 200-252 + 295-316: rows 0..M of the next points.
-__global__ void k_wake_step(Inst d, double dt) {
+__device__ inline void wake_step_body(const Inst& d, double dt, long q) {
     long P = d.P, M = d.M;
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= P * (M + 1)) return;
     long s = q / P;
     int j = (int)(q - s * P);
@@ -586,11 +589,12 @@ __global__ void k_wake_step(Inst d, double dt) {
     }
     store3(d.pts_next + 3 * q, out);
 }
+__global__ void k_wake_step(Inst d, double dt) { wake_step_body(d, dt, (long)blockIdx.x * blockDim.x + threadIdx.x); }
 
 // update_data.rs:262-292: last row from the two rows before it.
-__global__ void k_wake_last(Inst d) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.P) return;
+__device__ inline void wake_last_body(const Inst& d, long jj) {
+    if (jj >= d.P) return;
+    int j = (int)jj;
     long P = d.P, M = d.M;
     double last_len = d.last_rel * d.rep_chord;
     if (M > 1) {
@@ -602,11 +606,11 @@ __global__ void k_wake_last(Inst d) {
         store3(d.pts_next + 3 * (P + j), a + (last_len * normalize(load3(d.felt_span + 3 * j))));
     }
 }
+__global__ void k_wake_last(Inst d) { wake_last_body(d, (long)blockIdx.x * blockDim.x + threadIdx.x); }
 
 // update_data.rs:102-115 + panel.rs:26-50 + geometry_functions.rs:11-30, and
 // the strength shift of update_data.rs:319-328 (into str_next).
-__global__ void k_panels(Inst d, int shift) {
-    long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void panels_body(const Inst& d, int shift, long k) {
     long np = (long)d.N * d.M;
     if (k >= np) return;
     long s_row = k / d.N;
@@ -647,6 +651,7 @@ __global__ void k_panels(Inst d, int shift) {
     r.strength = g;
     d.rec[k] = r;
 }
+__global__ void k_panels(Inst d, int shift) { panels_body(d, shift, (long)blockIdx.x * blockDim.x + threadIdx.x); }
 
 // ---------------------------------------------------------------------------
 // Fixed-point solve (simple_iterative.rs:86-212), one workgroup
@@ -676,8 +681,7 @@ __device__ inline double circulation(const Strip& s, int viscous, double a, V3 u
     return -0.5 * s.c * len(u) * cl;
 }
 
-__global__ __launch_bounds__(1024) void k_solve(Inst d) {
-    extern __shared__ double sh[];
+__device__ inline void solve_body(const Inst& d, double* sh) {
     int N = d.N;
     double* G = sh;             // [N] current circulation (broadcast for the matvec)
     double* R = sh + N;         // [N] residual terms
@@ -789,12 +793,16 @@ __global__ __launch_bounds__(1024) void k_solve(Inst d) {
         d.out[OutLayout::residual(N, d.W)] = ctl[0];
     }
 }
+__global__ __launch_bounds__(1024) void k_solve(Inst d) {
+    extern __shared__ double sh[];
+    solve_body(d, sh);
+}
 
 // ---------------------------------------------------------------------------
 // Forces (force_calculations.rs:322-548, forces_and_moments.rs:163-245,
 // flow_derivatives.rs:31-62), one workgroup
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_forces(Inst d, double dt) {
+__device__ inline void forces_body(const Inst& d, double dt) {
     int N = d.N, W = d.W;
     const double* geo = d.geo;
     double* o = d.out;
@@ -888,6 +896,7 @@ __global__ __launch_bounds__(1024) void k_forces(Inst d, double dt) {
     }
     if (i == 0) d.scal[3] = count + 1.0;
 }
+__global__ __launch_bounds__(1024) void k_forces(Inst d, double dt) { forces_body(d, dt); }
 
 __global__ void k_copy3(const double* __restrict__ src, double* __restrict__ dst, long n) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -907,6 +916,168 @@ __global__ void k_libm(int fn, const double* x, const double* y, double* out, lo
         default: r = sbm::log(x[q]); break;
     }
     out[q] = r;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched instances
+// ---------------------------------------------------------------------------
+
+// Owner of flat block b: the last instance i with off[i] <= b.
+__device__ inline int owner_of(const long* off, int n, long b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void kb_prepare(const Inst* I, int first_step) {
+    const Inst d = I[blockIdx.x];
+    prepare_body(d, first_step);
+}
+
+__global__ void kb_init_wake(const Inst* I, const long* off, int n) {
+    long b = blockIdx.x;
+    int i = owner_of(off, n, b);
+    const Inst d = I[i];
+    long nb = off[i + 1] - off[i];
+    init_wake_body(d, d.dt, 1, V3{0.0, 0.0, 0.0}, 0, (b - off[i]) * blockDim.x + threadIdx.x, nb * blockDim.x);
+}
+
+__global__ void kb_wake_step(const Inst* I, const long* off, int n) {
+    long b = blockIdx.x;
+    int i = owner_of(off, n, b);
+    const Inst d = I[i];
+    wake_step_body(d, d.dt, (b - off[i]) * blockDim.x + threadIdx.x);
+}
+
+__global__ void kb_wake_last(const Inst* I, const long* off, int n) {
+    long b = blockIdx.x;
+    int i = owner_of(off, n, b);
+    const Inst d = I[i];
+    wake_last_body(d, (b - off[i]) * blockDim.x + threadIdx.x);
+}
+
+__global__ void kb_panels(const Inst* I, const long* off, int n, int shift) {
+    long b = blockIdx.x;
+    int i = owner_of(off, n, b);
+    const Inst d = I[i];
+    panels_body(d, shift, (b - off[i]) * blockDim.x + threadIdx.x);
+}
+
+__global__ void kb_factors(const Inst* I, const long* off, int n) {
+    long b = blockIdx.x;
+    int i = owner_of(off, n, b);
+    const Inst d = I[i];
+    factors_body(d, (b - off[i]) * blockDim.x + threadIdx.x);
+}
+
+__global__ void kb_velocity_copy(const Inst* I, const long* off, int n) {
+    long b = blockIdx.x;
+    int i = owner_of(off, n, b);
+    const Inst d = I[i];
+    long q = (b - off[i]) * blockDim.x + threadIdx.x;
+    if (q < 3L * d.P * (d.M + 1)) d.vel[q] = d.fs[3L * d.N + q];
+}
+
+__global__ __launch_bounds__(1024) void kb_solve(const Inst* I) {
+    extern __shared__ double sh[];
+    const Inst d = I[blockIdx.x];
+    solve_body(d, sh);
+}
+
+__global__ __launch_bounds__(1024) void kb_forces(const Inst* I) {
+    const Inst d = I[blockIdx.x];
+    forces_body(d, d.dt);
+}
+
+__global__ void kb_zero_counters(unsigned* c) {
+    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
+}
+
+__device__ inline int next_job(unsigned* head, int* slot) {
+    __syncthreads();  // the previous job's readers of *slot are done
+    if (threadIdx.x == 0) *slot = (int)__hip_atomic_fetch_add(head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(*slot);
+}
+
+// Fixed velocities of many instances (frozen_wake.rs:189-261).  Job = one
+// instance's 64 control points (lane = target) over its panels of rows >= 1.
+// The four waves evaluate different panels of each 16-panel block into LDS;
+// waves 0-2 then fold component x/y/z of the block's terms in ascending panel
+// order, so each target sees the same sequence of additions as the one-target
+// fold (a skipped panel adds +0, which leaves a sum that started at +0
+// unchanged).  Jobs are taken longest first from a queue by persistent
+// workgroups.
+constexpr int COOP_PB = 4;
+
+template <int MODE, int SYM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_fixed(
+    const Inst* I, const int2* __restrict__ jobs, int n_jobs, unsigned* head) {
+    __shared__ double T[4 * COOP_PB][3][WAVE];
+    __shared__ int slot;
+    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (;;) {
+        int j = next_job(head, &slot);
+        if (j >= n_jobs) break;
+        int2 jb = jobs[j];
+        const Inst d = I[jb.x];
+        int t = jb.y * WAVE + lane;
+        int tc = t < d.N ? t : d.N - 1;
+        V3 x = load3(d.geo + d.gl.ctrl + 3 * tc);
+        int wt = d.neglect_self ? d.wing_of[tc] : -1;
+        long p_begin = d.N, p_end = (long)d.N * d.nz_rows;
+        double acc = 0.0;
+        for (long base = p_begin; base < p_end; base += 4 * COOP_PB) {
+            for (int q = 0; q < COOP_PB; ++q) {
+                long k = base + wave * COOP_PB + q;
+                V3 term{0.0, 0.0, 0.0};
+                if (k < p_end) {
+                    int kk = (int)k;
+                    PanelRec r = d.rec[kk];
+                    bool skip = r.strength == 0.0 || (d.neglect_self && d.wing_of[kk % d.N] == wt);
+                    if (!skip) term = pair_unit<MODE, SYM>(d, r, kk, x) * r.strength;
+                }
+                T[wave * COOP_PB + q][0][lane] = term.x;
+                T[wave * COOP_PB + q][1][lane] = term.y;
+                T[wave * COOP_PB + q][2][lane] = term.z;
+            }
+            __syncthreads();
+            if (wave < 3) {
+                int cnt = p_end - base < 4 * COOP_PB ? (int)(p_end - base) : 4 * COOP_PB;
+                for (int q = 0; q < cnt; ++q) acc = acc + T[q][wave][lane];
+            }
+            __syncthreads();
+        }
+        if (wave < 3 && t < d.N) d.fixed[3 * t + wave] = acc;
+    }
+}
+
+// Free-wake self-induction of many instances (update_data.rs:125-141): job =
+// one instance's 256 wake points over all its panels, one target per lane,
+// panel records staged through LDS as in k_targets.
+template <int MODE, int SYM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_targets(
+    const Inst* I, const int2* __restrict__ jobs, int n_jobs, unsigned* head) {
+    __shared__ PanelRec tile[TILE];
+    __shared__ int slot;
+    for (;;) {
+        int j = next_job(head, &slot);
+        if (j >= n_jobs) break;
+        int2 jb = jobs[j];
+        const Inst d = I[jb.x];
+        long np = (long)d.P * (d.M + 1);
+        long end = d.end_index < np ? d.end_index : np;
+        long t = (long)jb.y * 256 + threadIdx.x;
+        long tc = t < end ? t : end - 1;
+        V3 x = load3(d.pts + 3 * tc);
+        V3 acc = sweep<MODE, SYM>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, (long)d.N * d.nz_rows);
+        if (t < end) store3(d.vel + 3 * t, load3(d.vel + 3 * t) + acc);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1087,6 +1258,104 @@ hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targe
 hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st) {
     hipLaunchKernelGGL(k_libm, dim3(blocks_for(n, 256)), dim3(256), 0, st, fn, x, y, out, n);
     return hipGetLastError();
+}
+
+
+// ---- batched launchers ----
+hipError_t launch_batch_prepare(const BatchDev& b, int first_step, hipStream_t st) {
+    hipLaunchKernelGGL(kb_prepare, dim3(b.n), dim3(256), 0, st, b.inst, first_step);
+    return hipGetLastError();
+}
+
+static const long* boff(const BatchDev& b, int kind) { return b.off + (long)kind * (b.n + 1); }
+
+hipError_t launch_batch_init_wake(const BatchDev& b, hipStream_t st) {
+    if (b.blocks[BK_POINTS] > 0)
+        hipLaunchKernelGGL(kb_init_wake, dim3((unsigned)b.blocks[BK_POINTS]), dim3(256), 0, st, b.inst,
+                           boff(b, BK_POINTS), b.n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_wake_step(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(kb_wake_step, dim3((unsigned)b.blocks[BK_POINTS]), dim3(256), 0, st, b.inst,
+                       boff(b, BK_POINTS), b.n);
+    hipLaunchKernelGGL(kb_wake_last, dim3((unsigned)b.blocks[BK_SPAN]), dim3(256), 0, st, b.inst, boff(b, BK_SPAN),
+                       b.n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_panels(const BatchDev& b, int shift, hipStream_t st) {
+    hipLaunchKernelGGL(kb_panels, dim3((unsigned)b.blocks[BK_PANELS]), dim3(256), 0, st, b.inst,
+                       boff(b, BK_PANELS), b.n, shift);
+    return hipGetLastError();
+}
+
+template <int MODE, int SYM>
+static void launch_kb_fixed(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL((kb_fixed<MODE, SYM>), dim3(b.fixed_grid), dim3(256), 0, st, b.inst, b.fixed_jobs,
+                       b.n_fixed_jobs, b.counters);
+}
+
+hipError_t launch_batch_fixed(const BatchDev& b, int mode, hipStream_t st) {
+    if (b.n_fixed_jobs <= 0) return hipSuccess;
+    if (mode == MODE_FAST) b.sym ? launch_kb_fixed<MODE_FAST, 1>(b, st) : launch_kb_fixed<MODE_FAST, 0>(b, st);
+    else b.sym ? launch_kb_fixed<MODE_EXACT, 1>(b, st) : launch_kb_fixed<MODE_EXACT, 0>(b, st);
+    hipLaunchKernelGGL(kb_zero_counters, dim3(1), dim3(64), 0, st, b.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_factors(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(kb_factors, dim3((unsigned)b.blocks[BK_FACTORS]), dim3(256), 0, st, b.inst,
+                       boff(b, BK_FACTORS), b.n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_solve(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(kb_solve, dim3(b.n), dim3(b.max_threads), b.solve_shm, st, b.inst);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_velocity_copy(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(kb_velocity_copy, dim3((unsigned)b.blocks[BK_VEL]), dim3(256), 0, st, b.inst,
+                       boff(b, BK_VEL), b.n);
+    return hipGetLastError();
+}
+
+template <int MODE, int SYM>
+static void launch_kb_targets(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL((kb_targets<MODE, SYM>), dim3(b.target_grid), dim3(256), 0, st, b.inst, b.target_jobs,
+                       b.n_target_jobs, b.counters + 1);
+}
+
+hipError_t launch_batch_self_induction(const BatchDev& b, int mode, hipStream_t st) {
+    if (b.n_target_jobs <= 0) return hipSuccess;
+    if (mode == MODE_FAST) b.sym ? launch_kb_targets<MODE_FAST, 1>(b, st) : launch_kb_targets<MODE_FAST, 0>(b, st);
+    else b.sym ? launch_kb_targets<MODE_EXACT, 1>(b, st) : launch_kb_targets<MODE_EXACT, 0>(b, st);
+    hipLaunchKernelGGL(kb_zero_counters, dim3(1), dim3(64), 0, st, b.counters);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_forces(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(kb_forces, dim3(b.n), dim3(b.max_threads), 0, st, b.inst);
+    return hipGetLastError();
+}
+
+template <class K>
+static int resident_groups(K kernel) {
+    int per_cu = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return per_cu * cus;
+}
+
+int batch_resident_groups(int which, int mode, int sym) {
+    if (which == 0) {
+        if (mode == MODE_FAST) return sym ? resident_groups(kb_fixed<MODE_FAST, 1>) : resident_groups(kb_fixed<MODE_FAST, 0>);
+        return sym ? resident_groups(kb_fixed<MODE_EXACT, 1>) : resident_groups(kb_fixed<MODE_EXACT, 0>);
+    }
+    if (mode == MODE_FAST) return sym ? resident_groups(kb_targets<MODE_FAST, 1>) : resident_groups(kb_targets<MODE_FAST, 0>);
+    return sym ? resident_groups(kb_targets<MODE_EXACT, 1>) : resident_groups(kb_targets<MODE_EXACT, 0>);
 }
 
 }  // namespace sb

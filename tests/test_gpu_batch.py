@@ -1,0 +1,96 @@
+"""GPU: many instances stepped together (sb_batch_*) against each instance stepped alone.
+
+Bar: bit-identical results and wake states per instance and per step, in
+EXACT and FAST mode, over uneven instances (strips, wings, wake rows, time
+steps, free and convected wakes, symmetry), across the first step (wake
+initialisation), the start of the free-wake self-induction and several
+replays of the captured step graph.  One instance is also held to the
+oracle directly.
+"""
+import numpy as np
+import pytest
+
+import stormbird_amd as sb
+from stormbird_amd import setups as S
+
+from test_gpu_parity import compare_results, same
+
+pytestmark = pytest.mark.gpu
+
+
+def mixed_setups(seed=7, n=10):
+    rng = np.random.default_rng(seed)
+    setups, cases = [], []
+    for _ in range(n):
+        js, case = S.polar_sweep_instance(rng, max_panels=40, min_panels=6)
+        setups.append(js)
+        cases.append(case)
+    # a free-wake instance and a symmetric one (self-induction and mirror paths)
+    js, case = S.polar_sweep_instance(rng, max_panels=30, min_panels=20)
+    d = __import__("json").loads(js)
+    d["simulation_settings"]["Dynamic"]["wake"]["ratio_of_wake_affected_by_induced_velocities"] = 1.0
+    setups.append(__import__("json").dumps(d))
+    cases.append(case)
+    js, case = S.polar_sweep_instance(rng, max_panels=25, min_panels=10)
+    d = __import__("json").loads(js)
+    d["simulation_settings"]["Dynamic"]["wake"]["symmetry_condition"] = "Z"
+    setups.append(__import__("json").dumps(d))
+    cases.append(case)
+    return setups, cases
+
+
+def run_pair(mode, steps):
+    setups, cases = mixed_setups()
+    batch = sb.Batch(setups, mode=mode)
+    alone = [sb.Simulation(js, mode=mode) for js in setups]
+    fs_parts = [S.uniform_freestream(s.nr_freestream_points(), c["velocity"]) for s, c in zip(alone, cases)]
+    fs_all = np.concatenate(fs_parts)
+    assert fs_all.shape[0] == batch.nr_freestream_points
+    dts = np.array([c["time_step"] for c in cases])
+    for k in range(steps):
+        print(f"batch step {k} mode {mode}", flush=True)
+        rb = batch.do_step(times=k * dts, time_steps=dts, freestream_velocity=fs_all)
+        for i, sim in enumerate(alone):
+            ra = sim.do_step(time=k * dts[i], time_step=dts[i], freestream_velocity=fs_parts[i])
+            compare_results(rb[i], ra, exact=True, where=f"step {k} instance {i}")
+        if k in (0, 3, steps - 1):
+            for i, sim in enumerate(alone):
+                wb, wa = batch.instances[i].get_wake_state(), sim.get_wake_state()
+                for key in ("points", "velocities", "strengths"):
+                    assert same(wb[key], wa[key]), f"step {k} instance {i}: wake {key}"
+    return batch
+
+
+def test_batch_equals_alone_exact():
+    batch = run_pair(sb.MODE_EXACT, 9)
+    assert batch.last_step_ms() > 0.0
+
+
+def test_batch_equals_alone_fast():
+    run_pair(sb.MODE_FAST, 6)
+
+
+def test_batch_instance_matches_oracle(oracle):
+    rng = np.random.default_rng(11)
+    setups, cases = [], []
+    for _ in range(3):
+        js, case = S.polar_sweep_instance(rng, max_panels=20, min_panels=8)
+        setups.append(js)
+        cases.append(case)
+    batch = sb.Batch(setups)
+    orc = oracle.Simulation(setups[1], variant="pm")
+    parts = [S.uniform_freestream(s.nr_freestream_points(), c["velocity"]) for s, c in zip(batch.instances, cases)]
+    dts = np.array([c["time_step"] for c in cases])
+    for k in range(8):
+        rb = batch.do_step(times=k * dts, time_steps=dts, freestream_velocity=np.concatenate(parts))
+        ro = orc.do_step(time=k * dts[1], time_step=dts[1], freestream_velocity=parts[1])
+        compare_results(rb[1], ro, exact=True, where=f"step {k}")
+
+
+def test_batch_rejects_bad_input():
+    js, _ = S.polar_sweep_instance(np.random.default_rng(1), max_panels=10, min_panels=5)
+    with pytest.raises(ValueError):
+        sb.Batch([js, "{not json"])
+    batch = sb.Batch([js, js])
+    with pytest.raises(RuntimeError):
+        batch.do_step(times=None, time_steps=0.1, freestream_velocity=np.zeros((3, 3)))
