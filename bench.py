@@ -110,6 +110,99 @@ def cpu_baseline(js, state, n_panels_total, pairs_per_step, budget_s=12.0):
     }
 
 
+def develop_wake(sim, steps_completed, rng):
+    """Fills every wake row of `sim` with shed strengths (row-0 circulation x (1 + 2 % noise))
+    and perturbs the free rows, keeping the points and velocities it has."""
+    st = sim.get_wake_state()
+    n, P = sim.nr_ctrl_points, sim.nr_span_points
+    M = sim.nr_panels // n
+    pts = st["points"].reshape(M + 1, P, 3).copy()
+    rows = np.arange(M + 1, dtype=np.float64)[:, None, None]
+    pts[2:] += rng.normal(0.0, 1.0, pts[2:].shape) * 0.01 * np.sqrt(rows[2:])
+    g0 = st["strengths"][:n]
+    gam = np.tile(g0, (M, 1)) * (1.0 + 0.02 * rng.standard_normal((M, n)))
+    sim.set_wake_state(pts.reshape(-1, 3), st["velocities"], gam.ravel(), steps_completed)
+
+
+def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
+    """512 polar-sweep instances (seeded: 1-6 sails, 16-64 strips, chords 5-15 m, AR 2-6,
+    200-3000 wake rows, AWA 20-180 deg, AWS 6-18 m/s) from a developed wake: one batched
+    step (sb_batch, graph-replayed) against the plain path (each instance stepped alone)."""
+    import torch
+
+    import stormbird_amd as sb
+    from stormbird_amd import setups as S
+
+    rng = np.random.default_rng(seed)
+    made = [S.polar_sweep_instance(rng) for _ in range(n_inst)]
+    setups = [m[0] for m in made]
+    cases = [m[1] for m in made]
+    t_create = time.perf_counter()
+    batch = sb.Batch(setups, mode=sb.MODE_EXACT)
+    t_create = time.perf_counter() - t_create
+    dts = np.array([c["time_step"] for c in cases])
+    fs_all = np.concatenate([S.uniform_freestream(s.nr_freestream_points(), c["velocity"])
+                             for s, c in zip(batch.instances, cases)])
+    fs_dev = torch.from_numpy(fs_all).to("cuda")
+    n_pts = fs_all.shape[0]
+    offs = batch.freestream_offsets
+    for k in range(3):
+        batch.do_step_device(times=k * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(), n_points=n_pts,
+                             want_result=False)
+    drng = np.random.default_rng(seed + 100)
+    for s in batch.instances:
+        develop_wake(s, 3, drng)
+    n_ctrl = np.array([s.nr_ctrl_points for s in batch.instances], dtype=np.float64)
+    rows = np.array([s.nr_panels // s.nr_ctrl_points for s in batch.instances], dtype=np.float64)
+    pairs = float(np.sum(n_ctrl * n_ctrl * (rows - 1) + n_ctrl * n_ctrl))
+    torch.cuda.synchronize()
+    # plain: every instance stepped alone
+    k0 = 3
+    t0 = time.perf_counter()
+    for k in range(plain_steps):
+        for i, s in enumerate(batch.instances):
+            s.do_step_device(time=(k0 + k) * dts[i], time_step=dts[i],
+                             freestream_ptr=fs_dev.data_ptr() + 24 * int(offs[i]),
+                             n_points=int(offs[i + 1] - offs[i]), want_result=False)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    k0 += plain_steps
+    # batched: first call captures the step graph
+    batch.do_step_device(times=k0 * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(), n_points=n_pts,
+                         want_result=False)
+    k0 += 1
+    torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        batch.do_step_device(times=(k0 + k) * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(),
+                             n_points=n_pts, want_result=True)
+        dev_ms.append(batch.last_step_ms())
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    plain_rate = n_inst * plain_steps / t_plain
+    batch_rate = n_inst * steps / t_batch
+    return {
+        "workload": (f"{n_inst} polar-sweep instances (seed {seed}): 1-6 sails, 16-64 strips per sail, chords "
+                     "5-15 m, AR 2-6, 200-3000 wake rows, AWA 20-180 deg, AWS 6-18 m/s, convected wakes, "
+                     "developed-wake steady state"),
+        "n_instances": n_inst,
+        "n_ctrl_points_total": int(n_ctrl.sum()),
+        "biot_savart_pairs_per_batched_step": pairs,
+        "batched_instance_steps_per_s": batch_rate,
+        "batched_ms_per_step": 1000.0 * t_batch / steps,
+        "batched_device_ms_per_step": float(np.mean(dev_ms)),
+        "batched_pairs_per_s": pairs * steps / t_batch,
+        "plain_instance_steps_per_s": plain_rate,
+        "plain_ms_per_sweep_step": 1000.0 * t_plain / plain_steps,
+        "batched_vs_plain": batch_rate / plain_rate,
+        "steps": steps,
+        "plain_steps": plain_steps,
+        "create_s": t_create,
+        "warm_start": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +212,8 @@ def main():
     ap.add_argument("--strips", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fast", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU layout (target points "
@@ -213,6 +308,9 @@ def main():
         "fast_mode": fast,
         "cpu_baseline": cpu,
     }
+    if not args.no_batch:
+        del sim
+        line["batched"] = batched_polar_sweep(n_inst=args.batch_instances)
     print(json.dumps(line))
 
 

@@ -173,12 +173,12 @@ struct sb_sim {
     }
 
     // Fixed velocities of the single-instance step: per-pair terms of up to
-    // 32 Mi doubles (256 MiB) per chunk, allocated at the first step.
+    // 8 Mi doubles (64 MiB) per chunk, allocated at the first step.
     void ensure_fold_buffers() {
         if (fb.terms) return;
         long ld = (N + 63) / 64 * 64;
         long need = 3L * ld * (long)N * (M > 1 ? M - 1 : 1);
-        fb.cap = need < (32L << 20) ? need : (32L << 20);
+        fb.cap = need < (8L << 20) ? need : (8L << 20);
         fb.terms = dalloc<double>(fb.cap);
         fb.cap_targets = N;
         fb.acc = dalloc<double>(3L * N);

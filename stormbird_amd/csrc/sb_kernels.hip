@@ -1006,19 +1006,24 @@ __device__ inline int next_job(unsigned* head, int* slot) {
 }
 
 // Fixed velocities of many instances (frozen_wake.rs:189-261).  Job = one
-// instance's 64 control points (lane = target) over its panels of rows >= 1.
-// The four waves evaluate different panels of each 16-panel block into LDS;
-// waves 0-2 then fold component x/y/z of the block's terms in ascending panel
-// order, so each target sees the same sequence of additions as the one-target
-// fold (a skipped panel adds +0, which leaves a sum that started at +0
-// unchanged).  Jobs are taken longest first from a queue by persistent
-// workgroups.
-constexpr int COOP_PB = 4;
+// instance's 64 control points (lane = target) over its panels of rows >= 1,
+// run by one 16-wave workgroup (a whole CU): the waves evaluate different
+// panels of each 32-panel block into LDS, then waves 0-2 fold component
+// x/y/z of the block's terms in ascending panel order, so each target sees the
+// same sequence of additions as the one-target fold (a skipped panel adds +0,
+// which leaves a sum that started at +0 unchanged).  Sixteen waves per job
+// keep the longest job (the largest instance) short; jobs are taken longest
+// first from a queue by persistent workgroups.
+constexpr int COOP_WAVES = 16;
+constexpr int COOP_PB = 2;                        // panels per wave per block
+constexpr int COOP_BLOCK = COOP_WAVES * COOP_PB;  // 32
+constexpr int COOP_TILE = 128;                    // panel records staged per tile (10 KiB)
 
 template <int MODE, int SYM>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_fixed(
-    const Inst* I, const int2* __restrict__ jobs, int n_jobs, unsigned* head) {
-    __shared__ double T[4 * COOP_PB][3][WAVE];
+__global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __restrict__ jobs, int n_jobs,
+                                                 unsigned* head) {
+    __shared__ double T[COOP_BLOCK][3][WAVE];
+    __shared__ PanelRec tile[COOP_TILE];
     __shared__ int slot;
     const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     for (;;) {
@@ -1032,26 +1037,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         int wt = d.neglect_self ? d.wing_of[tc] : -1;
         long p_begin = d.N, p_end = (long)d.N * d.nz_rows;
         double acc = 0.0;
-        for (long base = p_begin; base < p_end; base += 4 * COOP_PB) {
-            for (int q = 0; q < COOP_PB; ++q) {
-                long k = base + wave * COOP_PB + q;
-                V3 term{0.0, 0.0, 0.0};
-                if (k < p_end) {
-                    int kk = (int)k;
-                    PanelRec r = d.rec[kk];
-                    bool skip = r.strength == 0.0 || (d.neglect_self && d.wing_of[kk % d.N] == wt);
-                    if (!skip) term = pair_unit<MODE, SYM>(d, r, kk, x) * r.strength;
+        for (long tb = p_begin; tb < p_end; tb += COOP_TILE) {
+            int tcnt = (int)(p_end - tb < COOP_TILE ? p_end - tb : COOP_TILE);
+            __syncthreads();
+            const double* src = reinterpret_cast<const double*>(d.rec + tb);
+            double* dst = reinterpret_cast<double*>(tile);
+            for (int q = threadIdx.x; q < tcnt * RECD; q += blockDim.x) dst[q] = src[q];
+            __syncthreads();
+            for (int base = 0; base < tcnt; base += COOP_BLOCK) {
+                for (int q = 0; q < COOP_PB; ++q) {
+                    int kl = base + wave * COOP_PB + q;
+                    V3 term{0.0, 0.0, 0.0};
+                    if (kl < tcnt) {
+                        const PanelRec& r = tile[kl];
+                        int kk = (int)(tb + kl);
+                        bool skip = r.strength == 0.0 || (d.neglect_self && d.wing_of[kk % d.N] == wt);
+                        if (!skip) term = pair_unit<MODE, SYM>(d, r, kk, x) * r.strength;
+                    }
+                    T[wave * COOP_PB + q][0][lane] = term.x;
+                    T[wave * COOP_PB + q][1][lane] = term.y;
+                    T[wave * COOP_PB + q][2][lane] = term.z;
                 }
-                T[wave * COOP_PB + q][0][lane] = term.x;
-                T[wave * COOP_PB + q][1][lane] = term.y;
-                T[wave * COOP_PB + q][2][lane] = term.z;
+                __syncthreads();
+                if (wave < 3) {
+                    int cnt = tcnt - base < COOP_BLOCK ? tcnt - base : COOP_BLOCK;
+                    for (int q = 0; q < cnt; ++q) acc = acc + T[q][wave][lane];
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            if (wave < 3) {
-                int cnt = p_end - base < 4 * COOP_PB ? (int)(p_end - base) : 4 * COOP_PB;
-                for (int q = 0; q < cnt; ++q) acc = acc + T[q][wave][lane];
-            }
-            __syncthreads();
         }
         if (wave < 3 && t < d.N) d.fixed[3 * t + wave] = acc;
     }
@@ -1292,7 +1305,7 @@ hipError_t launch_batch_panels(const BatchDev& b, int shift, hipStream_t st) {
 
 template <int MODE, int SYM>
 static void launch_kb_fixed(const BatchDev& b, hipStream_t st) {
-    hipLaunchKernelGGL((kb_fixed<MODE, SYM>), dim3(b.fixed_grid), dim3(256), 0, st, b.inst, b.fixed_jobs,
+    hipLaunchKernelGGL((kb_fixed<MODE, SYM>), dim3(b.fixed_grid), dim3(COOP_WAVES * WAVE), 0, st, b.inst, b.fixed_jobs,
                        b.n_fixed_jobs, b.counters);
 }
 
@@ -1341,21 +1354,24 @@ hipError_t launch_batch_forces(const BatchDev& b, hipStream_t st) {
 }
 
 template <class K>
-static int resident_groups(K kernel) {
+static int resident_groups(K kernel, int threads) {
     int per_cu = 0, cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     return per_cu * cus;
 }
 
 int batch_resident_groups(int which, int mode, int sym) {
     if (which == 0) {
-        if (mode == MODE_FAST) return sym ? resident_groups(kb_fixed<MODE_FAST, 1>) : resident_groups(kb_fixed<MODE_FAST, 0>);
-        return sym ? resident_groups(kb_fixed<MODE_EXACT, 1>) : resident_groups(kb_fixed<MODE_EXACT, 0>);
+        const int th = COOP_WAVES * WAVE;
+        if (mode == MODE_FAST)
+            return sym ? resident_groups(kb_fixed<MODE_FAST, 1>, th) : resident_groups(kb_fixed<MODE_FAST, 0>, th);
+        return sym ? resident_groups(kb_fixed<MODE_EXACT, 1>, th) : resident_groups(kb_fixed<MODE_EXACT, 0>, th);
     }
-    if (mode == MODE_FAST) return sym ? resident_groups(kb_targets<MODE_FAST, 1>) : resident_groups(kb_targets<MODE_FAST, 0>);
-    return sym ? resident_groups(kb_targets<MODE_EXACT, 1>) : resident_groups(kb_targets<MODE_EXACT, 0>);
+    if (mode == MODE_FAST)
+        return sym ? resident_groups(kb_targets<MODE_FAST, 1>, 256) : resident_groups(kb_targets<MODE_FAST, 0>, 256);
+    return sym ? resident_groups(kb_targets<MODE_EXACT, 1>, 256) : resident_groups(kb_targets<MODE_EXACT, 0>, 256);
 }
 
 }  // namespace sb
