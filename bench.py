@@ -259,6 +259,32 @@ def main():
                 "self_induction_tflops": flops / (float(np.mean(kf)) * 1e-3) / 1e12}
         del simf
 
+    fp32 = None
+    if not args.no_fast:
+        # FP32-Kahan: one step from the developed state against the EXACT step from the same
+        # state (EXACT equals the f64 restatement bit for bit), then its step rate.
+        refs = {}
+        for md in (sb.MODE_EXACT, sb.MODE_FP32_KAHAN):
+            s_ = sb.Simulation(js, mode=md)
+            s_.set_wake_state(pts, vel, gam, 2000)
+            s_.previous_circulation_strength = prev
+            r_ = s_.do_step(time=2000 * dt, time_step=dt, freestream_velocity=fs)
+            refs[md] = (s_, r_.as_arrays(), s_.get_wake_state()["velocities"])
+        s32, a32, v32 = refs[sb.MODE_FP32_KAHAN]
+        _, a64, v64 = refs[sb.MODE_EXACT]
+
+        def rel(a, b):
+            return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+        e32, k32 = time_gpu(s32, fs_dev.data_ptr(), n_points, dt, args.steps, args.warmup, 2001)
+        fp32 = {"time_steps_per_s": args.steps / e32, "ms_per_step": 1000.0 * e32 / args.steps,
+                "self_induction_ms": float(np.mean(k32)),
+                "max_rel_dev_vs_f64": {"circulation": rel(a32["circulation_strength"], a64["circulation_strength"]),
+                                       "integrated_forces": rel(a32["integrated_forces"], a64["integrated_forces"]),
+                                       "wake_velocities": rel(v32, v64)},
+                "deviation_measured": "one step from the developed state, against the EXACT (f64) step"}
+        del refs, s32
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "self_induction_traffic.json")
     if os.path.exists(tpath):
@@ -306,6 +332,7 @@ def main():
         "kernel_ms": {k: sim.last_kernel_ms(k) for k in
                       ("wake_update", "fixed", "factors", "solve", "self_induction", "forces")},
         "fast_mode": fast,
+        "fp32_kahan_mode": fp32,
         "cpu_baseline": cpu,
     }
     if not args.no_batch:

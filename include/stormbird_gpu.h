@@ -55,8 +55,12 @@ typedef struct sb_step_result {
  *   equal the f64 CPU restatement bit for bit.
  * SB_MODE_FAST: fused multiply-add and reciprocal square root in the pair
  *   kernels; per-step deviation from the f64 restatement ~1e-13 relative.
- * SB_MODE_FP32_KAHAN: pair terms in fp32 with compensated (Kahan) per-target
- *   accumulation; solver, geometry and wake convection stay f64. */
+ * SB_MODE_FP32_KAHAN: pair terms in fp32 (relative vectors formed in f64, the
+ *   near/far switch decided in f64) with compensated (Kahan) fp32 per-target
+ *   accumulation of the free-wake self-induction; the fixed velocities fold the
+ *   fp32 terms in f64; unit factors, solver, geometry and wake convection stay
+ *   f64.  Not bit-exact: its deviation from the f64 path is measured and
+ *   reported.  Single simulations only (sb_batch_create takes EXACT or FAST). */
 enum { SB_MODE_EXACT = 0, SB_MODE_FAST = 1, SB_MODE_FP32_KAHAN = 2 };
 
 /* Message of the last error on this thread. */

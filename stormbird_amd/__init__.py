@@ -25,6 +25,7 @@ __all__ = ["Simulation", "SimulationResult", "MODE_EXACT", "MODE_FAST", "library
 
 MODE_EXACT = 0
 MODE_FAST = 1
+MODE_FP32_KAHAN = 2
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.path.join(_HERE, "libstormbird_amd.so")

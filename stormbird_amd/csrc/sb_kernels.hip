@@ -131,8 +131,71 @@ __device__ inline V3 doublet_quotients(double d2, V3 num) {
     return V3{num.x / d5, num.y / d5, num.z / d5};
 }
 
+// ---- single precision (MODE_FP32_KAHAN) ----
+// Relative vectors are formed in f64 (where the cancellation happens) and
+// rounded to f32; the element formulas then run in f32.  The near/far switch is
+// decided in f64, as the reference decides it, so every pair takes the same
+// branch as in the f64 path.
+struct F3 {
+    float x, y, z;
+};
+__device__ inline F3 f3_of(V3 a) { return F3{(float)a.x, (float)a.y, (float)a.z}; }
+__device__ inline float f3dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline F3 f3cross(F3 a, F3 b) {
+    return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// vortex_line.rs:18-106 in f32
+__device__ inline F3 segment_unit_f32(V3 l0, V3 l1, V3 x, float rc) {
+    F3 r1 = f3_of(x - l0), r2 = f3_of(x - l1);
+    float r1sq = f3dot(r1, r1), r2sq = f3dot(r2, r2);
+    float r1l = sqrtf(r1sq), r2l = sqrtf(r2sq);
+    float r1r2 = r1l * r2l;
+    float den = r1r2 * (r1r2 + f3dot(r1, r2));
+    if (fabsf(den) > (float)CLOSENESS_ERROR) {
+        float core = 1.0f;
+        if (rc != 0.0f) {
+            F3 rel = f3_of(l1 - l0);
+            float ll = sqrtf(f3dot(rel, rel));
+            float par = f3dot(r1, rel) / ll;
+            float d2 = par < 0.0f ? r1sq : (par > ll ? r2sq : r1sq - par * par);
+            float rc2 = rc * rc;
+            float cden = sqrtf(rc2 * rc2 + d2 * d2);
+            core = cden > 0.0f ? d2 / cden : 0.0f;
+        }
+        float kk = (r1l + r2l) / den * (core * (float)FOUR_PI_INVERSE);
+        F3 c = f3cross(r1, r2);
+        return F3{c.x * kk, c.y * kk, c.z * kk};
+    }
+    return F3{0.0f, 0.0f, 0.0f};
+}
+
+__device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 x) {
+    double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
+    double d2 = tx * tx + ty * ty + tz * tz;
+    if (d2 <= r.far2) {
+        V3 c[4];
+        panel_corners(d, r, k, c);
+        float rc = (float)d.core[k];
+        F3 a = segment_unit_f32(c[0], c[1], x, rc), b = segment_unit_f32(c[1], c[2], x, rc);
+        F3 e = segment_unit_f32(c[2], c[3], x, rc), f = segment_unit_f32(c[3], c[0], x, rc);
+        return F3{((a.x + b.x) + e.x) + f.x, ((a.y + b.y) + e.y) + f.y, ((a.z + b.z) + e.z) + f.z};
+    }
+    F3 t{(float)tx, (float)ty, (float)tz};
+    float q = (float)d2;
+    float y = __builtin_amdgcn_rsqf(q);
+    float y2 = y * y;
+    float sc = (float)r.area_term * ((y2 * y2) * y);
+    float h3 = 3.0f * (t.x * (float)r.nx + t.y * (float)r.ny + t.z * (float)r.nz);
+    return F3{(t.x * h3 - (float)r.nx * q) * sc, (t.y * h3 - (float)r.ny * q) * sc, (t.z * h3 - (float)r.nz * q) * sc};
+}
+
 template <int MODE>
 __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
+    if (MODE == MODE_FP32_KAHAN) {
+        F3 u = panel_unit_f32(d, r, k, x);
+        return V3{(double)u.x, (double)u.y, (double)u.z};
+    }
     // panel.rs:53-76: near/far switch on |x - c|^2 <= far^2
     double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
     if (MODE == MODE_EXACT) {
@@ -197,8 +260,41 @@ __device__ inline V3 pair_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
 // a zero-strength panel adds nothing, :202-210).  Every thread of the block
 // calls it with the same range: records are staged through the shared tile,
 // and the next record is read while the current one is evaluated.
+// Compensated (Kahan) f32 summation of f32 terms over panels [p0, p1): the
+// sum s and its running error c per component; s - c is added to acc.
+__device__ inline void kahan_add(float& s, float& c, float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+template <int SYM>
+__device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1) {
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    for (long base = p0; base < p1; base += TILE) {
+        int cnt = (int)((p1 - base) < TILE ? (p1 - base) : TILE);
+        __syncthreads();
+        const double* src = reinterpret_cast<const double*>(d.rec + base);
+        double* dst = reinterpret_cast<double*>(tile);
+        for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
+        __syncthreads();
+        for (int p = 0; p < cnt; ++p) {
+            const PanelRec& r = tile[p];
+            if (r.strength == 0.0) continue;
+            V3 u = pair_unit<MODE_FP32_KAHAN, SYM>(d, r, (int)(base + p), x);
+            float g = (float)r.strength;
+            kahan_add(sx, cx, (float)u.x * g);
+            kahan_add(sy, cy, (float)u.y * g);
+            kahan_add(sz, cz, (float)u.z * g);
+        }
+    }
+    return acc + V3{(double)sx - (double)cx, (double)sy - (double)cy, (double)sz - (double)cz};
+}
+
 template <int MODE, int SYM>
 __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1) {
+    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM>(d, tile, x, acc, p0, p1);
     for (long base = p0; base < p1; base += TILE) {
         int cnt = (int)((p1 - base) < TILE ? (p1 - base) : TILE);
         __syncthreads();
@@ -1140,6 +1236,9 @@ hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st
             if (mode == MODE_FAST)
                 hipLaunchKernelGGL(k_terms<MODE_FAST>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt, d.neglect_self,
                                    fb->terms, ld);
+            else if (mode == MODE_FP32_KAHAN)
+                hipLaunchKernelGGL(k_terms<MODE_FP32_KAHAN>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt,
+                                   d.neglect_self, fb->terms, ld);
             else
                 hipLaunchKernelGGL(k_terms<MODE_EXACT>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt, d.neglect_self,
                                    fb->terms, ld);
@@ -1153,6 +1252,9 @@ hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st
     unsigned nb = blocks_for(d.N, 4);
     if (mode == MODE_FAST)
         hipLaunchKernelGGL(k_fold<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, ctrl, d.N, p_begin, p_end,
+                           d.neglect_self, d.fixed);
+    else if (mode == MODE_FP32_KAHAN)
+        hipLaunchKernelGGL(k_fold<MODE_FP32_KAHAN>, dim3(nb), dim3(256), 0, st, d, ctrl, d.N, p_begin, p_end,
                            d.neglect_self, d.fixed);
     else
         hipLaunchKernelGGL(k_fold<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, ctrl, d.N, p_begin, p_end,
@@ -1194,8 +1296,11 @@ template <int MODE, int SYM>
 static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_targets, long n_panels, double* out,
                                    int add, McBuffers* mc, hipStream_t st) {
     long n_jobs = (n_targets + 255) / 256;
-    int& slots = mc ? mc->slots[MODE][SYM] : *(int*)nullptr;
-    if (mc && slots == 0) slots = mc_slots_for<MODE, SYM>();
+    int slots = 0;
+    if (mc) {
+        if (mc->slots[MODE][SYM] == 0) mc->slots[MODE][SYM] = mc_slots_for<MODE, SYM>();
+        slots = mc->slots[MODE][SYM];
+    }
     if (mc && slots > 0 && n_jobs > slots && mc->cap_jobs >= n_jobs && n_panels > 0) {
         McPlan pl;
         pl.L = n_panels;
@@ -1222,6 +1327,9 @@ hipError_t launch_targets(const Inst& d, const double* targets, long n_targets, 
     if (mode == MODE_FAST)
         return sym ? launch_targets_t<MODE_FAST, 1>(d, targets, n_targets, n_panels, out, add, mc, st)
                    : launch_targets_t<MODE_FAST, 0>(d, targets, n_targets, n_panels, out, add, mc, st);
+    if (mode == MODE_FP32_KAHAN)  // one workgroup per 256 targets (the Kahan state is not carried across pieces)
+        return sym ? launch_targets_t<MODE_FP32_KAHAN, 1>(d, targets, n_targets, n_panels, out, add, nullptr, st)
+                   : launch_targets_t<MODE_FP32_KAHAN, 0>(d, targets, n_targets, n_panels, out, add, nullptr, st);
     return sym ? launch_targets_t<MODE_EXACT, 1>(d, targets, n_targets, n_panels, out, add, mc, st)
                : launch_targets_t<MODE_EXACT, 0>(d, targets, n_targets, n_panels, out, add, mc, st);
 }
@@ -1250,6 +1358,9 @@ hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_t
         unsigned nb = blocks_for(n_targets, 4);
         if (mode == MODE_FAST)
             hipLaunchKernelGGL(k_fold<MODE_FAST>, dim3(nb), dim3(256), 0, st, d, targets, (int)n_targets, 0,
+                               (int)n_panels, 0, out);
+        else if (mode == MODE_FP32_KAHAN)
+            hipLaunchKernelGGL(k_fold<MODE_FP32_KAHAN>, dim3(nb), dim3(256), 0, st, d, targets, (int)n_targets, 0,
                                (int)n_panels, 0, out);
         else
             hipLaunchKernelGGL(k_fold<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, targets, (int)n_targets, 0,

@@ -23,7 +23,7 @@ sb_sim* sb_sim_create_ex(const char* json, int device, int mode) {
         g_error = "null setup string";
         return nullptr;
     }
-    if (mode != MODE_EXACT && mode != MODE_FAST) {
+    if (mode != MODE_EXACT && mode != MODE_FAST && mode != MODE_FP32_KAHAN) {
         g_error = "unsupported mode";
         return nullptr;
     }

@@ -334,3 +334,23 @@ def test_full_size_free_wake_step_sampled_bitwise(oracle):
     u = orc.induced_velocities(w["points"][idx])
     expect = fs[gpu.nr_ctrl_points:][idx] + u
     assert same(w["velocities"][idx], expect)
+
+
+def test_fp32_kahan_mode_deviation(oracle):
+    """FP32-Kahan pair sums: deviation from the f64 oracle over a free-wake trajectory and
+    from one developed state (reported by bench.py at full size); bound 1e-5 relative."""
+    js, case = S.four_sail_free_wake(nr_panels=60, nr_strips=12)
+    dt = case["time_step"]
+    g = sb.Simulation(js, mode=sb.MODE_FP32_KAHAN)
+    o = oracle.Simulation(js, variant="glibc")
+    fs = S.uniform_freestream(g.nr_freestream_points(), case["velocity"])
+    worst = 0.0
+    for k in range(25):
+        rg = g.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+        ro = o.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
+        ag, ao = rg.as_arrays(), ro.as_arrays()
+        for f in ("circulation_strength", "integrated_forces"):
+            worst = max(worst, rel_dev(ag[f], ao[f]))
+    print(f"fp32-kahan worst relative deviation over 25 steps: {worst:.3e}")
+    assert worst <= 1e-5
+    assert worst > 0.0  # it is a different arithmetic
