@@ -105,6 +105,16 @@ int sb_sim_set_previous_circulation(sb_sim* sim, const double* gamma, size_t n);
 
 /* LineForceModel::set_translation_and_rotation_with_finite_difference_for_the_velocity
  * (line_force_model/data_update.rs:188-205). */
+/* Back to the state of a newly built simulation: straight initial wake, zero
+ * strengths, no completed step (the line-force model's motion is kept).  No
+ * counterpart in the reference (which builds a new Simulation); it lets one
+ * handle serve many independent first-step solves. */
+int sb_sim_reset(sb_sim* sim);
+/* The solver's public settings (SimpleIterative: max_iterations_per_time_step,
+ * damping_factor, residual_tolerance_absolute, strength_difference_tolerance;
+ * solvers/simple_iterative.rs:59-76), used from the next step on. */
+int sb_sim_set_solver_settings(sb_sim* sim, long max_iterations, double damping, double residual_tol,
+                               double strength_tol);
 int sb_sim_set_translation_and_rotation_fd(sb_sim* sim, double time_step,
                                            const double* translation3, const double* rotation3);
 /* LineForceModel::set_local_wing_angles (data_update.rs:171-177). */
@@ -185,6 +195,44 @@ int sb_batch_do_step_device(sb_batch* b, const double* times, const double* dts,
                             size_t n_points, sb_step_result* results);
 /* Device time of the last batched step (ms, HIP events around the step). */
 double sb_batch_last_step_ms(const sb_batch* b);
+/* sb_sim_reset on every instance. */
+int sb_batch_reset(sb_batch* b);
+
+/* ---- Learned warm start ---------------------------------------------------
+ * A per-strip network (64 -> 256 -> 256 -> 256 -> 256 -> 1, GELU; bf16 matrix
+ * -core GEMMs with fp32 accumulation, fp32 master weights, Adam) predicts
+ * circulation / (c |u| / 2) from strip features; attached to a batch it
+ * supplies the solver's initial circulation, and the unchanged fixed-point
+ * solve finishes from it to the same tolerances.  An instance whose
+ * prediction is not finite or exceeds 10 in magnitude keeps the default guess
+ * (previous circulation, or zero at the first step) and counts as a fallback.
+ * No counterpart in the reference.  Parameters: fp32, for each layer W[out][in]
+ * then b[out]. */
+typedef struct sb_mlp sb_mlp;
+sb_mlp* sb_mlp_create(int device, unsigned long long seed);
+void sb_mlp_destroy(sb_mlp* m);
+long sb_mlp_nr_params(const sb_mlp* m);
+int sb_mlp_get_params(const sb_mlp* m, float* params, size_t cap);
+int sb_mlp_set_params(sb_mlp* m, const float* params, size_t n);
+/* Device pointers: x [n][64] fp32, y/t [n] fp32. */
+int sb_mlp_forward(sb_mlp* m, const float* x, long n, float* y);
+/* Mean squared error of the predictions against t and its gradient with
+ * respect to every parameter (host array, parameter layout). */
+int sb_mlp_gradients(sb_mlp* m, const float* x, const float* t, long n, float* grads, float* loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean squared error. */
+int sb_mlp_train_step(sb_mlp* m, const float* x, const float* t, long n, float learning_rate, float* loss);
+
+/* mode 0: off; 1: at the first step; 2: at every step.  The network must
+ * outlive its use by the batch. */
+int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode);
+/* Record the per-strip features at every step (also when no network is set). */
+int sb_batch_set_record_features(sb_batch* b, int on);
+long sb_batch_nr_strips(const sb_batch* b);
+/* Features [strips][64] and scale = c |u| / 2 [strips] of the last step, in
+ * instance order. */
+int sb_batch_get_features(const sb_batch* b, float* features, float* scale, size_t cap_strips);
+/* Instances that kept their default initial guess in the last step. */
+long sb_batch_last_fallbacks(const sb_batch* b);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,27 @@ def lib():
         L.sb_batch_do_step_device.argtypes = [vp, _dp, _dp, ctypes.c_void_p, ctypes.c_size_t, vp]
         L.sb_batch_last_step_ms.restype = ctypes.c_double
         L.sb_batch_last_step_ms.argtypes = [vp]
+        L.sb_batch_reset.argtypes = [vp]
+        L.sb_sim_reset.argtypes = [vp]
+        L.sb_sim_set_solver_settings.argtypes = [vp, ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.sb_mlp_create.restype = vp
+        L.sb_mlp_create.argtypes = [ctypes.c_int, ctypes.c_ulonglong]
+        L.sb_mlp_destroy.argtypes = [vp]
+        L.sb_mlp_nr_params.restype = ctypes.c_long
+        L.sb_mlp_nr_params.argtypes = [vp]
+        L.sb_mlp_get_params.argtypes = [vp, _fp, ctypes.c_size_t]
+        L.sb_mlp_set_params.argtypes = [vp, _fp, ctypes.c_size_t]
+        L.sb_mlp_forward.argtypes = [vp, ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]
+        L.sb_mlp_gradients.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, _fp, _fp]
+        L.sb_mlp_train_step.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_float, _fp]
+        L.sb_batch_set_warm_start.argtypes = [vp, vp, ctypes.c_int]
+        L.sb_batch_set_record_features.argtypes = [vp, ctypes.c_int]
+        L.sb_batch_nr_strips.restype = ctypes.c_long
+        L.sb_batch_nr_strips.argtypes = [vp]
+        L.sb_batch_get_features.argtypes = [vp, _fp, _fp, ctypes.c_size_t]
+        L.sb_batch_last_fallbacks.restype = ctypes.c_long
+        L.sb_batch_last_fallbacks.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -250,6 +271,21 @@ class Simulation:
             _err("get_frozen_wake", rc)
         return fixed, fac
 
+    def reset(self):
+        """State of a newly built simulation (straight initial wake, no completed step)."""
+        rc = lib().sb_sim_reset(self._h)
+        if rc != 0:
+            _err("reset", rc)
+
+    def set_solver_settings(self, *, max_iterations: int, damping: float, residual_tolerance: float,
+                            strength_tolerance: float):
+        """SimpleIterative's public fields (max_iterations_per_time_step, damping_factor,
+        residual_tolerance_absolute, strength_difference_tolerance), from the next step on."""
+        rc = lib().sb_sim_set_solver_settings(self._h, int(max_iterations), float(damping),
+                                              float(residual_tolerance), float(strength_tolerance))
+        if rc != 0:
+            _err("set_solver_settings", rc)
+
     def last_kernel_ms(self, name: str) -> float:
         return lib().sb_sim_last_kernel_ms(self._h, name.encode())
 
@@ -369,3 +405,89 @@ class Batch:
 
     def last_step_ms(self) -> float:
         return lib().sb_batch_last_step_ms(self._h)
+
+    def reset(self):
+        rc = lib().sb_batch_reset(self._h)
+        if rc != 0:
+            _err("sb_batch_reset", rc)
+
+    def set_warm_start(self, mlp, mode: int = 1):
+        """mode 0 off, 1 first step, 2 every step (keeps a reference to the network)."""
+        self._mlp = mlp if mode else None
+        rc = lib().sb_batch_set_warm_start(self._h, mlp._h if mlp is not None else None, int(mode))
+        if rc != 0:
+            _err("sb_batch_set_warm_start", rc)
+
+    def set_record_features(self, on: bool = True):
+        lib().sb_batch_set_record_features(self._h, 1 if on else 0)
+
+    @property
+    def nr_strips(self) -> int:
+        return lib().sb_batch_nr_strips(self._h)
+
+    def get_features(self):
+        """(features [strips, 64], scale [strips]) of the last step."""
+        n = self.nr_strips
+        x = np.zeros((n, 64), dtype=np.float32)
+        sc = np.zeros(n, dtype=np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        rc = lib().sb_batch_get_features(self._h, x.ctypes.data_as(fp), sc.ctypes.data_as(fp), n)
+        if rc != 0:
+            _err("sb_batch_get_features", rc)
+        return x, sc
+
+    def last_fallbacks(self) -> int:
+        return lib().sb_batch_last_fallbacks(self._h)
+
+
+class Mlp:
+    """Warm-start network (sb_mlp_*): 64 -> 256 x 4 (GELU) -> 1, bf16 matrix-core GEMMs,
+    fp32 master weights.  Inputs and targets are device pointers (e.g. torch tensors)."""
+
+    def __init__(self, *, device: int = 0, seed: int = 1):
+        L = lib()
+        self._h = L.sb_mlp_create(device, seed)
+        if not self._h:
+            raise RuntimeError(L.sb_last_error().decode())
+        self.nr_params = L.sb_mlp_nr_params(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().sb_mlp_destroy(self._h)
+            self._h = None
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.nr_params, dtype=np.float32)
+        rc = lib().sb_mlp_get_params(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.size)
+        if rc != 0:
+            _err("sb_mlp_get_params", rc)
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        rc = lib().sb_mlp_set_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), p.size)
+        if rc != 0:
+            _err("sb_mlp_set_params", rc)
+
+    def forward(self, x_ptr: int, n: int, y_ptr: int):
+        rc = lib().sb_mlp_forward(self._h, ctypes.c_void_p(x_ptr), int(n), ctypes.c_void_p(y_ptr))
+        if rc != 0:
+            _err("sb_mlp_forward", rc)
+
+    def gradients(self, x_ptr: int, t_ptr: int, n: int):
+        g = np.zeros(self.nr_params, dtype=np.float32)
+        loss = ctypes.c_float(0.0)
+        fp = ctypes.POINTER(ctypes.c_float)
+        rc = lib().sb_mlp_gradients(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(t_ptr), int(n),
+                                    g.ctypes.data_as(fp), ctypes.byref(loss))
+        if rc != 0:
+            _err("sb_mlp_gradients", rc)
+        return g, loss.value
+
+    def train_step(self, x_ptr: int, t_ptr: int, n: int, lr: float) -> float:
+        loss = ctypes.c_float(0.0)
+        rc = lib().sb_mlp_train_step(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(t_ptr), int(n), float(lr),
+                                     ctypes.byref(loss))
+        if rc != 0:
+            _err("sb_mlp_train_step", rc)
+        return loss.value

@@ -54,15 +54,25 @@ struct sb_batch {
     hipEvent_t ev[2] = {};
     double last_ms = 0.0;
     int use_graph = 1;
+    // learned warm start
+    sb_mlp* mlp = nullptr;
+    int warm_mode = 0;          // 0 off, 1 first step only, 2 every step
+    bool record_features = false;
+    long n_strips = 0;
+    long* d_strip_off = nullptr;
+    float *d_feat = nullptr, *d_scale = nullptr, *d_pred = nullptr;
+    unsigned *d_fallbacks = nullptr, *h_fallbacks = nullptr;
+    bool last_applied = false;
 
     ~sb_batch() {
         (void)hipSetDevice(device);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
-        void* dev[] = {d_geo, d_fs, d_out, d_inst, d_off, d_jobs, d_counters};
+        void* dev[] = {d_geo, d_fs, d_out, d_inst, d_off, d_jobs, d_counters, d_strip_off, d_feat, d_scale, d_pred,
+                       d_fallbacks};
         for (void* p : dev)
             if (p) (void)hipFree(p);
-        void* host[] = {h_geo, h_out, h_inst, h_jobs};
+        void* host[] = {h_geo, h_out, h_inst, h_jobs, h_fallbacks};
         for (void* p : host)
             if (p) (void)hipHostFree(p);
         for (auto& e : ev)
@@ -157,7 +167,31 @@ struct sb_batch {
         if (bd.fixed_grid <= 0 || bd.target_grid <= 0) throw HipError("occupancy query failed", -1);
         const char* g = getenv("SB_BATCH_GRAPH");
         use_graph = !(g && strcmp(g, "0") == 0);
+        // per-strip arrays of the warm start
+        std::vector<long> so(n + 1, 0);
+        for (int i = 0; i < n; ++i) so[i + 1] = so[i] + sims[i]->N;
+        n_strips = so[n];
+        d_strip_off = dalloc<long>(n + 1);
+        check(hipMemcpyAsync(d_strip_off, so.data(), sizeof(long) * (n + 1), hipMemcpyHostToDevice, stream), "upload");
+        d_feat = dalloc<float>(n_strips * MLP_FEATURES);
+        d_scale = dalloc<float>(n_strips);
+        d_pred = dalloc<float>(n_strips);
+        d_fallbacks = dalloc<unsigned>(1);
+        h_fallbacks = halloc<unsigned>(1);
+        *h_fallbacks = 0;
+        bd.strip_off = d_strip_off;
+        bd.n_strips = n_strips;
+        bd.feat = d_feat;
+        bd.scale = d_scale;
+        bd.pred = d_pred;
+        bd.fallbacks = d_fallbacks;
         check(hipStreamSynchronize(stream), "batch create");
+    }
+
+    void reset() {
+        check(hipSetDevice(device), "hipSetDevice");
+        for (auto& s : sims) s->reset_state();
+        first_done = false;
     }
 
     // Descriptor of instance i at the current host state, with the batch slabs.
@@ -226,9 +260,12 @@ struct sb_batch {
                              fs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
               "upload freestream");
         check(hipEventRecord(ev[0], stream), "event");
-        long sig[3] = {first ? 1 : 0, bd.n_fixed_jobs, bd.n_target_jobs};
+        const bool apply = mlp && (warm_mode == 2 || (warm_mode == 1 && first));
+        const bool feats = apply || record_features;
+        last_applied = apply;
+        long sig[3] = {first ? 1 : 0, bd.n_fixed_jobs, bd.n_target_jobs * 4 + (apply ? 2 : 0) + (feats ? 1 : 0)};
         if (first || !use_graph) {
-            enqueue(first);
+            enqueue(first, apply, feats);
         } else {
             if (!exec || sig[0] != graph_sig[0] || sig[1] != graph_sig[1] || sig[2] != graph_sig[2]) {
                 if (exec) (void)hipGraphExecDestroy(exec);
@@ -237,7 +274,7 @@ struct sb_batch {
                 graph = nullptr;
                 check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
                 try {
-                    enqueue(false);
+                    enqueue(false, apply, feats);
                 } catch (...) {
                     hipGraph_t g = nullptr;
                     (void)hipStreamEndCapture(stream, &g);
@@ -266,7 +303,7 @@ struct sb_batch {
     }
 
     // The launches of one step (captured into the graph after the first step).
-    void enqueue(bool first) {
+    void enqueue(bool first, bool apply, bool feats) {
         check(hipMemcpyAsync(d_geo, h_geo, sizeof(double) * geo_total, hipMemcpyHostToDevice, stream), "upload geometry");
         check(hipMemcpyAsync(d_inst, h_inst, sizeof(Inst) * kStages * n, hipMemcpyHostToDevice, stream), "upload");
         check(hipMemcpyAsync(d_jobs, h_jobs, sizeof(int2) * (cap_fixed_jobs + cap_target_jobs), hipMemcpyHostToDevice,
@@ -275,6 +312,10 @@ struct sb_batch {
         BatchDev b = bd;
         b.inst = d_inst;
         check(launch_batch_prepare(b, first ? 1 : 0, stream), "prepare");
+        if (feats) check(launch_batch_warm_start(b, mlp, apply ? 1 : 0, stream), "warm start");
+        if (apply)
+            check(hipMemcpyAsync(h_fallbacks, d_fallbacks, sizeof(unsigned), hipMemcpyDeviceToHost, stream),
+                  "download");
         if (first) check(launch_batch_init_wake(b, stream), "init wake");
         check(launch_batch_wake_step(b, stream), "wake step");
         b.inst = d_inst + 1L * n;
@@ -336,5 +377,45 @@ int sb_batch_do_step_device(sb_batch* b, const double* times, const double* dts,
 }
 
 double sb_batch_last_step_ms(const sb_batch* b) { return b ? b->last_ms : -1.0; }
+
+int sb_batch_reset(sb_batch* b) {
+    if (!b) return -1;
+    return guarded([&] { b->reset(); });
+}
+
+int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode) {
+    if (!b) return -1;
+    if (mode < 0 || mode > 2 || (mode > 0 && !m)) return -4;
+    return guarded([&] {
+        if (m) mlp_reserve(m, b->n_strips);
+        b->mlp = mode > 0 ? m : nullptr;
+        b->warm_mode = mode;
+    });
+}
+
+int sb_batch_set_record_features(sb_batch* b, int on) {
+    if (!b) return -1;
+    b->record_features = on != 0;
+    return 0;
+}
+
+long sb_batch_nr_strips(const sb_batch* b) { return b ? b->n_strips : -1; }
+
+int sb_batch_get_features(const sb_batch* b, float* features, float* scale, size_t cap_strips) {
+    if (!b) return -1;
+    if (cap_strips < (size_t)b->n_strips) return -3;
+    return guarded([&] {
+        check(hipSetDevice(b->device), "hipSetDevice");
+        if (features)
+            check(hipMemcpy(features, b->d_feat, sizeof(float) * MLP_FEATURES * b->n_strips, hipMemcpyDeviceToHost),
+                  "download");
+        if (scale) check(hipMemcpy(scale, b->d_scale, sizeof(float) * b->n_strips, hipMemcpyDeviceToHost), "download");
+    });
+}
+
+long sb_batch_last_fallbacks(const sb_batch* b) {
+    if (!b) return -1;
+    return b->last_applied ? (long)*b->h_fallbacks : 0;
+}
 
 }  // extern "C"

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sb_common.h"
+#include "sb_mlp.h"
 
 namespace sb {
 
@@ -157,6 +158,13 @@ struct BatchDev {
     unsigned* counters = nullptr;       // device [2] job-queue heads, zero between launches
     int fixed_grid = 0, target_grid = 0;  // persistent workgroups
     int sym = 0;                  // 1 when any instance uses a symmetry plane
+    // learned warm start (sb_mlp): per-strip features [n_strips][64], scale, prediction
+    const long* strip_off = nullptr;  // device [n + 1]: first strip of each instance
+    long n_strips = 0;
+    float* feat = nullptr;
+    float* scale = nullptr;
+    float* pred = nullptr;
+    unsigned* fallbacks = nullptr;    // device [1]: instances that kept their default guess
 };
 
 hipError_t launch_batch_prepare(const BatchDev& b, int first_step, hipStream_t st);
@@ -169,6 +177,9 @@ hipError_t launch_batch_solve(const BatchDev& b, hipStream_t st);
 hipError_t launch_batch_velocity_copy(const BatchDev& b, hipStream_t st);
 hipError_t launch_batch_self_induction(const BatchDev& b, int mode, hipStream_t st);
 hipError_t launch_batch_forces(const BatchDev& b, hipStream_t st);
+// Features of every strip (kb_features) and, when `apply`, the network's
+// initial circulation written into prev_gamma (after the prepare kernel).
+hipError_t launch_batch_warm_start(const BatchDev& b, sb_mlp* mlp, int apply, hipStream_t st);
 // Resident workgroups of the persistent pair-sum kernels (0 fixed, 1 self-induction).
 int batch_resident_groups(int which, int mode, int sym);
 

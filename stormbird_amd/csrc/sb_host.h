@@ -289,6 +289,20 @@ struct sb_sim {
         check(hipMemcpyAsync(d_foil, m.foils.data(), sizeof(FoilParams) * W, hipMemcpyHostToDevice, stream), "upload");
         check(hipMemcpyAsync(d_core, setup.wake.core.data(), sizeof(double) * nr_panels, hipMemcpyHostToDevice, stream),
               "upload");
+        h_out.assign(OutLayout::total(N, W), 0.0);
+        reset_state();
+    }
+
+    // The state of a newly built simulation: straight chord-based wake, zero
+    // strengths, no completed step (the line-force model's motion is kept).
+    void reset_state() {
+        check(hipSetDevice(device), "hipSetDevice");
+        const LineModel& m = setup.lfm;
+        first_done = false;
+        steps_completed = 0;
+        nz_rows = 0;
+        cur = 0;
+        scur = 0;
         check(hipMemsetAsync(d_prev, 0, sizeof(double) * N, stream), "memset");
         check(hipMemsetAsync(d_hist, 0, sizeof(double) * 6 * N, stream), "memset");
         check(hipMemsetAsync(d_scal, 0, sizeof(double) * 8, stream), "memset");
@@ -304,8 +318,7 @@ struct sb_sim {
         Inst d = inst();
         check(launch_init_wake(d, 1.0, 0, v, stream), "init wake");
         check(launch_panels(d, d.pts, 0, stream), "panels");
-        h_out.assign(OutLayout::total(N, W), 0.0);
-        check(hipStreamSynchronize(stream), "create");
+        check(hipStreamSynchronize(stream), "reset");
     }
 
     void upload_freestream(const double* fs, size_t n, bool on_device) {

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sb_device.h"
+#include "sb_mlp.h"
 
 namespace sb {
 
@@ -1189,6 +1190,111 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
+
+// ---- learned warm start: per-strip features and the initial guess ----
+// Features of strip i (wing strips [a, b)): for o = -4..4, strip j = i + o
+// inside the wing gives [c_j / mean chord, (j - a + 1/2)/(b - a), twist_j / 0.25,
+// alpha_j / 0.25, |u_j| / mean |u|] (zeros outside the wing), then the wing's
+// aspect ratio / 6 and the number of wings / 6.  u = felt ctrl-point velocity
+// without its spanwise part and alpha its angle of attack (the solver's first
+// evaluation with no induced velocity, line_force_model/mod.rs:235-254,
+// force_calculations.rs:50-69); twist = chord angle about the span axis
+// relative to the wing's first strip.  scale = c |u| / 2, so that
+// circulation = prediction * scale.
+constexpr int WS_FEATURES = 64;
+constexpr int WS_BASE = 5;
+
+__device__ inline void features_body(const Inst& d, float* X, float* scale) {
+    __shared__ float base[1024][WS_BASE];
+    __shared__ int wa[1024], wb[1024];
+    const int N = d.N;
+    const double* geo = d.geo;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        int w = d.wing_of[i];
+        int a = i, b = i + 1;
+        while (a > 0 && d.wing_of[a - 1] == w) --a;
+        while (b < N && d.wing_of[b] == w) ++b;
+        wa[i] = a;
+        wb[i] = b;
+        V3 felt = load3(d.felt + 3 * i);
+        V3 rel = load3(geo + d.gl.rel_g + 3 * i);
+        V3 dir = load3(geo + d.gl.dir_g + 3 * i);
+        V3 ch = load3(geo + d.gl.chord_g + 3 * i);
+        V3 u = felt - project(felt, rel);
+        double sp = len(u);
+        double alpha = sp > MIN_POSITIVE ? signed_angle_between(ch, u, dir) : 0.0;
+        V3 ch0 = load3(geo + d.gl.chord_g + 3 * a);
+        double twist = signed_angle_between(ch, ch0, dir);
+        double c = geo[d.gl.chord_len + i];
+        base[i][0] = (float)c;
+        base[i][1] = (float)(((double)(i - a) + 0.5) / (double)(b - a));
+        base[i][2] = (float)(twist / 0.25);
+        base[i][3] = (float)(alpha / 0.25);
+        base[i][4] = (float)sp;
+        scale[i] = (float)(0.5 * c * sp);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        int a = wa[i], b = wb[i];
+        double cs = 0.0, us = 0.0, span = 0.0;
+        for (int j = a; j < b; ++j) {
+            cs += base[j][0];
+            us += base[j][4];
+            span += geo[d.gl.local_len + j];
+        }
+        double cm = cs / (b - a), um = us / (b - a);
+        float* x = X + (long)i * WS_FEATURES;
+        for (int o = -4; o <= 4; ++o) {
+            int j = i + o;
+            float* f = x + (o + 4) * WS_BASE;
+            if (j < a || j >= b) {
+                for (int q = 0; q < WS_BASE; ++q) f[q] = 0.0f;
+                continue;
+            }
+            f[0] = (float)(base[j][0] / cm);
+            f[1] = base[j][1];
+            f[2] = base[j][2];
+            f[3] = base[j][3];
+            f[4] = um > 0.0 ? (float)(base[j][4] / um) : 0.0f;
+        }
+        x[45] = (float)(span / cm / 6.0);
+        x[46] = (float)(d.W / 6.0);
+        for (int q = 47; q < WS_FEATURES; ++q) x[q] = 0.0f;
+    }
+}
+
+__global__ void kb_features(const Inst* I, const long* strip_off, float* X, float* scale) {
+    const Inst d = I[blockIdx.x];
+    long o = strip_off[blockIdx.x];
+    features_body(d, X + o * WS_FEATURES, scale + o);
+}
+
+// prev_gamma = prediction * scale for every strip of the instance, unless a
+// prediction is not finite or exceeds 10 in magnitude (|CL| > 10): then the
+// instance keeps its default initial guess and counts as a fallback.
+__global__ void kb_warm_apply(const Inst* I, const long* strip_off, const float* pred, const float* scale,
+                              unsigned* fallbacks) {
+    __shared__ int bad;
+    const Inst d = I[blockIdx.x];
+    long o = strip_off[blockIdx.x];
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < d.N; i += blockDim.x) {
+        float y = pred[o + i];
+        if (!(fabsf(y) <= 10.0f)) atomicOr(&bad, 1);
+    }
+    __syncthreads();
+    if (bad) {
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(fallbacks, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    for (int i = threadIdx.x; i < d.N; i += blockDim.x) d.prev_gamma[i] = (double)pred[o + i] * (double)scale[o + i];
+}
+
+__global__ void kb_zero_u32(unsigned* c) {
+    if (threadIdx.x == 0) c[0] = 0u;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1386,6 +1492,17 @@ hipError_t launch_libm(int fn, const double* x, const double* y, double* out, lo
 
 
 // ---- batched launchers ----
+hipError_t launch_batch_warm_start(const BatchDev& b, sb_mlp* mlp, int apply, hipStream_t st) {
+    hipLaunchKernelGGL(kb_features, dim3(b.n), dim3(b.max_threads), 0, st, b.inst, b.strip_off, b.feat, b.scale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !apply) return e;
+    hipLaunchKernelGGL(kb_zero_u32, dim3(1), dim3(64), 0, st, b.fallbacks);
+    e = mlp_forward_on(mlp, b.feat, b.n_strips, b.pred, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kb_warm_apply, dim3(b.n), dim3(256), 0, st, b.inst, b.strip_off, b.pred, b.scale, b.fallbacks);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_prepare(const BatchDev& b, int first_step, hipStream_t st) {
     hipLaunchKernelGGL(kb_prepare, dim3(b.n), dim3(256), 0, st, b.inst, first_step);
     return hipGetLastError();

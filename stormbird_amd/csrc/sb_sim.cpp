@@ -103,6 +103,23 @@ int sb_sim_set_previous_circulation(sb_sim* s, const double* g, size_t n) {
     });
 }
 
+int sb_sim_reset(sb_sim* s) {
+    if (!s) return -1;
+    return guarded([&] { s->reset_state(); });
+}
+
+int sb_sim_set_solver_settings(sb_sim* s, long max_iterations, double damping, double residual_tol,
+                               double strength_tol) {
+    if (!s) return -1;
+    if (max_iterations < 1 || !(damping > 0.0) || damping > 1.0 || residual_tol < 0.0 || strength_tol < 0.0)
+        return -4;
+    s->setup.solver.max_iterations = (int)max_iterations;
+    s->setup.solver.damping = damping;
+    s->setup.solver.residual_tol = residual_tol;
+    s->setup.solver.strength_tol = strength_tol;
+    return 0;
+}
+
 int sb_sim_set_translation_and_rotation_fd(sb_sim* s, double dt, const double* t, const double* r) {
     if (!s || !t || !r) return -1;
     s->setup.lfm.rbm.update_translation_fd(v3(t[0], t[1], t[2]), dt);
