@@ -203,6 +203,144 @@ def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
     }
 
 
+def _perturbed(cases, rng):
+    """Per instance: the freestream rotated by up to +-6 deg and scaled by 0.8-1.2."""
+    out = []
+    for c in cases:
+        u = np.asarray(c["velocity"], dtype=np.float64)
+        a = rng.uniform(-6.0, 6.0) * math.pi / 180.0
+        k = rng.uniform(0.8, 1.2)
+        out.append(k * np.array([u[0] * math.cos(a) - u[1] * math.sin(a), u[0] * math.sin(a) + u[1] * math.cos(a), 0.0]))
+    return out
+
+
+def warm_start_polar(n_inst=512, seed=3, train_conditions=16, train_steps=3000, minibatch=65536):
+    """Learned warm start on the polar-sweep distribution (seed 3): first-step solves of
+    512 instances under perturbed inflow give the training set; the network is trained on
+    the GPU (bf16 matrix-core forward/backward, Adam); held-out conditions are then solved
+    to a 1e-6 relative tolerance (max |Gamma_est - Gamma| < 1e-6 max |Gamma*|) from zero
+    (cold) and from the network's guess (warm), batched, and cold one instance at a time."""
+    import torch
+
+    import stormbird_amd as sb
+    from stormbird_amd import setups as S
+
+    rng = np.random.default_rng(seed)
+    made = [S.polar_sweep_instance(rng) for _ in range(n_inst)]
+    setups = [m[0] for m in made]
+    cases = [m[1] for m in made]
+    batch = sb.Batch(setups, mode=sb.MODE_EXACT)
+    dts = np.array([c["time_step"] for c in cases])
+    scale0 = np.array([0.5 * math.sqrt(sum(x * x for x in c["velocity"])) * 10.0 for c in cases])
+
+    def solve(vel):
+        batch.reset()
+        fs = np.concatenate([S.uniform_freestream(s.nr_freestream_points(), v) for s, v in zip(batch.instances, vel)])
+        return batch.do_step(times=None, time_steps=dts, freestream_velocity=fs)
+
+    def settings(tol):
+        for s, t in zip(batch.instances, tol):
+            s.set_solver_settings(max_iterations=5000, damping=0.05, residual_tolerance=0.0, strength_tolerance=t)
+
+    # training set: tight first-step solves
+    t0 = time.perf_counter()
+    settings(1e-9 * scale0)
+    batch.set_record_features(True)
+    drng = np.random.default_rng(seed + 1)
+    X, T = [], []
+    for _ in range(train_conditions):
+        res = solve(_perturbed(cases, drng))
+        x, sc = batch.get_features()
+        g = np.concatenate([r.force_input.circulation_strength for r in res])
+        X.append(x)
+        T.append((g / sc).astype(np.float32))
+    X, T = np.concatenate(X), np.concatenate(T)
+    t_data = time.perf_counter() - t0
+    # training on the GPU
+    t0 = time.perf_counter()
+    mlp = sb.Mlp(seed=seed)
+    xd = torch.from_numpy(X).to("cuda")
+    td = torch.from_numpy(T).to("cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    nb = min(minibatch, X.shape[0])
+    loss = None
+    for it in range(train_steps):
+        idx = torch.randint(0, X.shape[0], (nb,), device="cuda", generator=gen)
+        xb = xd.index_select(0, idx).contiguous()
+        tb = td.index_select(0, idx).contiguous()
+        lr = 1e-3 if it < 0.7 * train_steps else 2e-4
+        loss = mlp.train_step(xb.data_ptr(), tb.data_ptr(), nb, lr)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # held-out conditions
+    vel = _perturbed(cases, np.random.default_rng(seed + 1000))
+    settings(1e-11 * scale0)
+    ref = solve(vel)
+    gref = [r.force_input.circulation_strength.copy() for r in ref]
+    tol = np.array([1e-6 * np.max(np.abs(g)) for g in gref])
+    settings(tol)
+    batch.set_warm_start(None, 0)
+    cold = solve(vel)
+    ms_cold = batch.last_step_ms()
+    x, sc = batch.get_features()
+    batch.set_warm_start(mlp, 1)
+    warm = solve(vel)
+    ms_warm = batch.last_step_ms()
+    fallbacks = batch.last_fallbacks()
+    # quality of the guess itself
+    pd = torch.zeros(x.shape[0], dtype=torch.float32, device="cuda")
+    xh = torch.from_numpy(x).to("cuda")
+    mlp.forward(xh.data_ptr(), x.shape[0], pd.data_ptr())
+    guess = pd.cpu().numpy().astype(np.float64) * sc
+    off = np.concatenate([[0], np.cumsum([s.nr_ctrl_points for s in batch.instances])])
+    guess_err = np.array([np.max(np.abs(guess[off[i]:off[i + 1]] - gref[i])) / np.max(np.abs(gref[i]))
+                          for i in range(n_inst)])
+    it_cold = np.array([r.iterations for r in cold], dtype=np.float64)
+    it_warm = np.array([r.iterations for r in warm], dtype=np.float64)
+    dev_c = max(float(np.max(np.abs(c.force_input.circulation_strength - g)) / np.max(np.abs(g)))
+                for c, g in zip(cold, gref))
+    dev_w = max(float(np.max(np.abs(w.force_input.circulation_strength - g)) / np.max(np.abs(g)))
+                for w, g in zip(warm, gref))
+    # plain: each instance's cold first-step solve alone, same settings
+    batch.set_warm_start(None, 0)
+    batch.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i, s in enumerate(batch.instances):
+        fs = S.uniform_freestream(s.nr_freestream_points(), vel[i])
+        s.do_step(time=0.0, time_step=dts[i], freestream_velocity=fs)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    return {
+        "workload": (f"{n_inst} polar-sweep instances (seed {seed}), first-step circulation solves under "
+                     "perturbed inflow (+-6 deg, x0.8-1.2), damping 0.05, stop at max |Gamma_est - Gamma| < "
+                     "1e-6 max |Gamma*| (residual test off)"),
+        "train_solves": int(n_inst * train_conditions),
+        "train_rows": int(X.shape[0]),
+        "train_steps": train_steps,
+        "minibatch": nb,
+        "final_train_mse": loss,
+        "data_s": t_data,
+        "train_s": t_train,
+        "guess_rel_error_median": float(np.median(guess_err)),
+        "guess_rel_error_p90": float(np.percentile(guess_err, 90)),
+        "iterations_cold_mean": float(it_cold.mean()),
+        "iterations_warm_mean": float(it_warm.mean()),
+        "iterations_cold_max": float(it_cold.max()),
+        "iterations_warm_max": float(it_warm.max()),
+        "fallback_rate": fallbacks / n_inst,
+        "max_rel_dev_cold_vs_converged": dev_c,
+        "max_rel_dev_warm_vs_converged": dev_w,
+        "batched_cold_ms": ms_cold,
+        "batched_warm_ms": ms_warm,
+        "plain_cold_ms": 1000.0 * t_plain,
+        "solves_per_s_batched_warm": n_inst / (ms_warm * 1e-3),
+        "solves_per_s_plain_cold": n_inst / t_plain,
+        "batched_warm_vs_plain": (n_inst / (ms_warm * 1e-3)) / (n_inst / t_plain),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +352,8 @@ def main():
     ap.add_argument("--no-fast", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
+    ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--train-conditions", type=int, default=16, help="inflow conditions per instance for training")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU layout (target points "
@@ -338,6 +478,11 @@ def main():
     if not args.no_batch:
         del sim
         line["batched"] = batched_polar_sweep(n_inst=args.batch_instances)
+    if not args.no_warm:
+        import gc
+
+        gc.collect()
+        line["warm_start"] = warm_start_polar(n_inst=args.batch_instances, train_conditions=args.train_conditions)
     print(json.dumps(line))
 
 

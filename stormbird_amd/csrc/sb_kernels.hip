@@ -1195,11 +1195,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // Features of strip i (wing strips [a, b)): for o = -4..4, strip j = i + o
 // inside the wing gives [c_j / mean chord, (j - a + 1/2)/(b - a), twist_j / 0.25,
 // alpha_j / 0.25, |u_j| / mean |u|] (zeros outside the wing), then the wing's
-// aspect ratio / 6 and the number of wings / 6.  u = felt ctrl-point velocity
+// aspect ratio / 6, the number of wings / 6 and its place among the wings.  u = felt ctrl-point velocity
 // without its spanwise part and alpha its angle of attack (the solver's first
 // evaluation with no induced velocity, line_force_model/mod.rs:235-254,
 // force_calculations.rs:50-69); twist = chord angle about the span axis
-// relative to the wing's first strip.  scale = c |u| / 2, so that
+// relative to the wing's first strip; then the wing's centre relative to the
+// centre of all strips along and across the mean felt velocity (in 4 mean
+// chords).  scale = c |u| / 2, so that
 // circulation = prediction * scale.
 constexpr int WS_FEATURES = 64;
 constexpr int WS_BASE = 5;
@@ -1259,7 +1261,21 @@ __device__ inline void features_body(const Inst& d, float* X, float* scale) {
         }
         x[45] = (float)(span / cm / 6.0);
         x[46] = (float)(d.W / 6.0);
-        for (int q = 47; q < WS_FEATURES; ++q) x[q] = 0.0f;
+        // the wing's place among the wings: its centre relative to the centre of
+        // all control points, along and across the mean felt velocity, in chords
+        V3 cw{0.0, 0.0, 0.0}, ca{0.0, 0.0, 0.0}, ua{0.0, 0.0, 0.0};
+        for (int j = 0; j < N; ++j) {
+            V3 cj = load3(geo + d.gl.ctrl + 3 * j);
+            ca = ca + cj;
+            ua = ua + load3(d.felt + 3 * j);
+            if (j >= a && j < b) cw = cw + cj;
+        }
+        V3 off = cw / (double)(b - a) - ca / (double)N;
+        V3 along = normalize(ua);
+        V3 across = normalize(cross(load3(geo + d.gl.dir_g + 3 * i), along));
+        x[47] = (float)(dot(off, along) / cm / 4.0);
+        x[48] = (float)(dot(off, across) / cm / 4.0);
+        for (int q = 49; q < WS_FEATURES; ++q) x[q] = 0.0f;
     }
 }
 
