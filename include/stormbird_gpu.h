@@ -222,8 +222,10 @@ int sb_mlp_gradients(sb_mlp* m, const float* x, const float* t, long n, float* g
 /* One Adam step (beta 0.9/0.999, eps 1e-8) on the mean squared error. */
 int sb_mlp_train_step(sb_mlp* m, const float* x, const float* t, long n, float learning_rate, float* loss);
 
-/* mode 0: off; 1: at the first step; 2: at every step.  The network must
- * outlive its use by the batch. */
+/* mode 0: off; 1: at the first step; 2: at every step; 3: at every step,
+ * judged: per instance the solver's first evaluation keeps whichever of the
+ * default guess (previous circulation) and the prediction has the smaller
+ * max |Gamma_est - Gamma|.  The network must outlive its use by the batch. */
 int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode);
 /* Record the per-strip features at every step (also when no network is set). */
 int sb_batch_set_record_features(sb_batch* b, int on);
@@ -231,8 +233,11 @@ long sb_batch_nr_strips(const sb_batch* b);
 /* Features [strips][64] and scale = c |u| / 2 [strips] of the last step, in
  * instance order. */
 int sb_batch_get_features(const sb_batch* b, float* features, float* scale, size_t cap_strips);
-/* Instances that kept their default initial guess in the last step. */
+/* Instances that kept their default initial guess in the last step because
+ * the prediction was not usable (not finite, or |prediction| > 10). */
 long sb_batch_last_fallbacks(const sb_batch* b);
+/* Instances whose last solve started from the network's guess. */
+long sb_batch_last_warm_taken(const sb_batch* b);
 
 #ifdef __cplusplus
 }

@@ -110,6 +110,8 @@ def lib():
         L.sb_batch_get_features.argtypes = [vp, _fp, _fp, ctypes.c_size_t]
         L.sb_batch_last_fallbacks.restype = ctypes.c_long
         L.sb_batch_last_fallbacks.argtypes = [vp]
+        L.sb_batch_last_warm_taken.restype = ctypes.c_long
+        L.sb_batch_last_warm_taken.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -412,7 +414,8 @@ class Batch:
             _err("sb_batch_reset", rc)
 
     def set_warm_start(self, mlp, mode: int = 1):
-        """mode 0 off, 1 first step, 2 every step (keeps a reference to the network)."""
+        """mode 0 off, 1 first step, 2 every step, 3 every step judged by the solver's first
+        evaluation (keeps a reference to the network)."""
         self._mlp = mlp if mode else None
         rc = lib().sb_batch_set_warm_start(self._h, mlp._h if mlp is not None else None, int(mode))
         if rc != 0:
@@ -438,6 +441,9 @@ class Batch:
 
     def last_fallbacks(self) -> int:
         return lib().sb_batch_last_fallbacks(self._h)
+
+    def last_warm_taken(self) -> int:
+        return lib().sb_batch_last_warm_taken(self._h)
 
 
 class Mlp:

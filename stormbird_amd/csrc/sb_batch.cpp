@@ -56,7 +56,7 @@ struct sb_batch {
     int use_graph = 1;
     // learned warm start
     sb_mlp* mlp = nullptr;
-    int warm_mode = 0;          // 0 off, 1 first step only, 2 every step
+    int warm_mode = 0;          // 0 off, 1 first step only, 2 every step, 3 every step judged
     bool record_features = false;
     long n_strips = 0;
     long* d_strip_off = nullptr;
@@ -176,9 +176,9 @@ struct sb_batch {
         d_feat = dalloc<float>(n_strips * MLP_FEATURES);
         d_scale = dalloc<float>(n_strips);
         d_pred = dalloc<float>(n_strips);
-        d_fallbacks = dalloc<unsigned>(1);
-        h_fallbacks = halloc<unsigned>(1);
-        *h_fallbacks = 0;
+        d_fallbacks = dalloc<unsigned>(2);
+        h_fallbacks = halloc<unsigned>(2);
+        h_fallbacks[0] = h_fallbacks[1] = 0;
         bd.strip_off = d_strip_off;
         bd.n_strips = n_strips;
         bd.feat = d_feat;
@@ -260,7 +260,7 @@ struct sb_batch {
                              fs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
               "upload freestream");
         check(hipEventRecord(ev[0], stream), "event");
-        const bool apply = mlp && (warm_mode == 2 || (warm_mode == 1 && first));
+        const bool apply = mlp && (warm_mode >= 2 || (warm_mode == 1 && first));
         const bool feats = apply || record_features;
         last_applied = apply;
         long sig[3] = {first ? 1 : 0, bd.n_fixed_jobs, bd.n_target_jobs * 4 + (apply ? 2 : 0) + (feats ? 1 : 0)};
@@ -312,10 +312,8 @@ struct sb_batch {
         BatchDev b = bd;
         b.inst = d_inst;
         check(launch_batch_prepare(b, first ? 1 : 0, stream), "prepare");
-        if (feats) check(launch_batch_warm_start(b, mlp, apply ? 1 : 0, stream), "warm start");
-        if (apply)
-            check(hipMemcpyAsync(h_fallbacks, d_fallbacks, sizeof(unsigned), hipMemcpyDeviceToHost, stream),
-                  "download");
+        const bool judged = apply && warm_mode == 3;
+        if (feats) check(launch_batch_warm_start(b, mlp, apply ? (judged ? 2 : 1) : 0, stream), "warm start");
         if (first) check(launch_batch_init_wake(b, stream), "init wake");
         check(launch_batch_wake_step(b, stream), "wake step");
         b.inst = d_inst + 1L * n;
@@ -323,6 +321,10 @@ struct sb_batch {
         b.inst = d_inst + 2L * n;
         check(launch_batch_fixed(b, mode, stream), "fixed velocities");
         check(launch_batch_factors(b, stream), "factors");
+        if (judged) check(launch_batch_warm_judge(b, stream), "warm judge");
+        if (apply)
+            check(hipMemcpyAsync(h_fallbacks, d_fallbacks, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, stream),
+                  "download");
         check(launch_batch_solve(b, stream), "solve");
         b.inst = d_inst + 3L * n;
         check(launch_batch_velocity_copy(b, stream), "velocity copy");
@@ -385,7 +387,7 @@ int sb_batch_reset(sb_batch* b) {
 
 int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode) {
     if (!b) return -1;
-    if (mode < 0 || mode > 2 || (mode > 0 && !m)) return -4;
+    if (mode < 0 || mode > 3 || (mode > 0 && !m)) return -4;
     return guarded([&] {
         if (m) mlp_reserve(m, b->n_strips);
         b->mlp = mode > 0 ? m : nullptr;
@@ -415,7 +417,13 @@ int sb_batch_get_features(const sb_batch* b, float* features, float* scale, size
 
 long sb_batch_last_fallbacks(const sb_batch* b) {
     if (!b) return -1;
-    return b->last_applied ? (long)*b->h_fallbacks : 0;
+    return b->last_applied ? (long)b->h_fallbacks[0] : 0;
+}
+
+long sb_batch_last_warm_taken(const sb_batch* b) {
+    if (!b) return -1;
+    if (!b->last_applied) return 0;
+    return b->warm_mode == 3 ? (long)b->h_fallbacks[1] : (long)b->n - (long)b->h_fallbacks[0];
 }
 
 }  // extern "C"

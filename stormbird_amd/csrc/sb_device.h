@@ -164,7 +164,8 @@ struct BatchDev {
     float* feat = nullptr;
     float* scale = nullptr;
     float* pred = nullptr;
-    unsigned* fallbacks = nullptr;    // device [1]: instances that kept their default guess
+    unsigned* fallbacks = nullptr;    // device [2]: instances that kept their default guess (invalid
+                                      // prediction); instances started from the prediction (judged mode)
 };
 
 hipError_t launch_batch_prepare(const BatchDev& b, int first_step, hipStream_t st);
@@ -177,9 +178,13 @@ hipError_t launch_batch_solve(const BatchDev& b, hipStream_t st);
 hipError_t launch_batch_velocity_copy(const BatchDev& b, hipStream_t st);
 hipError_t launch_batch_self_induction(const BatchDev& b, int mode, hipStream_t st);
 hipError_t launch_batch_forces(const BatchDev& b, hipStream_t st);
-// Features of every strip (kb_features) and, when `apply`, the network's
-// initial circulation written into prev_gamma (after the prepare kernel).
+// Features of every strip (kb_features) and, when `apply` is 1, the network's
+// initial circulation written into prev_gamma (after the prepare kernel); with
+// `apply` 2 only the prediction is made, for launch_batch_warm_judge.
 hipError_t launch_batch_warm_start(const BatchDev& b, sb_mlp* mlp, int apply, hipStream_t st);
+// Per instance, the better of the default guess and the prediction by the
+// solver's first evaluation (after the factors, before the solve).
+hipError_t launch_batch_warm_judge(const BatchDev& b, hipStream_t st);
 // Resident workgroups of the persistent pair-sum kernels (0 fixed, 1 self-induction).
 int batch_resident_groups(int which, int mode, int sym);
 

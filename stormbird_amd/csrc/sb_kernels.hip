@@ -778,15 +778,7 @@ __device__ inline double circulation(const Strip& s, int viscous, double a, V3 u
     return -0.5 * s.c * len(u) * cl;
 }
 
-__device__ inline void solve_body(const Inst& d, double* sh) {
-    int N = d.N;
-    double* G = sh;             // [N] current circulation (broadcast for the matvec)
-    double* R = sh + N;         // [N] residual terms
-    double* wm = R + N;         // [16] per-wave maxima
-    double* ctl = wm + 16;      // [0] residual, [1] converged flag
-    int i = threadIdx.x;
-    bool act = i < N;
-    int ic = act ? i : N - 1;
+__device__ inline Strip load_strip(const Inst& d, int ic) {
     const double* geo = d.geo;
     Strip s;
     s.felt = load3(d.felt + 3 * ic);
@@ -798,6 +790,55 @@ __device__ inline void solve_body(const Inst& d, double* sh) {
     s.c = geo[d.gl.chord_len + ic];
     s.local_len = geo[d.gl.local_len + ic];
     s.foil = d.foil[d.wing_of[ic]];
+    return s;
+}
+
+// One evaluation of the fixed-point map for strip i from the circulations in G
+// (simple_iterative.rs:110-160): induced velocity (frozen_wake.rs:268-301),
+// velocity correction (velocity_corrections.rs:18-100), spanwise part removed,
+// angle of attack, circulation estimate.
+__device__ inline double estimate_circulation(const Inst& d, const Strip& s, const double* G, int i, V3& u,
+                                              double& a) {
+    const int N = d.N;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const double* Fi = d.F + 3 * i;
+    for (int j = 0; j < N; ++j) {
+        const double* f = Fi + 3L * N * j;
+        double gj = G[j];
+        sx += f[0] * gj;
+        sy += f[1] * gj;
+        sz += f[2] * gj;
+    }
+    V3 ind = s.fixed;
+    ind.x += sx;
+    ind.y += sy;
+    ind.z += sz;
+    if (d.vc_mode == 1) {
+        double um = len(ind), fm = len(s.felt);
+        double mx = d.vc_ratio * fm;
+        V3 cvel = um > mx ? normalize(ind) * mx : ind;
+        u = s.felt + cvel;
+    } else if (d.vc_mode == 2) {
+        V3 tot = s.felt + ind;
+        u = normalize(tot) * len(s.felt);
+    } else {
+        u = s.felt + ind;
+    }
+    u = u - project(u, s.rel_g);  // line_force_model/mod.rs:235-254
+    a = len(u) > MIN_POSITIVE ? signed_angle_between(s.chord_g, u, s.dir_g) : 0.0;
+    return circulation(s, d.viscous_lift_in_circulation, a, u);
+}
+
+__device__ inline void solve_body(const Inst& d, double* sh) {
+    int N = d.N;
+    double* G = sh;             // [N] current circulation (broadcast for the matvec)
+    double* R = sh + N;         // [N] residual terms
+    double* wm = R + N;         // [16] per-wave maxima
+    double* ctl = wm + 16;      // [0] residual, [1] converged flag
+    int i = threadIdx.x;
+    bool act = i < N;
+    int ic = act ? i : N - 1;
+    Strip s = load_strip(d, ic);
     double rho = d.density;
 
     double g = d.prev_gamma[ic];
@@ -822,35 +863,7 @@ __device__ inline void solve_body(const Inst& d, double* sh) {
         __syncthreads();
         double ad = 0.0;
         if (act) {
-            // frozen_wake.rs:268-301
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            const double* Fi = d.F + 3 * i;
-            for (int j = 0; j < N; ++j) {
-                const double* f = Fi + 3L * N * j;
-                double gj = G[j];
-                sx += f[0] * gj;
-                sy += f[1] * gj;
-                sz += f[2] * gj;
-            }
-            V3 ind = s.fixed;
-            ind.x += sx;
-            ind.y += sy;
-            ind.z += sz;
-            // velocity_corrections.rs:18-100
-            if (d.vc_mode == 1) {
-                double um = len(ind), fm = len(s.felt);
-                double mx = d.vc_ratio * fm;
-                V3 cvel = um > mx ? normalize(ind) * mx : ind;
-                u = s.felt + cvel;
-            } else if (d.vc_mode == 2) {
-                V3 tot = s.felt + ind;
-                u = normalize(tot) * len(s.felt);
-            } else {
-                u = s.felt + ind;
-            }
-            u = u - project(u, s.rel_g);  // line_force_model/mod.rs:235-254
-            a = len(u) > MIN_POSITIVE ? signed_angle_between(s.chord_g, u, s.dir_g) : 0.0;
-            est = circulation(s, d.viscous_lift_in_circulation, a, u);
+            est = estimate_circulation(d, s, G, i, u, a);
             R[i] = residual_term(s, rho, g, a, u);
             double diff = est - g;
             ad = fabs(diff);
@@ -1307,8 +1320,63 @@ __global__ void kb_warm_apply(const Inst* I, const long* strip_off, const float*
     for (int i = threadIdx.x; i < d.N; i += blockDim.x) d.prev_gamma[i] = (double)pred[o + i] * (double)scale[o + i];
 }
 
+// Judged warm start: the solver's own first evaluation decides between the
+// default initial guess (previous circulation) and the network's guess, per
+// instance: the one with the smaller max |Gamma_est - Gamma| is kept.  A guess
+// that is not finite or exceeds 10 in magnitude is never taken.  Runs after
+// the fixed velocities and factors, before the solve; `taken` counts the
+// instances that start from the network's guess.
+__global__ __launch_bounds__(1024) void kb_warm_judge(const Inst* I, const long* strip_off, const float* pred,
+                                                      const float* scale, unsigned* taken, unsigned* fallbacks) {
+    extern __shared__ double sh[];
+    const Inst d = I[blockIdx.x];
+    const int N = d.N;
+    double* G = sh;        // [N]
+    double* wm = sh + N;   // [16]
+    __shared__ int bad;
+    __shared__ double best[2];
+    long o = strip_off[blockIdx.x];
+    int i = threadIdx.x;
+    bool act = i < N;
+    int ic = act ? i : N - 1;
+    if (i == 0) bad = fabs(d.correction_factor - 1.0) > MIN_POSITIVE ? 1 : 0;
+    __syncthreads();
+    if (act && !(fabsf(pred[o + i]) <= 10.0f)) atomicOr(&bad, 1);
+    __syncthreads();
+    if (bad) {
+        if (i == 0) __hip_atomic_fetch_add(fallbacks, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    Strip s = load_strip(d, ic);
+    double cand[2] = {d.prev_gamma[ic], (double)pred[o + ic] * (double)scale[o + ic]};
+    for (int c = 0; c < 2; ++c) {
+        if (act) G[i] = cand[c];
+        __syncthreads();
+        double ad = 0.0;
+        if (act) {
+            V3 u{0.0, 0.0, 0.0};
+            double a = 0.0;
+            ad = fabs(estimate_circulation(d, s, G, i, u, a) - cand[c]);
+        }
+        double m = ad;
+        for (int off = WAVE / 2; off > 0; off >>= 1) m = fmax_(m, __shfl_xor(m, off, WAVE));
+        if ((i % WAVE) == 0) wm[i / WAVE] = m;
+        __syncthreads();
+        if (i == 0) {
+            double mx = 0.0;
+            for (int w = 0; w < (int)((blockDim.x + WAVE - 1) / WAVE); ++w) mx = fmax_(mx, wm[w]);
+            best[c] = mx;
+        }
+        __syncthreads();
+    }
+    if (best[1] < best[0]) {
+        if (act) d.prev_gamma[i] = cand[1];
+        if (i == 0) __hip_atomic_fetch_add(taken, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 __global__ void kb_zero_u32(unsigned* c) {
-    if (threadIdx.x == 0) c[0] = 0u;
+    if (threadIdx.x < 2) c[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1515,7 +1583,16 @@ hipError_t launch_batch_warm_start(const BatchDev& b, sb_mlp* mlp, int apply, hi
     hipLaunchKernelGGL(kb_zero_u32, dim3(1), dim3(64), 0, st, b.fallbacks);
     e = mlp_forward_on(mlp, b.feat, b.n_strips, b.pred, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kb_warm_apply, dim3(b.n), dim3(256), 0, st, b.inst, b.strip_off, b.pred, b.scale, b.fallbacks);
+    if (apply == 1)
+        hipLaunchKernelGGL(kb_warm_apply, dim3(b.n), dim3(256), 0, st, b.inst, b.strip_off, b.pred, b.scale,
+                           b.fallbacks);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_warm_judge(const BatchDev& b, hipStream_t st) {
+    size_t shm = sizeof(double) * ((size_t)b.max_threads + 16);
+    hipLaunchKernelGGL(kb_warm_judge, dim3(b.n), dim3(b.max_threads), shm, st, b.inst, b.strip_off, b.pred, b.scale,
+                       b.fallbacks + 1, b.fallbacks);
     return hipGetLastError();
 }
 

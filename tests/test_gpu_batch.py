@@ -94,3 +94,29 @@ def test_batch_rejects_bad_input():
     batch = sb.Batch([js, js])
     with pytest.raises(RuntimeError):
         batch.do_step(times=None, time_steps=0.1, freestream_velocity=np.zeros((3, 3)))
+
+
+def test_route_simulation_batch_equals_alone():
+    """Rolling ships with gusts and free wakes: batched steps equal each ship stepped alone."""
+    from stormbird_amd.route import RouteSimulation
+
+    route = RouteSimulation(n_ships=3, seed=5, nr_strips=8, nr_panels=12)
+    alone = [sb.Simulation(js, mode=sb.MODE_EXACT) for js in
+             [__import__("stormbird_amd.route", fromlist=["ship_setup"]).ship_setup(a, 8, 12)[0] for a in route.awa]]
+    for k in range(7):
+        print(f"route step {k}", flush=True)
+        rb = route.step()
+        for i, sim in enumerate(alone):
+            t = (route.k - 1) * route.dt[i]
+            sim.set_translation_and_rotation_with_finite_difference_for_the_velocity(
+                float(route.dt[i]), [0.0, 0.0, 0.0], [route.roll(t), 0.0, 0.0])
+            fs = route.freestream_host(i)
+            ra = sim.do_step(time=t, time_step=float(route.dt[i]), freestream_velocity=fs)
+            compare_results(rb[i], ra, exact=True, where=f"step {k} ship {i}")
+    # a judged warm start (untrained network) changes only the initial guess
+    m = sb.Mlp(seed=1)
+    route.batch.set_warm_start(m, 3)
+    res = route.step()
+    assert route.batch.last_warm_taken() + route.batch.last_fallbacks() <= 3
+    for r in res:
+        assert np.all(np.isfinite(r.force_input.circulation_strength))
