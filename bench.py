@@ -312,7 +312,7 @@ def warm_start_polar(n_inst=512, seed=3, train_conditions=16, train_steps=3000, 
         s.do_step(time=0.0, time_step=dts[i], freestream_velocity=fs)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    return {
+    result = {
         "workload": (f"{n_inst} polar-sweep instances (seed {seed}), first-step circulation solves under "
                      "perturbed inflow (+-6 deg, x0.8-1.2), damping 0.05, stop at max |Gamma_est - Gamma| < "
                      "1e-6 max |Gamma*| (residual test off)"),
@@ -339,6 +339,67 @@ def warm_start_polar(n_inst=512, seed=3, train_conditions=16, train_steps=3000, 
         "solves_per_s_plain_cold": n_inst / t_plain,
         "batched_warm_vs_plain": (n_inst / (ms_warm * 1e-3)) / (n_inst / t_plain),
     }
+    return result, mlp
+
+
+def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
+    """Route simulation: 64 ships x 4 wing sails x 48 strips, free wake of 1,500 rows
+    (self-induction on every wake point), von Karman gusts (15 %, 200 m), roll 10 deg at
+    12 s; timed from a synthesized developed wake (3 real steps, then every row shed) and
+    extrapolated to 10,000 steps.  With a trained network, one more step runs with the
+    judged per-step warm start."""
+    import torch
+
+    from stormbird_amd.route import RouteSimulation
+
+    t0 = time.perf_counter()
+    route = RouteSimulation(n_ships=n_ships, seed=seed)
+    for _ in range(3):
+        route.step(want_result=False)
+    drng = np.random.default_rng(seed + 100)
+    for s in route.batch.instances:
+        develop_wake(s, 3, drng)
+    t_setup = time.perf_counter() - t0
+    pairs = 0.0
+    for s in route.batch.instances:
+        n, P = s.nr_ctrl_points, s.nr_span_points
+        M = s.nr_panels // n
+        pairs += float(P * (M + 1)) * float(n * M) + float(n * n * (M - 1)) + float(n * n)
+    route.step(want_result=False)  # captures the step graph
+    torch.cuda.synchronize()
+    dev_ms, iters = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res = route.step(want_result=True)
+        dev_ms.append(route.batch.last_step_ms())
+        iters.append(np.mean([r.iterations for r in res]))
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    out = {
+        "workload": (f"{n_ships} ships (seed {seed}) x 4 wing sails 30x10 m x 48 strips, free wake of 1500 rows, "
+                     "all wake points self-induced, von Karman gusts 15 % / 200 m, roll 10 deg at 12 s, "
+                     "developed-wake steady state"),
+        "n_ships": n_ships,
+        "biot_savart_pairs_per_batched_step": pairs,
+        "batched_s_per_step": t,
+        "batched_device_ms_per_step": float(np.mean(dev_ms)),
+        "ship_steps_per_s": n_ships / t,
+        "pairs_per_s": pairs / t,
+        "fp64_tflops_far_field_estimate": pairs * FLOP_FAR / t / 1e12,
+        "extrapolated_10000_steps_h": 10000 * t / 3600.0,
+        "iterations_mean": float(np.mean(iters)),
+        "setup_s": t_setup,
+        "freestream": "resident in device memory, refilled per ship per step (inside the timed step)",
+    }
+    if mlp is not None:
+        route.batch.set_warm_start(mlp, 3)
+        route.step(want_result=False)  # captures the warm-start graph
+        res = route.step(want_result=True)
+        out["judged_warm_start"] = {"iterations_mean": float(np.mean([r.iterations for r in res])),
+                                    "taken_fraction": route.batch.last_warm_taken() / n_ships,
+                                    "fallbacks": route.batch.last_fallbacks(),
+                                    "device_ms_per_step": route.batch.last_step_ms()}
+    return out
 
 
 def main():
@@ -353,7 +414,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--no-warm", action="store_true")
+    ap.add_argument("--no-route", action="store_true")
+    ap.add_argument("--route-ships", type=int, default=64)
     ap.add_argument("--train-conditions", type=int, default=16, help="inflow conditions per instance for training")
+    ap.add_argument("--only", choices=("batched", "warm", "route"), default=None,
+                    help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU layout (target points "
@@ -368,6 +433,16 @@ def main():
         sys.exit(1)
     import stormbird_amd as sb
     from stormbird_amd import setups as S
+
+    if args.only:
+        if args.only == "batched":
+            out = batched_polar_sweep(n_inst=args.batch_instances, steps=args.steps)
+        elif args.only == "warm":
+            out, _ = warm_start_polar(n_inst=args.batch_instances, train_conditions=args.train_conditions)
+        else:
+            out = route_simulation(n_ships=args.route_ships, steps=args.steps)
+        print(json.dumps({"section": args.only, **out}))
+        return
 
     js, case = S.four_sail_free_wake(nr_panels=args.panels, nr_strips=args.strips)
     dt = case["time_step"]
@@ -475,6 +550,7 @@ def main():
         "fp32_kahan_mode": fp32,
         "cpu_baseline": cpu,
     }
+    mlp = None
     if not args.no_batch:
         del sim
         line["batched"] = batched_polar_sweep(n_inst=args.batch_instances)
@@ -482,7 +558,13 @@ def main():
         import gc
 
         gc.collect()
-        line["warm_start"] = warm_start_polar(n_inst=args.batch_instances, train_conditions=args.train_conditions)
+        line["warm_start"], mlp = warm_start_polar(n_inst=args.batch_instances,
+                                                   train_conditions=args.train_conditions)
+    if not args.no_route:
+        import gc
+
+        gc.collect()
+        line["route"] = route_simulation(n_ships=args.route_ships, mlp=mlp)
     print(json.dumps(line))
 
 

@@ -458,9 +458,9 @@ class Mlp:
         self.nr_params = L.sb_mlp_nr_params(self._h)
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            lib().sb_mlp_destroy(self._h)
-            self._h = None
+        h, self._h = getattr(self, "_h", None), None
+        if h and _lib is not None:
+            _lib.sb_mlp_destroy(h)
 
     def get_params(self) -> np.ndarray:
         out = np.zeros(self.nr_params, dtype=np.float32)
