@@ -501,7 +501,8 @@ __global__ __launch_bounds__(256) void k_fold(Inst d, const double* __restrict__
 //               a skipped panel (zero strength, or the target's own wing in
 //               neglect-self mode) writes +0;
 //  k_fold_terms one lane per (component, target) chain adds the chunk's terms
-//               in ascending panel order to the running sum acc[c][t].
+//               in ascending panel order to the running sum acc[c][t] (fed
+//               through LDS by a whole workgroup of loaders).
 // Adding +0 leaves a sum unchanged unless the sum is -0, and a sum that starts
 // at +0 never becomes -0, so the result equals the reference's fold that
 // skips those panels (induced_velocity_calc.rs:111-150, 202-210).
@@ -533,17 +534,59 @@ __global__ __launch_bounds__(256) void k_terms(Inst d, const double* __restrict_
     }
 }
 
-__global__ __launch_bounds__(64) void k_fold_terms(const double* __restrict__ terms, int ld, int n_targets, long cnt,
-                                                   int first, double* __restrict__ acc, double* __restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= 3 * n_targets) return;
-    int c = i / n_targets, t = i % n_targets;
-    const double* src = terms + c * cnt * ld + t;
-    double s = first ? 0.0 : acc[i];
-#pragma unroll 8
-    for (long q = 0; q < cnt; ++q) s = s + src[q * ld];
-    acc[i] = s;
-    if (out) out[3 * t + c] = s;
+// One workgroup (16 waves) per 16 chains of one component: every thread keeps
+// FOLD_K loads of the next 512-row block in flight while lanes 0-15 of wave 0
+// add the current block (double-buffered in LDS) in ascending row order.
+constexpr int FOLD_CH = 16;
+constexpr int FOLD_K = 8;
+constexpr int FOLD_THREADS = 1024;
+constexpr int FOLD_ROWS = FOLD_THREADS * FOLD_K / FOLD_CH;  // 512
+
+__global__ __launch_bounds__(1024) void k_fold_terms(const double* __restrict__ terms, int ld, int n_targets,
+                                                     long cnt, int first, double* __restrict__ acc,
+                                                     double* __restrict__ out) {
+    __shared__ double buf[2][FOLD_ROWS][FOLD_CH];
+    const int groups = (n_targets + FOLD_CH - 1) / FOLD_CH;
+    const int c = blockIdx.x / groups;
+    const int t0 = (blockIdx.x % groups) * FOLD_CH;
+    const double* src = terms + (long)c * cnt * ld + t0;  // t0 + FOLD_CH <= ld: loads stay in the chunk
+    const int lc = threadIdx.x % FOLD_CH, r0 = threadIdx.x / FOLD_CH;
+    constexpr int RSTEP = FOLD_THREADS / FOLD_CH;  // 64
+    const long nb = (cnt + FOLD_ROWS - 1) / FOLD_ROWS;
+    double r[FOLD_K];
+    auto load = [&](long b) {
+#pragma unroll
+        for (int k = 0; k < FOLD_K; ++k) {
+            long q = b * FOLD_ROWS + r0 + RSTEP * k;
+            r[k] = q < cnt ? src[q * ld + lc] : 0.0;
+        }
+    };
+    auto store = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < FOLD_K; ++k) buf[slot][r0 + RSTEP * k][lc] = r[k];
+    };
+    const int t = t0 + (int)threadIdx.x;
+    const bool folder = threadIdx.x < FOLD_CH && t < n_targets;
+    double s = (folder && !first) ? acc[(long)c * n_targets + t] : 0.0;
+    if (nb > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (long b = 0; b < nb; ++b) {
+        if (b + 1 < nb) load(b + 1);
+        if (folder) {
+            int rows = cnt - b * FOLD_ROWS < FOLD_ROWS ? (int)(cnt - b * FOLD_ROWS) : FOLD_ROWS;
+            const double(*cur)[FOLD_CH] = buf[b & 1];
+            for (int q = 0; q < rows; ++q) s = s + cur[q][threadIdx.x];
+        }
+        if (b + 1 < nb) store((int)((b + 1) & 1));
+        __syncthreads();
+    }
+    if (folder) {
+        acc[(long)c * n_targets + t] = s;
+        if (out) out[3 * t + c] = s;
+    }
 }
 
 // frozen_wake.rs:198-261: F[j][i] = unit velocity of first-row panel j at ctrl point i.
@@ -1433,8 +1476,8 @@ hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st
                 hipLaunchKernelGGL(k_terms<MODE_EXACT>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt, d.neglect_self,
                                    fb->terms, ld);
             bool last = b + cnt >= p_end;
-            hipLaunchKernelGGL(k_fold_terms, dim3(blocks_for(3L * d.N, 64)), dim3(64), 0, st, fb->terms, ld, d.N,
-                               cnt, first, fb->acc, last ? d.fixed : nullptr);
+            hipLaunchKernelGGL(k_fold_terms, dim3(3 * ((d.N + FOLD_CH - 1) / FOLD_CH)), dim3(FOLD_THREADS), 0, st,
+                               fb->terms, ld, d.N, cnt, first, fb->acc, last ? d.fixed : nullptr);
             first = 0;
         }
         return hipGetLastError();
