@@ -105,6 +105,18 @@ int sb_sim_set_previous_circulation(sb_sim* sim, const double* gamma, size_t n);
 
 /* LineForceModel::set_translation_and_rotation_with_finite_difference_for_the_velocity
  * (line_force_model/data_update.rs:188-205). */
+/* Wake export in the reference's formats (wake/dynamic_wake/export.rs:30-210):
+ * DynamicWake::write_wake_to_obj_file, ::write_wake_to_vtk_file, and the panel
+ * corner indices [nr_panels][4] export_to_plotly_mesh builds its triangles from
+ * (panel_point_indices, dynamic_wake/mod.rs:82-91).  Numbers print as Rust's
+ * `{}` prints an f64.  With `write_wake_data_to_file` set, every step writes
+ * `{wake_files_folder_path}/wake_{(time / time_step) as usize}.vtp`
+ * (simulation.rs:181-186). */
+int sb_sim_write_wake_obj(const sb_sim* sim, const char* path);
+int sb_sim_write_wake_vtk(const sb_sim* sim, const char* path);
+int sb_sim_get_panel_point_indices(const sb_sim* sim, long* indices, size_t cap);
+/* Rust `{}` formatting of one f64 into buf (returns its length); test hook. */
+int sb_debug_format_f64(double value, char* buf, size_t cap);
 /* Back to the state of a newly built simulation: straight initial wake, zero
  * strengths, no completed step (the line-force model's motion is kept).  No
  * counterpart in the reference (which builds a new Simulation); it lets one

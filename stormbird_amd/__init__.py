@@ -90,6 +90,10 @@ def lib():
         L.sb_batch_last_step_ms.restype = ctypes.c_double
         L.sb_batch_last_step_ms.argtypes = [vp]
         L.sb_batch_reset.argtypes = [vp]
+        L.sb_sim_write_wake_obj.argtypes = [vp, ctypes.c_char_p]
+        L.sb_sim_write_wake_vtk.argtypes = [vp, ctypes.c_char_p]
+        L.sb_sim_get_panel_point_indices.argtypes = [vp, ctypes.POINTER(ctypes.c_long), ctypes.c_size_t]
+        L.sb_debug_format_f64.argtypes = [ctypes.c_double, ctypes.c_char_p, ctypes.c_size_t]
         L.sb_sim_reset.argtypes = [vp]
         L.sb_sim_set_solver_settings.argtypes = [vp, ctypes.c_long, ctypes.c_double, ctypes.c_double, ctypes.c_double]
         _fp = ctypes.POINTER(ctypes.c_float)
@@ -272,6 +276,38 @@ class Simulation:
         if rc != 0:
             _err("get_frozen_wake", rc)
         return fixed, fac
+
+    def write_wake_to_obj_file(self, path: str):
+        """DynamicWake::write_wake_to_obj_file (wake/dynamic_wake/export.rs:30-64)."""
+        rc = lib().sb_sim_write_wake_obj(self._h, str(path).encode())
+        if rc != 0:
+            _err("write_wake_to_obj_file", rc)
+
+    def write_wake_to_vtk_file(self, path: str):
+        """DynamicWake::write_wake_to_vtk_file (wake/dynamic_wake/export.rs:66-159)."""
+        rc = lib().sb_sim_write_wake_vtk(self._h, str(path).encode())
+        if rc != 0:
+            _err("write_wake_to_vtk_file", rc)
+
+    def panel_point_indices(self) -> np.ndarray:
+        out = np.zeros((self.nr_panels, 4), dtype=np.int64)
+        rc = lib().sb_sim_get_panel_point_indices(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), out.size)
+        if rc != 0:
+            _err("panel_point_indices", rc)
+        return out
+
+    def export_to_plotly_mesh(self) -> dict:
+        """DynamicWake::export_to_plotly_mesh (export.rs:165-210): points x, y, z; two triangles
+        (0, 1, 2) and (0, 2, 3) per panel as i, j, k; the panel strength per triangle."""
+        st = self.get_wake_state()
+        p = st["points"]
+        ix = self.panel_point_indices().astype(np.float64)
+        tri_i = np.stack([ix[:, 0], ix[:, 0]], axis=1).reshape(-1)
+        tri_j = np.stack([ix[:, 1], ix[:, 2]], axis=1).reshape(-1)
+        tri_k = np.stack([ix[:, 2], ix[:, 3]], axis=1).reshape(-1)
+        strength = np.repeat(st["strengths"], 2)
+        return {"x": p[:, 0].tolist(), "y": p[:, 1].tolist(), "z": p[:, 2].tolist(), "i": tri_i.tolist(),
+                "j": tri_j.tolist(), "k": tri_k.tolist(), "strength": strength.tolist()}
 
     def reset(self):
         """State of a newly built simulation (straight initial wake, no completed step)."""
@@ -497,3 +533,12 @@ class Mlp:
         if rc != 0:
             _err("sb_mlp_train_step", rc)
         return loss.value
+
+
+def format_f64(v: float) -> str:
+    """The library's Rust-`{}` formatting of an f64 (test hook)."""
+    buf = ctypes.create_string_buffer(512)
+    n = lib().sb_debug_format_f64(float(v), buf, 512)
+    if n < 0:
+        _err("sb_debug_format_f64", n)
+    return buf.value.decode()

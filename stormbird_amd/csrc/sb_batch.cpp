@@ -299,6 +299,7 @@ struct sb_batch {
             s.steps_completed += 1;
             std::copy(h_out + out_off[i], h_out + out_off[i + 1], s.h_out.begin());
             if (results) s.write_result(times ? times[i] : 0.0, results + i);
+            if (s.setup.wake.write_files) write_wake_step_file(s, times ? times[i] : 0.0, dts[i]);
         }
     }
 

@@ -48,6 +48,16 @@ constexpr int kTimers = 6;
 
 }  // namespace sb_host
 
+struct sb_sim;
+
+namespace sb_host {
+// Wake export (sb_export.cpp), in the formats of wake/dynamic_wake/export.rs.
+std::string rust_f64(double v);
+void write_wake_obj(const sb_sim& s, const std::string& path);
+void write_wake_vtk(const sb_sim& s, const std::string& path);
+void write_wake_step_file(const sb_sim& s, double time, double dt);
+}  // namespace sb_host
+
 using namespace sb;
 using namespace sb_host;
 
@@ -386,6 +396,7 @@ struct sb_sim {
             timer_ms[k] = ms;
         }
         write_result(time, out);
+        if (setup.wake.write_files) write_wake_step_file(*this, time, dt);  // simulation.rs:181-186
     }
 
     void write_result(double time, sb_step_result* o) const {

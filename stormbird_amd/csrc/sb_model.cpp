@@ -393,9 +393,11 @@ static void parse_wake(WakeConfig& w, const Node* o, const LineModel& m) {  // b
     if (const Node* v = f("shape_damping_factor")) w.damping = number(v, "shape_damping_factor");
     if (const Node* v = f("neglect_self_induced_velocities")) w.neglect_self = boolean(v, "neglect_self_induced_velocities");
     if (const Node* v = f("initial_relative_wake_length")) w.initial_relative_wake_length = number(v, "initial_relative_wake_length");
-    if (const Node* v = f("write_wake_data_to_file"))
-        if (boolean(v, "write_wake_data_to_file"))
-            throw std::runtime_error("write_wake_data_to_file is not supported on the GPU path");
+    if (const Node* v = f("write_wake_data_to_file")) w.write_files = boolean(v, "write_wake_data_to_file");
+    if (const Node* v = f("wake_files_folder_path")) {
+        if (v->t != Node::Str) throw std::runtime_error("wake_files_folder_path: expected a string");
+        w.files_folder = v->s;
+    }
     int N = m.n();
     long P = (long)m.span_points.size();
     long nr_points = P * (long)(w.M + 1);

@@ -67,6 +67,8 @@ struct WakeConfig {
     double rep_chord = 0.0;
     double initial_relative_wake_length = 50.0;
     std::vector<double> core;  // per panel, [M][N]
+    bool write_files = false;  // write_wake_data_to_file
+    std::string files_folder;  // wake_files_folder_path
 };
 
 struct SolverConfig {
