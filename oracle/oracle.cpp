@@ -713,6 +713,65 @@ struct FrozenWake {
     }
 };
 
+// Quasi-steady wake (wake/settings.rs:33-58) and the steady frozen wake built
+// from horseshoe vortices (frozen_wake.rs:68-129, horseshoe_vortex.rs:23-103).
+struct QuasiSteadyWake {
+    double wake_length_factor = 100.0;
+    Symmetry sym = Symmetry::None;
+    int core_kind = 0;  // 0 Relative, 1 Absolute, 2 NoViscousCore
+    double core_value = 0.1;
+};
+
+struct Horseshoe {
+    V3 b0, b1, s0, s1, e0, e1;  // bound, start trailing, end trailing (two points each)
+    double core;
+    V3 unit(const V3& x) const {  // horseshoe_vortex.rs:23-51
+        V3 u;
+        u += segment_unit_velocity(b0, b1, x, core);
+        u += segment_unit_velocity(s0, s1, x, core);
+        u += segment_unit_velocity(e0, e1, x, core);
+        return u;
+    }
+};
+
+void update_as_steady(FrozenWake& fw, const LineForceModel& lfm, const std::vector<V3>& span_vel,
+                      const QuasiSteadyWake& w) {  // frozen_wake.rs:68-129
+    const size_t n = lfm.n();
+    double csum = 0.0;
+    for (double c : lfm.chord_lengths) csum += c;
+    double avg_chord = csum / (double)n;
+    double wake_length = w.wake_length_factor * avg_chord;
+    std::vector<V3> wv(span_vel.size());
+    for (size_t i = 0; i < span_vel.size(); ++i) wv[i] = normalize(span_vel[i]) * wake_length;
+    double core = 0.0;
+    if (w.core_kind == 0) {
+        double lsum = 0.0;
+        for (size_t i = 0; i < n; ++i) lsum += lfm.span_lines_global[i].len();
+        core = w.core_value * (lsum / (double)n);
+    } else if (w.core_kind == 1) {
+        core = w.core_value;
+    }
+    std::vector<Horseshoe> hs;  // horseshoe_vortex.rs:55-103
+    size_t k = 0;
+    for (size_t wi = 0; wi < lfm.nw(); ++wi) {
+        for (size_t li = lfm.wing_indices[wi].start; li < lfm.wing_indices[wi].end; ++li) {
+            const SpanLine& L = lfm.span_lines_global[li];
+            hs.push_back({L.start, L.end, L.start + wv[k], L.start, L.end, L.end + wv[k + 1], core});
+            k += 1;
+        }
+        k += 1;
+    }
+    for (size_t r = 0; r < n; ++r) {
+        const V3& x = lfm.ctrl_points_global[r];
+        for (size_t c = 0; c < n; ++c) {
+            V3 u = hs[c].unit(x);
+            V3 m;
+            if (mirrored_point(w.sym, x, m)) u = corrected_velocity(w.sym, u, hs[c].unit(m));
+            fw.factors[r * n + c] = u;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Solver (stormbird/src/lifting_line/solvers/{simple_iterative,velocity_corrections}.rs)
 // ---------------------------------------------------------------------------
@@ -734,14 +793,14 @@ struct SolverResult {
     double residual = 0.0;
 };
 
-std::vector<V3> corrected_velocity(const SimpleIterative& s, const std::vector<V3>& fs,
+std::vector<V3> corrected_velocity(VelCorr vc, double vc_ratio, const std::vector<V3>& fs,
                                    const std::vector<V3>& ui) {  // velocity_corrections.rs:18-100
     std::vector<V3> out(fs.size());
-    if (s.vc == VelCorr::MaxRatio) {
+    if (vc == VelCorr::MaxRatio) {
         for (size_t i = 0; i < fs.size(); ++i) {
             double um = length(ui[i]);
             double fm = length(fs[i]);
-            double mx = s.vc_ratio * fm;
+            double mx = vc_ratio * fm;
             V3 c = um > mx ? normalize(ui[i]) * mx : ui[i];
             out[i] = fs[i] + c;
         }
@@ -752,12 +811,116 @@ std::vector<V3> corrected_velocity(const SimpleIterative& s, const std::vector<V
     return out;
 }
 
+// Gaussian elimination with partial pivoting (stormath/src/matrix/linalg.rs:41-104);
+// A row-major n x n.  Throws on a (nearly) singular pivot, where the reference's
+// Linearized solver unwraps an error (linearized.rs:133).
+std::vector<double> solve_gaussian_elimination(const std::vector<double>& A, const std::vector<double>& b,
+                                               size_t n) {
+    const size_t m = n + 1;
+    std::vector<double> aug(n * m);
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t j = 0; j < n; ++j) aug[i * m + j] = A[i * n + j];
+        aug[i * m + n] = b[i];
+    }
+    for (size_t k = 0; k < n; ++k) {
+        size_t max_row = k;
+        for (size_t i = k + 1; i < n; ++i)
+            if (std::fabs(aug[i * m + k]) > std::fabs(aug[max_row * m + k])) max_row = i;
+        if (max_row != k)
+            for (size_t j = 0; j <= n; ++j) std::swap(aug[k * m + j], aug[max_row * m + j]);
+        if (std::fabs(aug[k * m + k]) < 1e-12)
+            throw std::runtime_error("Matrix is singular or nearly singular at row " + std::to_string(k));
+        for (size_t i = k + 1; i < n; ++i) {
+            double factor = aug[i * m + k] / aug[k * m + k];
+            for (size_t j = k; j <= n; ++j) aug[i * m + j] -= factor * aug[k * m + j];
+        }
+    }
+    std::vector<double> x(n, 0.0);
+    for (size_t ii = n; ii-- > 0;) {
+        double sum = 0.0;
+        for (size_t j = ii + 1; j < n; ++j) sum += aug[ii * m + j] * x[j];
+        x[ii] = (aug[ii * m + n] - sum) / aug[ii * m + ii];
+    }
+    return x;
+}
+
+enum class IvcMethod { NoCorrection, FullCorrection };
+
+struct Linearized {  // solvers/linearized.rs:17-48
+    VelCorr vc = VelCorr::None;
+    double vc_ratio = 0.0;
+    bool disable_viscous = false;
+    IvcMethod ivc = IvcMethod::FullCorrection;
+};
+
+// linearized.rs:54-222.  check_for_zero_linear_lift (:224-266) starts its flag at
+// true and so never ends the solve early; it has no other effect.
+SolverResult solve_linearized(const Linearized& s, const LineForceModel& lfm, const std::vector<V3>& felt,
+                              FrozenWake& fw) {
+    const size_t n = lfm.n();
+    std::vector<V3> u = felt;
+    for (size_t i = 0; i < n; ++i) u[i] += fw.fixed[i];
+    u = lfm.remove_span_velocity(u);
+    std::vector<V3> vdir(n), ndir(n);
+    for (size_t i = 0; i < n; ++i) {
+        V3 un = normalize(u[i]);
+        V3 sd = normalize(lfm.span_lines_global[i].rel());
+        vdir[i] = un;
+        ndir[i] = cross(un, sd);
+    }
+    std::vector<double> a = lfm.angles_of_attack(u);
+    std::vector<double> cl_lin(n), dcl(n);
+    for (size_t i = 0; i < n; ++i) {
+        cl_lin[i] = lfm.foils[lfm.wing_of(i)].cl_linear(a[i]);  // force_calculations.rs:188-210
+        dcl[i] = lfm.foils[lfm.wing_of(i)].cl_initial_slope;    // :212-229
+    }
+    std::vector<double> A(n * n), rhs(n);
+    for (size_t r = 0; r < n; ++r) {
+        double ut = dot(u[r], vdir[r]);
+        for (size_t c = 0; c < n; ++c) {
+            double uin = dot(fw.factors[r * n + c], ndir[r]);
+            A[r * n + c] = -dcl[r] * uin / ut;
+            if (r == c) A[r * n + c] += 2.0 / (lfm.chord_lengths[r] * ut);
+        }
+        rhs[r] = -cl_lin[r];
+    }
+    std::vector<double> g = solve_gaussian_elimination(A, rhs, n);
+    fw.matvec(g);
+    auto felt_plus_induced = [&]() {
+        if (s.vc != VelCorr::None) {
+            u = corrected_velocity(s.vc, s.vc_ratio, felt, fw.induced);
+        } else {
+            for (size_t i = 0; i < n; ++i) u[i] = felt[i] + fw.induced[i];
+        }
+    };
+    felt_plus_induced();
+    if (!s.disable_viscous) {
+        a = lfm.angles_of_attack(u);
+        std::vector<double> cl_full = lfm.cl_pre(a);
+        for (size_t i = 0; i < n; ++i) {
+            double cll = lfm.foils[lfm.wing_of(i)].cl_linear(a[i]);
+            if (std::fabs(cll) > MIN_POSITIVE) g[i] *= cl_full[i] / cll;
+        }
+        if (s.ivc == IvcMethod::FullCorrection) fw.matvec(g);
+        felt_plus_induced();
+    }
+    a = lfm.angles_of_attack(u);
+    SolverResult r;
+    r.residual = lfm.average_residual_absolute(g, a, u);
+    r.input_velocity = felt;
+    r.gamma = g;
+    r.output_velocity = u;
+    r.iterations = 1;
+    return r;
+}
+
 SolverResult solve(const SimpleIterative& s, const LineForceModel& lfm, const std::vector<V3>& felt,
                    FrozenWake& fw, const std::vector<double>& initial) {  // simple_iterative.rs:86-212
     size_t n = lfm.n();
     std::vector<double> g = initial;
     if (std::fabs(s.correction_factor_initial - 1.0) > MIN_POSITIVE)
         for (auto& x : g) x *= s.correction_factor_initial;
+    if (s.start_linearized) g = solve_linearized(Linearized(), lfm, felt, fw).gamma;  // :103-113
     std::vector<V3> u(n);
     std::vector<double> a(n, 0.0);
     double residual = lfm.average_residual_absolute(g, a, u);
@@ -766,7 +929,7 @@ SolverResult solve(const SimpleIterative& s, const LineForceModel& lfm, const st
     while (it < s.max_iterations && !converged) {
         it += 1;
         fw.matvec(g);
-        if (s.vc != VelCorr::None) u = corrected_velocity(s, felt, fw.induced);
+        if (s.vc != VelCorr::None) u = corrected_velocity(s.vc, s.vc_ratio, felt, fw.induced);
         else
             for (size_t i = 0; i < n; ++i) u[i] = felt[i] + fw.induced[i];
         u = lfm.remove_span_velocity(u);
@@ -1135,34 +1298,109 @@ void build_wake(DynamicWake& w, const Value* o, const LineForceModel& lfm) {  //
     w.initialize_based_on_chord(lfm, init_len);
 }
 
-SimpleIterative parse_solver(const Value* v) {  // solvers/mod.rs:21-24, simple_iterative.rs:59-84
-    SimpleIterative s;
-    if (!v) return s;
+VelCorr parse_vc(const Value* x, double& ratio) {  // velocity_corrections.rs:9-16
+    const Value* q;
+    std::string vt = enum_tag(x, &q, "VelocityCorrections");
+    if (vt == "NoCorrection") return VelCorr::None;
+    if (vt == "MaxInducedVelocityMagnitudeRatio") {
+        ratio = num(q, "ratio");
+        return VelCorr::MaxRatio;
+    }
+    if (vt == "FixedMagnitudeEqualToFreestream") return VelCorr::FixedMagnitude;
+    throw std::runtime_error("bad VelocityCorrections");
+}
+
+struct SolverSpec {  // solvers/mod.rs:21-55
+    bool linearized = false;
+    SimpleIterative si;
+    Linearized lin;
+};
+
+Linearized parse_linearized(const Value* p) {  // linearized.rs:26-48
+    Linearized l;
+    if (!p) return l;
+    check_keys(p, {"velocity_corrections", "disable_viscous_corrections", "induced_velocity_correction_method"},
+               "Linearized");
+    if (auto* x = p->get("velocity_corrections")) l.vc = parse_vc(x, l.vc_ratio);
+    if (auto* x = p->get("disable_viscous_corrections")) l.disable_viscous = x->kind == Value::Bool && x->b;
+    if (auto* x = p->get("induced_velocity_correction_method")) {
+        const Value* q;
+        std::string t = enum_tag(x, &q, "InducedVelocityCorrectionMethod");
+        if (t == "NoCorrection") l.ivc = IvcMethod::NoCorrection;
+        else if (t == "FullCorrection") l.ivc = IvcMethod::FullCorrection;
+        else throw std::runtime_error("bad InducedVelocityCorrectionMethod");
+    }
+    return l;
+}
+
+// Solver (dynamic settings, simple_iterative.rs:59-84) or QuasiSteadySolverBuilder
+// (quasi_steady: simple_iterative.rs:14-45 with 1000 iterations and damping 0.04;
+// default Linearized, solvers/mod.rs:31-36).
+SolverSpec parse_solver(const Value* v, bool quasi) {
+    SolverSpec spec;
+    if (quasi) {
+        spec.si.max_iterations = 1000;
+        spec.si.damping = 0.04;
+        if (!v) {
+            spec.linearized = true;
+            return spec;
+        }
+    }
+    if (!v) return spec;
     const Value* p;
     std::string t = enum_tag(v, &p, "Solver");
+    if (t == "Linearized") {
+        spec.linearized = true;
+        spec.lin = parse_linearized(p);
+        return spec;
+    }
     if (t != "SimpleIterative") throw std::runtime_error("Solver `" + t + "` is not supported by this path");
-    check_keys(p, {"max_iterations_per_time_step", "damping_factor", "residual_tolerance_absolute",
-                   "strength_difference_tolerance", "velocity_corrections",
-                   "use_raw_circulation_during_iterations", "start_with_linearized_solution",
-                   "correction_factor_initial_solution"},
-               "SimpleIterative");
+    SimpleIterative& s = spec.si;
+    if (quasi)
+        check_keys(p, {"max_iterations_per_time_step", "damping_factor", "residual_tolerance_absolute",
+                       "strength_difference_tolerance", "velocity_corrections", "start_with_linearized_solution",
+                       "correction_factor_initial_solution"},
+                   "QuasiSteadySimpleIterativeBuilder");
+    else
+        check_keys(p, {"max_iterations_per_time_step", "damping_factor", "residual_tolerance_absolute",
+                       "strength_difference_tolerance", "velocity_corrections",
+                       "use_raw_circulation_during_iterations", "start_with_linearized_solution",
+                       "correction_factor_initial_solution"},
+                   "SimpleIterative");
     if (auto* x = p->get("max_iterations_per_time_step")) s.max_iterations = (size_t)num(x, "max_it");
     if (auto* x = p->get("damping_factor")) s.damping = num(x, "damping");
     if (auto* x = p->get("residual_tolerance_absolute")) s.residual_tol = num(x, "rtol");
     if (auto* x = p->get("strength_difference_tolerance")) s.strength_tol = num(x, "stol");
     if (auto* x = p->get("use_raw_circulation_during_iterations")) s.use_raw = x->kind == Value::Bool && x->b;
     if (auto* x = p->get("start_with_linearized_solution")) s.start_linearized = x->kind == Value::Bool && x->b;
-    if (s.start_linearized) throw std::runtime_error("start_with_linearized_solution is not supported by this path");
     if (auto* x = p->get("correction_factor_initial_solution")) s.correction_factor_initial = num(x, "cf");
-    if (auto* x = p->get("velocity_corrections")) {
+    if (auto* x = p->get("velocity_corrections")) s.vc = parse_vc(x, s.vc_ratio);
+    return spec;
+}
+
+QuasiSteadyWake parse_quasi_wake(const Value* o) {  // wake/settings.rs:33-58
+    QuasiSteadyWake w;
+    if (!o) return w;
+    check_keys(o, {"wake_length_factor", "symmetry_condition", "viscous_core_length"}, "QuasiSteadyWakeSettings");
+    if (auto* x = o->get("wake_length_factor")) w.wake_length_factor = num(x, "wake_length_factor");
+    if (auto* x = o->get("symmetry_condition")) {
         const Value* q;
-        std::string vt = enum_tag(x, &q, "VelocityCorrections");
-        if (vt == "NoCorrection") s.vc = VelCorr::None;
-        else if (vt == "MaxInducedVelocityMagnitudeRatio") { s.vc = VelCorr::MaxRatio; s.vc_ratio = num(q, "ratio"); }
-        else if (vt == "FixedMagnitudeEqualToFreestream") s.vc = VelCorr::FixedMagnitude;
-        else throw std::runtime_error("bad VelocityCorrections");
+        std::string t = enum_tag(x, &q, "SymmetryCondition");
+        if (t == "NoSymmetry") w.sym = Symmetry::None;
+        else if (t == "X") w.sym = Symmetry::X;
+        else if (t == "Y") w.sym = Symmetry::Y;
+        else if (t == "Z") w.sym = Symmetry::Z;
+        else throw std::runtime_error("bad SymmetryCondition");
     }
-    return s;
+    if (auto* x = o->get("viscous_core_length")) {
+        const Value* q;
+        std::string t = enum_tag(x, &q, "ViscousCoreLength");
+        if (t == "Relative") { w.core_kind = 0; w.core_value = num(q, "Relative"); }
+        else if (t == "Absolute") { w.core_kind = 1; w.core_value = num(q, "Absolute"); }
+        else if (t == "NoViscousCore") { w.core_kind = 2; w.core_value = 0.0; }
+        else throw std::runtime_error("bad ViscousCoreLength");
+    }
+    return w;
 }
 
 }  // namespace
@@ -1175,21 +1413,25 @@ struct orc_sim {
     FlowDerivatives fd;
     DynamicWake wake;
     FrozenWake frozen;
-    SimpleIterative solver;
+    SolverSpec solver;
+    bool quasi = false;      // WakeData::QuasiSteady
+    QuasiSteadyWake qs;
     std::vector<double> prev_gamma;
     bool first_done = false;
 
     std::vector<V3> freestream_points() const {  // simulation.rs:45-63
         std::vector<V3> p = lfm.ctrl_points_global;
-        p.insert(p.end(), wake.points.begin(), wake.points.end());
+        if (!quasi) p.insert(p.end(), wake.points.begin(), wake.points.end());
         return p;
     }
     void initialize(const V3* ctrl_fs, double dt) {  // simulation.rs:65-94
-        V3 sum;
-        for (size_t i = 0; i < lfm.n(); ++i) sum = sum + ctrl_fs[i];
-        V3 avg = sum / (double)lfm.n();
-        for (auto& v : wake.velocity_at_points) v = avg;
-        wake.initialize_with_velocity(lfm, avg, dt);
+        if (!quasi) {
+            V3 sum;
+            for (size_t i = 0; i < lfm.n(); ++i) sum = sum + ctrl_fs[i];
+            V3 avg = sum / (double)lfm.n();
+            for (auto& v : wake.velocity_at_points) v = avg;
+            wake.initialize_with_velocity(lfm, avg, dt);
+        }
         fd.reset(std::vector<V3>(ctrl_fs, ctrl_fs + lfm.n()));
         first_done = true;
         prev_gamma.assign(lfm.n(), 0.0);
@@ -1201,10 +1443,15 @@ struct orc_sim {
         std::vector<V3> felt = lfm.felt_ctrl_velocity(fs);
         std::vector<V3> felt_span = lfm.span_point_values(felt);
         if (!first_done) initialize(fs, dt);
-        wake.update_before_solving(dt, lfm, felt_span);
-        frozen.update_from_full_wake(lfm.ctrl_points_global, wake);
-        SolverResult sr = solve(solver, lfm, felt, frozen, prev_gamma);
-        wake.update_after_solving(sr.gamma, fs + n);
+        if (quasi) {
+            update_as_steady(frozen, lfm, felt_span, qs);
+        } else {
+            wake.update_before_solving(dt, lfm, felt_span);
+            frozen.update_from_full_wake(lfm.ctrl_points_global, wake);
+        }
+        SolverResult sr = solver.linearized ? solve_linearized(solver.lin, lfm, felt, frozen)
+                                            : solve(solver.si, lfm, felt, frozen, prev_gamma);
+        if (!quasi) wake.update_after_solving(sr.gamma, fs + n);
         prev_gamma = sr.gamma;
         std::vector<V3> acc = fd.acceleration(sr.output_velocity, dt);
         fd.update(sr.output_velocity);
@@ -1312,6 +1559,27 @@ extern "C" {
 
 const char* orc_last_error(void) { return g_err.c_str(); }
 
+int orc_solve_gaussian_elimination(const double* A, const double* b, size_t n, double* x) {
+    try {
+        std::vector<double> a(A, A + n * n), r(b, b + n);
+        std::vector<double> s = solve_gaussian_elimination(a, r, n);
+        for (size_t i = 0; i < n; ++i) x[i] = s[i];
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+void orc_horseshoe_unit_velocity(const double* start, const double* end, const double* wake_start,
+                                 const double* wake_end, const double* x, double core, double* out) {
+    V3 a(start[0], start[1], start[2]), b(end[0], end[1], end[2]);
+    V3 ws(wake_start[0], wake_start[1], wake_start[2]), we(wake_end[0], wake_end[1], wake_end[2]);
+    Horseshoe h{a, b, a + ws, a, b, b + we, core};
+    V3 u = h.unit(V3(x[0], x[1], x[2]));
+    for (int c = 0; c < 3; ++c) out[c] = u[c];
+}
+
 orc_sim* orc_sim_create(const char* json) {
     if (!json) { g_err = "null setup string"; return nullptr; }
     try {
@@ -1321,14 +1589,21 @@ orc_sim* orc_sim_create(const char* json) {
         const Value* lfmv = root->get("line_force_model");
         if (!lfmv) throw std::runtime_error("missing field `line_force_model`");
         sim->lfm = build_lfm(lfmv);
-        const Value* ss = root->get("simulation_settings");
-        if (!ss) throw std::runtime_error("QuasiSteady simulations are not supported by this path");
-        const Value* p;
-        std::string t = enum_tag(ss, &p, "SimulationSettings");
-        if (t != "Dynamic") throw std::runtime_error("SimulationSettings `" + t + "` is not supported by this path");
-        if (p) check_keys(p, {"solver", "wake"}, "DynamicSettings");
-        sim->solver = parse_solver(p ? p->get("solver") : nullptr);
-        build_wake(sim->wake, p ? p->get("wake") : nullptr, sim->lfm);
+        const Value* ss = root->get("simulation_settings");  // simulation_builder.rs:21-49 (default QuasiSteady)
+        const Value* p = nullptr;
+        std::string t = ss ? enum_tag(ss, &p, "SimulationSettings") : std::string("QuasiSteady");
+        if (t == "QuasiSteady") {
+            if (p) check_keys(p, {"solver", "wake"}, "QuasiSteadySettings");
+            sim->quasi = true;
+            sim->solver = parse_solver(p ? p->get("solver") : nullptr, true);
+            sim->qs = parse_quasi_wake(p ? p->get("wake") : nullptr);
+        } else if (t == "Dynamic") {
+            if (p) check_keys(p, {"solver", "wake"}, "DynamicSettings");
+            sim->solver = parse_solver(p ? p->get("solver") : nullptr, false);
+            build_wake(sim->wake, p ? p->get("wake") : nullptr, sim->lfm);
+        } else {
+            throw std::runtime_error("SimulationSettings `" + t + "` is not supported by this path");
+        }
         sim->frozen.init(sim->lfm.n());
         sim->prev_gamma.assign(sim->lfm.n(), 0.0);
         sim->fd.reset(std::vector<V3>(sim->lfm.n(), V3()));

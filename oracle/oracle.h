@@ -38,6 +38,13 @@ typedef struct orc_step_result {
 } orc_step_result;
 
 const char* orc_last_error(void);
+/* Gaussian elimination with partial pivoting (stormath matrix/linalg.rs:41-104);
+ * A row-major n x n. Returns 0, or -1 for a (nearly) singular pivot. */
+int orc_solve_gaussian_elimination(const double* A, const double* b, size_t n, double* x);
+/* Unit-strength velocity of one horseshoe vortex (horseshoe_vortex.rs:23-103):
+ * bound segment start->end, trailing legs start+wake_start->start and end->end+wake_end. */
+void orc_horseshoe_unit_velocity(const double* start, const double* end, const double* wake_start,
+                                 const double* wake_end, const double* x, double core, double* out);
 
 orc_sim* orc_sim_create(const char* setup_json);
 void orc_sim_destroy(orc_sim* sim);

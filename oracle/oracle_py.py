@@ -71,6 +71,8 @@ def lib(variant: str = "glibc"):
         L.orc_sigmoid.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_libm_variant.restype = ctypes.c_int
         L.orc_libm_eval.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
+        L.orc_solve_gaussian_elimination.argtypes = [_dp, _dp, ctypes.c_size_t, _dp]
+        L.orc_horseshoe_unit_velocity.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -212,4 +214,23 @@ def libm_eval(fn: int, x, y=None, variant: str = "glibc") -> np.ndarray:
     y = _f64(y) if y is not None else x
     out = np.zeros_like(x)
     lib(variant).orc_libm_eval(fn, _p(x), _p(y), _p(out), x.size)
+    return out
+
+
+def solve_gaussian_elimination(A, b):
+    """stormath Matrix::solve_gaussian_elimination (linalg.rs:41-104); ValueError when singular."""
+    A = _f64(A)
+    b = _f64(b)
+    n = b.size
+    x = np.zeros(n)
+    if lib().orc_solve_gaussian_elimination(_p(A), _p(b), n, _p(x)) != 0:
+        raise ValueError(lib().orc_last_error().decode())
+    return x
+
+
+def horseshoe_unit_velocity(start, end, wake_start, wake_end, x, core=0.0):
+    """HorseshoeVortex::induced_velocity_with_unit_strength (horseshoe_vortex.rs:23-103)."""
+    out = np.zeros(3)
+    lib().orc_horseshoe_unit_velocity(_p(_f64(start)), _p(_f64(end)), _p(_f64(wake_start)), _p(_f64(wake_end)),
+                                      _p(_f64(x)), float(core), _p(out))
     return out

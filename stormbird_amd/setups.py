@@ -61,8 +61,39 @@ def dynamic_settings(nr_panels=100, core=None, ratio=0.0, first_rel=None, last_r
         wake.update(extra_wake)
     d = {"wake": wake}
     if solver is not None:
-        d["solver"] = {"SimpleIterative": dict(solver)}
+        d["solver"] = {"SimpleIterative": dict(solver)} if "Linearized" not in solver else dict(solver)
     return {"Dynamic": d}
+
+
+def quasi_steady_settings(solver=None, wake_length_factor=None, core=None, symmetry=None):
+    """QuasiSteadySettings (simulation_builder.rs:21-29, wake/settings.rs:33-58).
+
+    ``solver`` is None (the default Linearized solver), {"Linearized": {...}} or the fields of a
+    QuasiSteadySimpleIterativeBuilder (simple_iterative.rs:14-45)."""
+    d = {}
+    if solver is not None:
+        d["solver"] = dict(solver) if "Linearized" in solver else {"SimpleIterative": dict(solver)}
+    wake = {}
+    if wake_length_factor is not None:
+        wake["wake_length_factor"] = float(wake_length_factor)
+    if core is not None:
+        wake["viscous_core_length"] = core
+    if symmetry is not None:
+        wake["symmetry_condition"] = symmetry
+    if wake:
+        d["wake"] = wake
+    return {"QuasiSteady": d}
+
+
+def with_settings(setup_json: str, settings) -> str:
+    """The same line force model with other simulation settings (None drops the key, which the
+    reference reads as the default QuasiSteady settings, simulation_builder.rs:51-64)."""
+    d = json.loads(setup_json)
+    if settings is None:
+        d.pop("simulation_settings", None)
+    else:
+        d["simulation_settings"] = settings
+    return json.dumps(d)
 
 
 def rectangular_wing(aspect_ratio=5.0, cl_zero_angle=0.5, angle_of_attack=0.0, nr_strips=20,

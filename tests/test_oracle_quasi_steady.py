@@ -1,0 +1,225 @@
+"""Pins the oracle's quasi-steady path: Gaussian elimination, horseshoe vortices,
+the steady frozen wake and the Linearized solver (CPU only).
+
+* stormath's own solver test (linalg.rs:112-136) and numpy.linalg.solve on random
+  systems, including ones that need row exchanges; the singular-pivot error;
+* a horseshoe vortex against the textbook finite-segment law
+  V = G/(4 pi h) (cos t1 - cos t2) (angle form, not the vector form the oracle uses);
+* the symmetry image: a half wing with a Z symmetry plane equals the mirrored full wing;
+* the Linearized solution satisfies the lifting-line equations written here from the
+  frozen-wake matrix and the wing's known geometry;
+* the reference's steady_lift test (compare_models.rs:28-168, quasi-steady part):
+  CL and CD within 7 % and 6 % of elliptic-wing theory;
+* the quasi-steady SimpleIterative solver converges to the Linearized solution, and
+  start_with_linearized_solution (simple_iterative.rs:103-113) starts it there.
+"""
+import json
+import math
+
+import numpy as np
+import pytest
+
+import stormbird_amd.setups as S
+
+
+@pytest.fixture(scope="module")
+def orc():
+    import oracle_py
+
+    oracle_py.build()
+    return oracle_py
+
+
+def test_reference_matrix_solver(orc):
+    """linalg.rs:112-136: A x = b with x = [0.6, -0.4, 2.0] (1e-4)."""
+    A = np.array([[3.0, 2.0, 0.0], [2.0, 3.0, 1.0], [-1.0, 1.0, 2.0]])
+    x = orc.solve_gaussian_elimination(A, [1.0, 2.0, 3.0])
+    assert np.max(np.abs(x - [0.6, -0.4, 2.0])) < 1e-4
+    assert np.max(np.abs(x - np.linalg.solve(A, [1.0, 2.0, 3.0]))) < 1e-14
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 40, 129])
+def test_solver_matches_numpy(orc, n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + n * 0.1 * np.eye(n)
+    b = rng.standard_normal(n)
+    x = orc.solve_gaussian_elimination(A, b)
+    ref = np.linalg.solve(A, b)
+    assert np.max(np.abs(x - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_solver_exchanges_rows(orc):
+    """A zero leading pivot needs the partial-pivoting row exchange."""
+    A = np.array([[0.0, 2.0, 1.0], [1.0, 0.0, 0.0], [0.0, 1.0, 3.0]])
+    x = orc.solve_gaussian_elimination(A, [5.0, 1.0, 10.0])
+    assert np.max(np.abs(x - [1.0, 1.0, 3.0])) < 1e-15
+    assert np.max(np.abs(A @ x - [5.0, 1.0, 10.0])) < 1e-14
+
+
+def test_solver_rejects_singular(orc):
+    with pytest.raises(ValueError, match="singular"):
+        orc.solve_gaussian_elimination([[1.0, 2.0], [2.0, 4.0]], [1.0, 1.0])
+    with pytest.raises(ValueError, match="singular"):
+        orc.solve_gaussian_elimination([[1e-13]], [1.0])
+
+
+def segment_angle_form(a, b, x):
+    """Finite straight vortex segment of unit strength from a to b: magnitude
+    (cos t1 - cos t2) / (4 pi h), direction (b - a) x (x - a) normalised."""
+    a, b, x = map(np.asarray, (a, b, x))
+    l = b - a
+    e = l / np.linalg.norm(l)
+    r1 = x - a
+    foot = a + e * np.dot(r1, e)
+    hvec = x - foot
+    h = np.linalg.norm(hvec)
+    cos1 = np.dot(e, r1) / np.linalg.norm(r1)
+    cos2 = np.dot(e, x - b) / np.linalg.norm(x - b)
+    d = np.cross(e, hvec / h)
+    return d * (cos1 - cos2) / (4 * math.pi * h)
+
+
+def test_horseshoe_against_segment_law(orc):
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        s, e = rng.uniform(-2, 2, 3), rng.uniform(-2, 2, 3)
+        ws, we = rng.uniform(5, 50, 3), rng.uniform(5, 50, 3)
+        x = rng.uniform(-3, 3, 3)
+        got = orc.horseshoe_unit_velocity(s, e, ws, we, x, 0.0)
+        ref = (segment_angle_form(s, e, x) + segment_angle_form(s + ws, s, x)
+               + segment_angle_form(e, e + we, x))
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_horseshoe_downwash_closed_form(orc):
+    """Long trailing legs: at the bound vortex's midpoint the legs induce G/(pi b) (two
+    semi-infinite lines at b/2), the bound vortex nothing."""
+    b = 2.0
+    got = orc.horseshoe_unit_velocity([0, 0, -1], [0, 0, 1], [1e9, 0, 0], [1e9, 0, 0], [0, 0, 0], 0.0)
+    assert abs(abs(got[1]) - 1.0 / (math.pi * b)) < 1e-9
+    assert abs(got[0]) < 1e-15 and abs(got[2]) < 1e-15
+
+
+def lfm_wing(z0, z1, nr_strips, aoa, cl_zero=0.3):
+    chord = S.rotate_around_axis([1.0, 0.0, 0.0], -aoa, [0.0, 0.0, 1.0])
+    return {
+        "section_points": [S._v(0, 0, z0), S._v(0, 0, z1)],
+        "chord_vectors": [S._v(*chord), S._v(*chord)],
+        "section_model": S.foil(cl_zero_angle=cl_zero, mean_positive_stall_angle=45 * S.DEG,
+                                mean_negative_stall_angle=45 * S.DEG),
+        "non_zero_circulation_at_ends": [False, False],
+    }
+
+
+def one_wing(z0, z1, nr, settings, aoa=3 * S.DEG):
+    return json.dumps({"line_force_model": {"wing_builders": [lfm_wing(z0, z1, nr, aoa)], "nr_sections": nr},
+                       "simulation_settings": settings})
+
+
+def step(sim, u=(8.0, 0.0, 0.0)):
+    fs = S.uniform_freestream(sim.nr_freestream_points(), list(u))
+    return sim.do_step(time=0.0, time_step=0.1, freestream_velocity=fs)
+
+
+def test_quasi_steady_has_no_wake_points(orc):
+    sim = orc.Simulation(S.with_settings(one_wing(0, 4, 16, S.quasi_steady_settings()), None))
+    assert sim.nr_freestream_points() == sim.nr_ctrl_points == 16
+    assert sim.nr_panels == 0
+
+
+def test_symmetry_plane_equals_mirrored_full_wing(orc):
+    half = orc.Simulation(one_wing(0, 4, 16, S.quasi_steady_settings(symmetry="Z")))
+    full = orc.Simulation(one_wing(-4, 4, 32, S.quasi_steady_settings()))
+    gh = step(half).force_input.circulation_strength
+    gf = step(full).force_input.circulation_strength
+    assert np.max(np.abs(gf[16:] - gh)) <= 1e-12 * np.max(np.abs(gh))
+    assert np.max(np.abs(gf[:16][::-1] - gh)) <= 1e-12 * np.max(np.abs(gh))
+
+
+def test_linearized_solution_solves_the_lifting_line_equations(orc):
+    """With viscous corrections off, the Linearized circulation satisfies the linear lifting-line
+    equation of every strip: the lift of the circulation equals the linear section lift at the
+    geometric angle corrected by the induced normal velocity."""
+    aoa = 4 * S.DEG
+    sim = orc.Simulation(one_wing(0, 6, 24, S.quasi_steady_settings(
+        solver={"Linearized": {"disable_viscous_corrections": True}}), aoa=aoa))
+    U = 8.0
+    r = step(sim, (U, 0, 0))
+    g = r.force_input.circulation_strength
+    fixed, F = sim.get_frozen_wake()
+    assert np.all(fixed == 0.0)
+    ind = np.einsum("ijc,j->ic", F, g)
+    # inflow along x, span along +z: normal = u_dir x span_dir = x x z = -y; chord length 1
+    induced_normal = ind @ np.array([0.0, -1.0, 0.0])
+    cl = 0.3 + 2 * math.pi * math.sin(aoa)
+    # linearized.rs:107-131: 2 g / (c U) - slope (F g . n) / U = -CL_lin(alpha_geo)
+    lhs = 2 * g / U - 2 * math.pi * induced_normal / U
+    assert np.max(np.abs(lhs + cl)) < 1e-12 * np.max(np.abs(cl))
+
+
+def test_reference_steady_lift_quasi_steady(orc):
+    """compare_models.rs:28-168 (QuasiSteadySettings::default(), one step): CL within 7 %, CD 6 %."""
+    aoa = 2.0 * math.pi / 180
+    js = S.with_settings(S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=1.2, angle_of_attack=aoa,
+                                            nr_strips=64), S.quasi_steady_settings())
+    sim = orc.Simulation(js)
+    fs = S.uniform_freestream(sim.nr_freestream_points(), [1.2, 0, 0])
+    r = sim.do_step(time=0.0, time_step=0.25 / 1.2, freestream_velocity=fs)
+    assert r.iterations == 1
+    ff = 0.5 * 13.2 * 1.2 ** 2 * 5.0
+    cl, cd = r.integrated_forces_sum()[1] / ff, r.integrated_forces_sum()[0] / ff
+    cl_t = (1.2 + 2 * math.pi * aoa) / (1 + 2 / 5.0)
+    cd_t = cl_t ** 2 / (math.pi * 5)
+    assert abs(cl - cl_t) / cl_t < 0.07
+    assert abs(cd - cd_t) / cd_t < 0.06
+
+
+def test_simple_iterative_converges_to_linearized(orc):
+    wing = one_wing(0, 5, 20, None, aoa=2 * S.DEG)
+    lin = orc.Simulation(S.with_settings(wing, S.quasi_steady_settings()))
+    it = orc.Simulation(S.with_settings(wing, S.quasi_steady_settings(
+        solver={"max_iterations_per_time_step": 4000, "residual_tolerance_absolute": 1e-12,
+                "strength_difference_tolerance": 1e-12})))
+    gl = step(lin).force_input.circulation_strength
+    ri = step(it)
+    assert 1 < ri.iterations < 4000
+    # the fixed point of the nonlinear iteration and the linearized solution differ by the
+    # curvature of sin(alpha) and the stall blend over the induced angle: well under 1 %
+    assert np.max(np.abs(ri.force_input.circulation_strength - gl)) < 5e-3 * np.max(np.abs(gl))
+
+
+def test_start_with_linearized_solution_cuts_iterations(orc):
+    wing = one_wing(0, 5, 20, None, aoa=2 * S.DEG)
+    base = {"max_iterations_per_time_step": 4000, "residual_tolerance_absolute": 1e-10,
+            "strength_difference_tolerance": 1e-10}
+    cold = orc.Simulation(S.with_settings(wing, S.quasi_steady_settings(solver=base)))
+    warm = orc.Simulation(S.with_settings(wing, S.quasi_steady_settings(
+        solver=dict(base, start_with_linearized_solution=True))))
+    rc, rw = step(cold), step(warm)
+    assert rw.iterations < rc.iterations
+    gc, gw = rc.force_input.circulation_strength, rw.force_input.circulation_strength
+    assert np.max(np.abs(gc - gw)) < 1e-6 * np.max(np.abs(gc))
+
+
+def test_default_settings_are_quasi_steady_linearized(orc):
+    """simulation_builder.rs:51-64 and solvers/mod.rs:37-41: no settings = QuasiSteady + Linearized."""
+    wing = one_wing(0, 5, 20, None)
+    a = orc.Simulation(S.with_settings(wing, None))
+    b = orc.Simulation(S.with_settings(wing, S.quasi_steady_settings(solver={"Linearized": {}})))
+    ra, rb = step(a), step(b)
+    assert ra.iterations == 1
+    assert np.array_equal(ra.force_input.circulation_strength, rb.force_input.circulation_strength)
+
+
+def test_dynamic_linearized_solver(orc):
+    """The Linearized solver also runs under Dynamic settings (solvers/mod.rs:21-24): every step
+    is one direct solve, and the wake still convects."""
+    js = S.rectangular_wing(aspect_ratio=4.0, cl_zero_angle=0.2, angle_of_attack=3 * S.DEG, nr_strips=16,
+                            nr_panels=30, solver={"Linearized": {}})
+    sim = orc.Simulation(js)
+    fs = S.uniform_freestream(sim.nr_freestream_points(), [6.0, 0, 0])
+    for k in range(5):
+        r = sim.do_step(time=k * 0.1, time_step=0.1, freestream_velocity=fs)
+        assert r.iterations == 1
+    assert np.all(np.isfinite(r.force_input.circulation_strength))
+    assert np.any(sim.get_wake_state()["strengths"] != 0.0)
