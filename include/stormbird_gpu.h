@@ -2,7 +2,7 @@
  * stormbird_gpu.h — C ABI of the MI355X (gfx950) dynamic lifting-line library.
  *
  * Drop-in for Stormbird's `Simulation` (stormbird 0.9.0, Rust) on its dynamic
- * wake path: the same JSON setup string, the same step semantics and the same
+ * and quasi-steady lifting-line paths: the same JSON setup string, the same step semantics and the same
  * result fields, with the wake state resident in GPU memory.  The convention
  * follows the reference's own C interface (interfaces/cstormbird): opaque
  * handle built from a NUL-terminated JSON string (NULL on error), explicit
@@ -67,7 +67,14 @@ enum { SB_MODE_EXACT = 0, SB_MODE_FAST = 1, SB_MODE_FP32_KAHAN = 2 };
 const char* sb_last_error(void);
 
 /* Replaces Simulation::new_from_string (stormbird/src/lifting_line/simulation.rs:33-39).
- * Returns NULL on a parse error or an unsupported setup (see sb_last_error). */
+ * Returns NULL on a parse error or an unsupported setup (see sb_last_error).
+ * Both SimulationSettings variants are taken: Dynamic (shed free wake) and
+ * QuasiSteady (one horseshoe vortex per strip, no wake points: the freestream
+ * is given at the control points only), which is also the default when
+ * simulation_settings is absent (simulation_builder.rs:51-64).  Solvers:
+ * SimpleIterative (with start_with_linearized_solution) and Linearized
+ * (linearized.rs:54-222; a singular equation system fails the step with -4
+ * where the reference's unwrap panics). */
 sb_sim* sb_sim_create(const char* setup_json);
 /* Same, on GPU `device` with a Biot-Savart arithmetic mode (SB_MODE_*). */
 sb_sim* sb_sim_create_ex(const char* setup_json, int device, int mode);
@@ -96,7 +103,8 @@ int sb_sim_do_step_device(sb_sim* sim, double time, double time_step,
                           const double* freestream_xyz_device, size_t n_points,
                           sb_step_result* out);
 
-/* Replaces Simulation::induced_velocities (simulation.rs:207-216). */
+/* Replaces Simulation::induced_velocities (simulation.rs:207-216); -4 for a
+ * quasi-steady simulation, where the reference panics. */
 int sb_sim_induced_velocities(sb_sim* sim, const double* points_xyz, size_t n, double* out_xyz);
 
 /* The warm-start hook: Simulation::previous_circulation_strength (simulation.rs:26,165,189). */
@@ -173,6 +181,10 @@ int sb_wake_induced_velocities(const double* panel_records, size_t n_panels,
 int sb_debug_libm_device(int fn, const double* x, const double* y, double* out, size_t n);
 /* Same functions evaluated by the library's host code. */
 int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n);
+/* 1 when the Linearized solve of n_strips keeps its augmented matrix in the
+ * workgroup's shared memory on the current device, 0 when in global memory,
+ * -1 for n_strips outside [1, 1024]. */
+int sb_debug_linearized_in_shared_memory(long n_strips);
 
 /* ---- Batched simulations ------------------------------------------------
  * Many independent simulations (each built from its own setup string, with

@@ -92,6 +92,10 @@ struct sb_batch {
             auto s = std::make_unique<sb_sim>();
             s->create(jsons[i], dev, md);
             if (s->N > 1024) throw std::invalid_argument("batched instances support at most 1024 strips");
+            if (s->setup.quasi || s->setup.solver.kind != 0 || s->setup.solver.start_linearized)
+                throw std::invalid_argument(
+                    "batched instances run dynamic SimpleIterative simulations (no quasi-steady wake, no Linearized "
+                    "solver, no start_with_linearized_solution)");
             sims.push_back(std::move(s));
         }
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");

@@ -104,6 +104,9 @@ SB_HD double foil_amount_of_stall(const FoilParams& f, double a) {  // foil.rs:2
     double m = e >= 0.0 ? sbm::fabs_(f.mean_positive_stall_angle) : sbm::fabs_(f.mean_negative_stall_angle);
     return sigmoid_zero_to_one(sbm::fabs_(e), m, f.stall_range);
 }
+SB_HD double foil_cl_linear(const FoilParams& f, double a) {  // foil.rs:161-163
+    return f.cl_zero_angle + f.cl_initial_slope * sbm::sin(a);
+}
 SB_HD double foil_cl_pre_raw(const FoilParams& f, double a) {  // foil.rs:161-181
     double hp = f.cl_high_order_power > 0.0 ? sbm::pow(sbm::fabs_(a), f.cl_high_order_power) * signum(a) : 0.0;
     double k = a >= 0.0 ? f.cl_high_order_factor_positive : f.cl_high_order_factor_negative;

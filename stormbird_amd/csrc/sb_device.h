@@ -25,6 +25,7 @@ struct GeoLayout {
     int chord_out;     // [N][3] chord vectors in the output coordinate system
     int is_virtual;    // [N] 1.0 for virtual strips
     int span_pts;      // [P][3] span points (global)
+    int end_g;         // [N][3] span line end points (global)
     int chord_span;    // [P][3] chord vectors at span points
     int rbm;           // [18]: translation, rotation, v_lin, v_ang, (unused 6)
     int body_axes;     // [9]: rotated unit x, y, z axes (body-fixed output)
@@ -48,6 +49,16 @@ struct Inst {
     double solver_damping, residual_tol, strength_tol, vc_ratio, correction_factor;
     double density;
     double dt;           // time step (batched launches read it here)
+    // quasi-steady wake and Linearized solver
+    int quasi;           // horseshoe frozen wake, no shed wake (M == 0)
+    int solver_kind;     // 0 SimpleIterative, 1 Linearized
+    int start_linearized;     // SimpleIterative starts from a default Linearized solve
+    int lin_disable_viscous;  // Linearized: skip the viscous circulation correction
+    int lin_ivc;              // Linearized: 1 recompute the induced velocity after it
+    double hs_wake_length;    // horseshoe trailing-leg length
+    double hs_core;           // horseshoe viscous core length
+    double* lin;         // Linearized augmented matrix [N + 1][N] when it does not fit in LDS
+    double* lin_gamma;   // [N] Linearized circulation that starts SimpleIterative
 
     double* pts;         // current wake points [P*(M+1)][3]
     double* pts_next;    // next wake points (double buffer)
@@ -70,7 +81,7 @@ struct Inst {
     double* F;           // unit-strength factors [N(cols j)][N(rows i)][3]
     double* prev_gamma;  // [N] previous circulation (initial guess)
     double* hist;        // flow-derivative history [2][N][3]
-    double* scal;        // scalars: [0..2] mean ctrl freestream, [3] fd count
+    double* scal;        // scalars: [0..2] mean ctrl freestream, [3] fd count, [4] singular-matrix row + 1
     double* out;         // packed step output (OutLayout)
 };
 
@@ -120,6 +131,13 @@ hipError_t launch_panels(const Inst& d, const double* pts, int shift_strengths, 
 // holds room for at least one 64-panel chunk, else one wave per target.
 hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st);
 hipError_t launch_factors(const Inst& d, hipStream_t st);
+// Quasi-steady factors: F[j][i] = unit velocity of strip j's horseshoe vortex at ctrl point i.
+hipError_t launch_horseshoe_factors(const Inst& d, hipStream_t st);
+// Linearized solve; as_start = 1 writes only the circulation to d.lin_gamma, with
+// the default Linearized settings (start of a SimpleIterative solve).  Needs d.lin
+// unless linearized_in_lds(d.N).
+hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st);
+bool linearized_in_lds(int N);
 hipError_t launch_solve(const Inst& d, hipStream_t st);
 hipError_t launch_set_row0(const Inst& d, hipStream_t st);
 hipError_t launch_velocity_copy(const Inst& d, hipStream_t st);

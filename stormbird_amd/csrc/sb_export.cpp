@@ -53,6 +53,7 @@ void panel_point_indices(const sb_sim& s, long k, long out[4]) {
 namespace {
 
 void download_wake(const sb_sim& s, std::vector<double>& pts, std::vector<double>& str) {
+    if (s.setup.quasi) throw std::runtime_error("a quasi-steady simulation has no dynamic wake to export");
     check(hipSetDevice(s.device), "hipSetDevice");
     pts.resize(3 * s.nr_points);
     str.resize(s.nr_panels);

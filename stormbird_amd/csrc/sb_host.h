@@ -78,6 +78,9 @@ struct sb_sim {
     double *d_core = nullptr, *d_geo = nullptr, *d_fs = nullptr, *d_felt = nullptr, *d_felt_span = nullptr;
     double *d_fixed = nullptr, *d_F = nullptr, *d_prev = nullptr, *d_hist = nullptr, *d_scal = nullptr;
     double *d_out = nullptr, *d_tmp = nullptr;
+    double *d_lin = nullptr, *d_lin_gamma = nullptr;  // Linearized solve scratch
+    double hs_core = 0.0;                             // quasi-steady horseshoe core, per step
+    double h_singular = 0.0;
     size_t tmp_cap = 0;
     PanelRec* d_rec = nullptr;
     int *d_col = nullptr, *d_wing = nullptr, *d_span_src = nullptr;
@@ -97,7 +100,7 @@ struct sb_sim {
         void* bufs[] = {d_pts[0], d_pts[1], d_vel, d_str[0], d_str[1], d_core, d_geo, d_fs, d_felt,
                         d_felt_span, d_fixed, d_F, d_prev, d_hist, d_scal, d_out, d_tmp, d_rec,
                         d_col, d_wing, d_span_src, d_foil, mc.partial, mc.flags, tmp_mc.partial, tmp_mc.flags,
-                        fb.terms, fb.acc};
+                        fb.terms, fb.acc, d_lin, d_lin_gamma};
         for (void* p : bufs)
             if (p) (void)hipFree(p);
         for (auto& e : ev)
@@ -114,7 +117,7 @@ struct sb_sim {
         d.P = P;
         d.M = M;
         d.nz_rows = nz_rows;
-        d.symmetry = w.symmetry;
+        d.symmetry = setup.quasi ? setup.qs.symmetry : w.symmetry;
         d.neglect_self = w.neglect_self ? 1 : 0;
         d.first_dir = w.first_dir;
         d.steps_completed = (int)steps_completed;
@@ -134,6 +137,15 @@ struct sb_sim {
         d.vc_ratio = s.velocity_correction_ratio;
         d.correction_factor = s.correction_factor_initial;
         d.density = setup.lfm.density;
+        d.quasi = setup.quasi ? 1 : 0;
+        d.solver_kind = s.kind;
+        d.start_linearized = s.start_linearized ? 1 : 0;
+        d.lin_disable_viscous = s.disable_viscous_corrections ? 1 : 0;
+        d.lin_ivc = s.induced_velocity_correction;
+        d.hs_wake_length = setup.qs.wake_length_factor * w.rep_chord;  // frozen_wake.rs:78-81
+        d.hs_core = hs_core;
+        d.lin = d_lin;
+        d.lin_gamma = d_lin_gamma;
         d.pts = d_pts[cur];
         d.pts_next = d_pts[cur ^ 1];
         d.vel = d_vel;
@@ -176,6 +188,7 @@ struct sb_sim {
         gl.chord_out = take(3 * N);
         gl.is_virtual = take(N);
         gl.span_pts = take(3 * P);
+        gl.end_g = take(3 * N);
         gl.chord_span = take(3 * P);
         gl.rbm = take(18);
         gl.body_axes = take(9);
@@ -218,6 +231,7 @@ struct sb_sim {
             put(g + gl.ctrl_out + 3 * i, O.ctrl());
             put(g + gl.chord_out + 3 * i, m.output_body ? m.chord_local[i] : m.chord_global[i]);
             g[gl.is_virtual + i] = m.is_virtual[i] ? 1.0 : 0.0;
+            put(g + gl.end_g + 3 * i, G.end);
         }
         for (int j = 0; j < P; ++j) {
             put(g + gl.span_pts + 3 * j, m.span_points[j]);
@@ -248,7 +262,7 @@ struct sb_sim {
         P = (int)m.span_points.size();
         M = setup.wake.M;
         if (N > 1024) throw std::runtime_error("more than 1024 strips per simulation is not supported");
-        nr_points = (long)P * (M + 1);
+        nr_points = setup.quasi ? 0 : (long)P * (M + 1);  // quasi-steady: no wake points
         nr_panels = (long)N * M;
         layout();
         d_pts[0] = dalloc<double>(3 * nr_points);
@@ -272,6 +286,10 @@ struct sb_sim {
         d_wing = dalloc<int>(N);
         d_span_src = dalloc<int>(2 * P);
         d_foil = dalloc<FoilParams>(W);
+        if (setup.solver.kind == 1 || setup.solver.start_linearized) {
+            if (!linearized_in_lds(N)) d_lin = dalloc<double>((size_t)N * (N + 1));
+            d_lin_gamma = dalloc<double>(N);
+        }
         // SB_PAIR_SCHEDULE=plain keeps one workgroup per 256 targets for the pair
         // sums (no load balancing); the default balances over the resident workgroups.
         const char* sched = getenv("SB_PAIR_SCHEDULE");
@@ -297,8 +315,10 @@ struct sb_sim {
         check(hipMemcpyAsync(d_wing, m.wing_of.data(), sizeof(int) * N, hipMemcpyHostToDevice, stream), "upload");
         check(hipMemcpyAsync(d_span_src, span_src.data(), sizeof(int) * 2 * P, hipMemcpyHostToDevice, stream), "upload");
         check(hipMemcpyAsync(d_foil, m.foils.data(), sizeof(FoilParams) * W, hipMemcpyHostToDevice, stream), "upload");
-        check(hipMemcpyAsync(d_core, setup.wake.core.data(), sizeof(double) * nr_panels, hipMemcpyHostToDevice, stream),
-              "upload");
+        if (nr_panels > 0)
+            check(hipMemcpyAsync(d_core, setup.wake.core.data(), sizeof(double) * nr_panels, hipMemcpyHostToDevice,
+                                 stream),
+                  "upload");
         h_out.assign(OutLayout::total(N, W), 0.0);
         reset_state();
     }
@@ -317,7 +337,12 @@ struct sb_sim {
         check(hipMemsetAsync(d_hist, 0, sizeof(double) * 6 * N, stream), "memset");
         check(hipMemsetAsync(d_scal, 0, sizeof(double) * 8, stream), "memset");
         check(hipMemsetAsync(d_str[1], 0, sizeof(double) * nr_panels, stream), "memset");
+        check(hipMemsetAsync(d_fixed, 0, sizeof(double) * 3 * N, stream), "memset");
         pack_geometry();
+        if (setup.quasi) {  // no shed wake
+            check(hipStreamSynchronize(stream), "reset");
+            return;
+        }
         // DynamicWakeBuilder::build -> initialize_based_on_chord_length (initialization.rs:46-71)
         V3 sum = v3(0, 0, 0);
         for (const V3& c : m.chord_global_at_span) sum = sum + c;
@@ -343,10 +368,15 @@ struct sb_sim {
         setup.lfm.update_global();
         pack_geometry();
         upload_freestream(fs, n, fs_on_device);
+        if (setup.quasi) update_horseshoe_core();
         Inst d = inst();
         bool first = !first_done;
         check(hipEventRecord(ev[0], stream), "event");
         check(launch_prepare(d, first ? 1 : 0, stream), "prepare");
+        if (setup.quasi) {
+            quasi_steady_step(d, time, dt, out);
+            return;
+        }
         if (first) {
             check(launch_init_wake(d, dt, 1, nullptr, stream), "init wake");
             first_done = true;
@@ -367,8 +397,8 @@ struct sb_sim {
         check(hipEventRecord(ev[2], stream), "event");
         check(launch_factors(d, stream), "factors");
         check(hipEventRecord(ev[3], stream), "event");
-        // SimpleIterative::solve, then update_wing_strength
-        check(launch_solve(d, stream), "solve");
+        // Solver::solve (SimpleIterative or Linearized), then update_wing_strength
+        launch_solver(d);
         nz_rows = nz_rows < 1 ? 1 : nz_rows;
         d = inst();
         // update_velocity_at_points (update_data.rs:125-141)
@@ -386,8 +416,56 @@ struct sb_sim {
         check(hipEventRecord(ev[5], stream), "event");
         check(launch_forces(d, dt, stream), "forces");
         check(hipEventRecord(ev[6], stream), "event");
+        finish_step(time, out);
+        if (setup.wake.write_files) write_wake_step_file(*this, time, dt);  // simulation.rs:181-186
+    }
+
+    // frozen_wake.rs:84-99: Relative core = factor * mean global span-line length
+    void update_horseshoe_core() {
+        const QuasiConfig& q = setup.qs;
+        if (q.core_kind == 0) {
+            double sum = 0.0;
+            for (int i = 0; i < N; ++i) sum += setup.lfm.global[i].length();
+            hs_core = q.core_value * (sum / (double)N);
+        } else {
+            hs_core = q.core_kind == 1 ? q.core_value : 0.0;
+        }
+    }
+
+    void launch_solver(const Inst& d) {
+        if (setup.solver.kind == 1) {
+            check(launch_linearized(d, 0, stream), "linearized solve");
+            return;
+        }
+        if (setup.solver.start_linearized) check(launch_linearized(d, 1, stream), "linearized start");
+        check(launch_solve(d, stream), "solve");
+    }
+
+    // simulation.rs:107-203 with WakeData::QuasiSteady: horseshoe factors from
+    // the felt span-point velocities, solve, forces (no wake to update).
+    void quasi_steady_step(const Inst& d, double time, double dt, sb_step_result* out) {
+        first_done = true;
+        check(hipEventRecord(ev[1], stream), "event");
+        check(hipEventRecord(ev[2], stream), "event");
+        check(launch_horseshoe_factors(d, stream), "horseshoe factors");
+        check(hipEventRecord(ev[3], stream), "event");
+        launch_solver(d);
+        check(hipEventRecord(ev[4], stream), "event");
+        check(hipEventRecord(ev[5], stream), "event");
+        check(launch_forces(d, dt, stream), "forces");
+        check(hipEventRecord(ev[6], stream), "event");
+        steps_completed += 1;
+        pairs[0] = 0;
+        pairs[1] = 0;
+        pairs[2] = (double)N * (double)N;
+        finish_step(time, out);
+    }
+
+    void finish_step(double time, sb_step_result* out) {
         check(hipMemcpyAsync(h_out.data(), d_out, sizeof(double) * h_out.size(), hipMemcpyDeviceToHost, stream),
               "download");
+        bool lin = setup.solver.kind == 1 || setup.solver.start_linearized;
+        if (lin) check(hipMemcpyAsync(&h_singular, d_scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream), "download");
         check(hipStreamSynchronize(stream), "step");
         static const int order[kTimers][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
         for (int k = 0; k < kTimers; ++k) {
@@ -395,8 +473,15 @@ struct sb_sim {
             (void)hipEventElapsedTime(&ms, ev[order[k][0]], ev[order[k][1]]);
             timer_ms[k] = ms;
         }
+        if (lin && h_singular != 0.0) {  // linalg.rs:77-81, unwrapped at linearized.rs:139
+            long row = (long)h_singular - 1;
+            h_singular = 0.0;
+            check(hipMemsetAsync(d_scal + 4, 0, sizeof(double), stream), "memset");
+            check(hipStreamSynchronize(stream), "step");
+            throw std::runtime_error("Linearized solver: matrix is singular or nearly singular at row " +
+                                     std::to_string(row));
+        }
         write_result(time, out);
-        if (setup.wake.write_files) write_wake_step_file(*this, time, dt);  // simulation.rs:181-186
     }
 
     void write_result(double time, sb_step_result* o) const {
@@ -451,6 +536,8 @@ struct sb_sim {
 
     void induced(const double* pts, size_t n, double* out) {
         check(hipSetDevice(device), "hipSetDevice");
+        if (setup.quasi)  // simulation.rs:207-216
+            throw std::runtime_error("Induced velocities not implemented for pure quasi-steady simulations yet");
         if (n == 0) return;
         double* buf = tmp(6 * n);
         check(hipMemcpyAsync(buf, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, stream), "upload points");

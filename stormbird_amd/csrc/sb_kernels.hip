@@ -836,12 +836,9 @@ __device__ inline Strip load_strip(const Inst& d, int ic) {
     return s;
 }
 
-// One evaluation of the fixed-point map for strip i from the circulations in G
-// (simple_iterative.rs:110-160): induced velocity (frozen_wake.rs:268-301),
-// velocity correction (velocity_corrections.rs:18-100), spanwise part removed,
-// angle of attack, circulation estimate.
-__device__ inline double estimate_circulation(const Inst& d, const Strip& s, const double* G, int i, V3& u,
-                                              double& a) {
+// frozen_wake.rs:268-301: fixed velocity plus the factor row times G (three
+// running sums in column order, added to the fixed velocity at the end).
+__device__ inline V3 induced_at(const Inst& d, V3 fixed, const double* G, int i) {
     const int N = d.N;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double* Fi = d.F + 3 * i;
@@ -852,23 +849,41 @@ __device__ inline double estimate_circulation(const Inst& d, const Strip& s, con
         sy += f[1] * gj;
         sz += f[2] * gj;
     }
-    V3 ind = s.fixed;
+    V3 ind = fixed;
     ind.x += sx;
     ind.y += sy;
     ind.z += sz;
-    if (d.vc_mode == 1) {
-        double um = len(ind), fm = len(s.felt);
-        double mx = d.vc_ratio * fm;
+    return ind;
+}
+
+// velocity_corrections.rs:18-100: felt velocity plus the (corrected) induced velocity.
+__device__ inline V3 corrected_velocity(int vc_mode, double vc_ratio, V3 felt, V3 ind) {
+    if (vc_mode == 1) {
+        double um = len(ind), fm = len(felt);
+        double mx = vc_ratio * fm;
         V3 cvel = um > mx ? normalize(ind) * mx : ind;
-        u = s.felt + cvel;
-    } else if (d.vc_mode == 2) {
-        V3 tot = s.felt + ind;
-        u = normalize(tot) * len(s.felt);
-    } else {
-        u = s.felt + ind;
+        return felt + cvel;
     }
+    if (vc_mode == 2) {
+        V3 tot = felt + ind;
+        return normalize(tot) * len(felt);
+    }
+    return felt + ind;
+}
+
+__device__ inline double angle_of_attack(const Strip& s, V3 u) {  // force_calculations.rs:79-107
+    return len(u) > MIN_POSITIVE ? signed_angle_between(s.chord_g, u, s.dir_g) : 0.0;
+}
+
+// One evaluation of the fixed-point map for strip i from the circulations in G
+// (simple_iterative.rs:110-160): induced velocity, velocity correction,
+// spanwise part removed, angle of attack, circulation estimate.
+__device__ inline double estimate_circulation(const Inst& d, const Strip& s, const double* G, int i, V3& u,
+                                              double& a) {
+    V3 ind = induced_at(d, s.fixed, G, i);
+    u = corrected_velocity(d.vc_mode, d.vc_ratio, s.felt, ind);
     u = u - project(u, s.rel_g);  // line_force_model/mod.rs:235-254
-    a = len(u) > MIN_POSITIVE ? signed_angle_between(s.chord_g, u, s.dir_g) : 0.0;
+    a = angle_of_attack(s, u);
     return circulation(s, d.viscous_lift_in_circulation, a, u);
 }
 
@@ -886,6 +901,7 @@ __device__ inline void solve_body(const Inst& d, double* sh) {
 
     double g = d.prev_gamma[ic];
     if (fabs(d.correction_factor - 1.0) > MIN_POSITIVE) g *= d.correction_factor;
+    if (d.start_linearized) g = d.lin_gamma[ic];  // simple_iterative.rs:103-113
     V3 u{0.0, 0.0, 0.0};
     double a = 0.0;
     if (act) R[i] = residual_term(s, rho, g, a, u);
@@ -933,12 +949,14 @@ __device__ inline void solve_body(const Inst& d, double* sh) {
         double gamma = circulation(s, d.viscous_lift_in_circulation, a, u);
         double* o = d.out;
         o[OutLayout::gamma(N) + i] = gamma;
-        o[OutLayout::alpha(N) + i] = len(u) > MIN_POSITIVE ? signed_angle_between(s.chord_g, u, s.dir_g) : 0.0;
+        o[OutLayout::alpha(N) + i] = angle_of_attack(s, u);
         store3(o + OutLayout::u(N) + 3 * i, u);
         store3(o + OutLayout::felt(N) + 3 * i, s.felt);
         // update_wing_strength (update_data.rs:146-150): first-row strengths
-        d.str[i] = gamma;
-        d.rec[i].strength = gamma;
+        if (d.M > 0) {
+            d.str[i] = gamma;
+            d.rec[i].strength = gamma;
+        }
         d.prev_gamma[i] = gamma;
     }
     if (i == 0) {
@@ -949,6 +967,240 @@ __device__ inline void solve_body(const Inst& d, double* sh) {
 __global__ __launch_bounds__(1024) void k_solve(Inst d) {
     extern __shared__ double sh[];
     solve_body(d, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Quasi-steady frozen wake and the Linearized solver
+// ---------------------------------------------------------------------------
+
+// horseshoe_vortex.rs:23-51: bound segment, then the trailing legs
+// (start + w0 -> start) and (end -> end + w1), summed from zero.
+__device__ inline V3 horseshoe_unit(V3 b0, V3 b1, V3 w0, V3 w1, V3 x, double rc) {
+    V3 u{0.0, 0.0, 0.0};
+    u = u + segment_unit(b0, b1, x, rc);
+    u = u + segment_unit(b0 + w0, b0, x, rc);
+    u = u + segment_unit(b1, b1 + w1, x, rc);
+    return u;
+}
+
+// frozen_wake.rs:68-129 with horseshoe_vortex.rs:55-103: F[j][i] = unit velocity
+// of strip j's horseshoe at ctrl point i, with the symmetry image.  The legs of
+// strip j run along the normalized felt velocity of span points j + wing and
+// j + wing + 1, hs_wake_length long.
+__device__ inline void horseshoe_factors_body(const Inst& d, long idx) {
+    long NN = (long)d.N * d.N;
+    if (idx >= NN) return;
+    int j = (int)(idx / d.N);            // horseshoe (column)
+    int i = (int)(idx - (long)j * d.N);  // ctrl point (row)
+    int c = d.col[j];
+    V3 b0 = load3(d.geo + d.gl.span_pts + 3 * c);
+    V3 b1 = load3(d.geo + d.gl.end_g + 3 * j);
+    V3 w0 = normalize(load3(d.felt_span + 3 * c)) * d.hs_wake_length;
+    V3 w1 = normalize(load3(d.felt_span + 3 * (c + 1))) * d.hs_wake_length;
+    V3 x = load3(d.geo + d.gl.ctrl + 3 * i);
+    int nq = d.symmetry == 0 ? 1 : 2;
+    V3 s{d.symmetry == 1 ? -1.0 : 1.0, d.symmetry == 2 ? -1.0 : 1.0, d.symmetry == 3 ? -1.0 : 1.0};
+    V3 out{0.0, 0.0, 0.0};
+#pragma nounroll
+    for (int q = 0; q < nq; ++q) {  // symmetry_condition.rs:27-43, as unit_velocity_sym
+        V3 p = q == 0 ? x : V3{x.x * s.x, x.y * s.y, x.z * s.z};
+        V3 v = horseshoe_unit(b0, b1, w0, w1, p, d.hs_core);
+        out = q == 0 ? v : V3{out.x + s.x * v.x, out.y + s.y * v.y, out.z + s.z * v.z};
+    }
+    store3(d.F + 3 * idx, out);
+}
+__global__ void k_horseshoe_factors(Inst d) {
+    horseshoe_factors_body(d, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Shared-memory doubles of the Linearized solve besides the matrix.
+SB_HD inline long linearized_aux(int N) { return 4L * N + 1 + 16 + 16 + 4; }
+
+// Linearized::solve (linearized.rs:54-222), one workgroup of T >= N threads.
+// The augmented matrix [A | b] is kept column-major (column j at aug + j*N),
+// so that a pivot column is contiguous and the trailing update is coalesced.
+// Gaussian elimination with partial pivoting follows linalg.rs:41-104 element
+// by element: every entry receives the same sequence of a - f*p updates (f and
+// p rounded as there), the pivot is the first row of largest magnitude (a NaN
+// diagonal is kept), and back substitution sums in ascending column order.
+// Sub-diagonal entries of eliminated columns are never read again, so they are
+// neither updated nor swapped.  A pivot below 1e-12 in magnitude (where the
+// reference's unwrap panics) records its row + 1 in scal[4]; the circulation is
+// then zero.  as_start = 1: default settings, circulation to lin_gamma only.
+template <bool LDS_MATRIX>
+__device__ inline void linearized_body(const Inst& d, double* sh, int as_start) {
+    const int N = d.N;
+    const int T = blockDim.x;
+    const int t = threadIdx.x;
+    double* G = sh;            // [N] solution, then circulation
+    double* fac = G + N;       // [N] factors of the current pivot column
+    double* prow = fac + N;    // [N + 1] current pivot row
+    double* R = prow + N + 1;  // [N] residual terms
+    double* rv = R + N;        // [16] per-wave largest pivot magnitude
+    double* rix = rv + 16;     // [16] and its row
+    double* ctl = rix + 16;    // [0] pivot row, [1] singular row + 1, [2] residual
+    double* aug = LDS_MATRIX ? ctl + 4 : d.lin;
+    const int vc = as_start ? 0 : d.vc_mode;
+    const double vc_ratio = d.vc_ratio;
+    const int viscous = as_start ? 1 : !d.lin_disable_viscous;
+    const int ivc = as_start ? 1 : d.lin_ivc;
+
+    bool act = t < N;
+    int ic = act ? t : N - 1;
+    Strip s = load_strip(d, ic);
+    {
+        V3 u = s.felt + s.fixed;
+        u = u - project(u, s.rel_g);  // remove_span_velocity
+        V3 vdir = normalize(u);
+        V3 ndir = cross(vdir, normalize(s.rel_g));
+        double a = angle_of_attack(s, u);
+        double cll = foil_cl_linear(s.foil, a);
+        double dcl = s.foil.cl_initial_slope;
+        double ut = dot(u, vdir);
+        if (act) {  // row t of the equation system (linearized.rs:107-131)
+            const double* Fi = d.F + 3L * t;
+            for (int c = 0; c < N; ++c) {
+                V3 f = load3(Fi + 3L * N * c);
+                double v = -dcl * dot(f, ndir) / ut;
+                if (c == t) v += 2.0 / (s.c * ut);
+                aug[(long)c * N + t] = v;
+            }
+            aug[(long)N * N + t] = -cll;
+        }
+    }
+    if (t == 0) ctl[1] = 0.0;
+    __syncthreads();
+
+    const int nw = (T + WAVE - 1) / WAVE;
+    for (int k = 0; k < N; ++k) {
+        const double* colk = aug + (long)k * N;
+        double bv = -1.0;
+        int bi = N;
+        for (int r = k + t; r < N; r += T) {
+            double v = fabs(colk[r]);
+            if (v > bv) {  // a NaN never wins
+                bv = v;
+                bi = r;
+            }
+        }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {  // larger wins, ties to the lower row
+            double ov = __shfl_xor(bv, off, WAVE);
+            int oi = __shfl_xor(bi, off, WAVE);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if ((t % WAVE) == 0) {
+            rv[t / WAVE] = bv;
+            rix[t / WAVE] = (double)bi;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double best = rv[0];
+            int p = (int)rix[0];
+            for (int w = 1; w < nw; ++w) {
+                double ov = rv[w];
+                int oi = (int)rix[w];
+                if (ov > best || (ov == best && oi < p)) {
+                    best = ov;
+                    p = oi;
+                }
+            }
+            double akk = colk[k];
+            if (akk != akk || p >= N) p = k;
+            ctl[0] = (double)p;
+            if (fabs(colk[p]) < 1e-12) ctl[1] = (double)(k + 1);
+        }
+        __syncthreads();
+        if (ctl[1] != 0.0) break;
+        const int p = (int)ctl[0];
+        for (int j = k + t; j <= N; j += T) {
+            double* cj = aug + (long)j * N;
+            double a0 = cj[k], a1 = cj[p];
+            if (p != k) {
+                cj[k] = a1;
+                cj[p] = a0;
+            }
+            prow[j] = p != k ? a1 : a0;
+        }
+        __syncthreads();
+        for (int r = k + 1 + t; r < N; r += T) fac[r] = colk[r] / prow[k];
+        __syncthreads();
+        const int rows = N - k - 1;
+        if (rows > 0) {
+            const int per = T / rows;  // columns per pass; T >= N > rows
+            const int r = t % rows, c0 = t / rows;
+            if (c0 < per) {
+                const double f = fac[k + 1 + r];
+                for (int j = k + 1 + c0; j <= N; j += per) {
+                    double* e = aug + (long)j * N + (k + 1 + r);
+                    *e = *e - f * prow[j];
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const bool singular = ctl[1] != 0.0;
+    if (t == 0) {
+        if (singular) {
+            for (int q = 0; q < N; ++q) G[q] = 0.0;
+            d.scal[4] = ctl[1];
+        } else {
+            for (int ii = N - 1; ii >= 0; --ii) {
+                double sum = 0.0;
+                for (int j = ii + 1; j < N; ++j) sum += aug[(long)j * N + ii] * G[j];
+                G[ii] = (aug[(long)N * N + ii] - sum) / aug[(long)ii * N + ii];
+            }
+        }
+    }
+    __syncthreads();
+
+    double g = G[ic];
+    const V3 ind1 = induced_at(d, s.fixed, G, ic);
+    V3 u = corrected_velocity(vc, vc_ratio, s.felt, ind1);
+    double a;
+    if (viscous) {  // linearized.rs:156-197
+        a = angle_of_attack(s, u);
+        double clf = foil_cl_pre_dropoff(s.foil, a);
+        double cll = foil_cl_linear(s.foil, a);
+        if (fabs(cll) > MIN_POSITIVE) g *= clf / cll;
+        __syncthreads();
+        if (act) G[t] = g;
+        __syncthreads();
+        // NoCorrection keeps the induced velocity of the linear solution
+        u = corrected_velocity(vc, vc_ratio, s.felt, ivc ? induced_at(d, s.fixed, G, ic) : ind1);
+    }
+    a = angle_of_attack(s, u);
+    if (as_start) {
+        if (act) d.lin_gamma[t] = g;
+        return;
+    }
+    if (act) R[t] = residual_term(s, d.density, g, a, u);
+    __syncthreads();
+    if (act) {
+        double* o = d.out;
+        o[OutLayout::gamma(N) + t] = g;
+        o[OutLayout::alpha(N) + t] = a;
+        store3(o + OutLayout::u(N) + 3 * t, u);
+        store3(o + OutLayout::felt(N) + 3 * t, s.felt);
+        if (d.M > 0) {  // update_wing_strength (update_data.rs:146-150)
+            d.str[t] = g;
+            d.rec[t].strength = g;
+        }
+        d.prev_gamma[t] = g;
+    }
+    if (t == 0) {
+        double sum = 0.0;
+        for (int q = 0; q < N; ++q) sum += R[q];
+        d.out[OutLayout::iters(N, d.W)] = 1.0;
+        d.out[OutLayout::residual(N, d.W)] = sum / (double)N;
+    }
+}
+template <bool LDS_MATRIX>
+__global__ __launch_bounds__(1024) void k_linearized(Inst d, int as_start) {
+    extern __shared__ double sh[];
+    linearized_body<LDS_MATRIX>(d, sh, as_start);
 }
 
 // ---------------------------------------------------------------------------
@@ -1492,6 +1744,50 @@ hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st
     else
         hipLaunchKernelGGL(k_fold<MODE_EXACT>, dim3(nb), dim3(256), 0, st, d, ctrl, d.N, p_begin, p_end,
                            d.neglect_self, d.fixed);
+    return hipGetLastError();
+}
+
+hipError_t launch_horseshoe_factors(const Inst& d, hipStream_t st) {
+    long n = (long)d.N * d.N;
+    hipLaunchKernelGGL(k_horseshoe_factors, dim3(blocks_for(n, 256)), dim3(256), 0, st, d);
+    return hipGetLastError();
+}
+
+static long lds_limit_bytes() {
+    static long cached = -1;
+    if (cached < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess)
+            cached = v;
+        else
+            cached = 65536;
+    }
+    return cached;
+}
+
+bool linearized_in_lds(int N) {
+    long bytes = (long)sizeof(double) * (linearized_aux(N) + (long)N * (N + 1));
+    return bytes <= lds_limit_bytes();
+}
+
+hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
+    int threads = ((d.N + WAVE - 1) / WAVE) * WAVE;
+    if (threads < 256) threads = 256;
+    if (threads > 1024 || d.N < 1) return hipErrorInvalidValue;
+    if (linearized_in_lds(d.N)) {
+        size_t shm = sizeof(double) * (linearized_aux(d.N) + (size_t)d.N * (d.N + 1));
+        if (shm > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void*)k_linearized<true>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_linearized<true>, dim3(1), dim3(threads), shm, st, d, as_start);
+    } else {
+        if (!d.lin) return hipErrorInvalidValue;
+        size_t shm = sizeof(double) * linearized_aux(d.N);
+        hipLaunchKernelGGL(k_linearized<false>, dim3(1), dim3(threads), shm, st, d, as_start);
+    }
     return hipGetLastError();
 }
 

@@ -440,33 +440,103 @@ static void parse_wake(WakeConfig& w, const Node* o, const LineModel& m) {  // b
     w.rep_chord = csum / (double)m.chord_len.size();
 }
 
-static void parse_solver(SolverConfig& s, const Node* v) {  // solvers/mod.rs:21-55, simple_iterative.rs:59-84
+static void parse_velocity_correction(SolverConfig& s, const Node* x) {  // velocity_corrections.rs:9-16
+    const Node* q;
+    std::string vt = variant(x, &q, "VelocityCorrections");
+    if (vt == "NoCorrection") s.velocity_correction = 0;
+    else if (vt == "MaxInducedVelocityMagnitudeRatio") {
+        s.velocity_correction = 1;
+        s.velocity_correction_ratio = number(q, "MaxInducedVelocityMagnitudeRatio");
+    } else if (vt == "FixedMagnitudeEqualToFreestream") s.velocity_correction = 2;
+    else throw std::runtime_error("unknown VelocityCorrections `" + vt + "`");
+}
+
+static void parse_linearized(SolverConfig& s, const Node* p) {  // linearized.rs:26-48
+    s.kind = 1;
+    if (!p) return;
+    allow_only(p, {"velocity_corrections", "disable_viscous_corrections", "induced_velocity_correction_method"},
+               "Linearized");
+    if (const Node* x = p->find("velocity_corrections")) parse_velocity_correction(s, x);
+    if (const Node* x = p->find("disable_viscous_corrections"))
+        s.disable_viscous_corrections = boolean(x, "disable_viscous_corrections");
+    if (const Node* x = p->find("induced_velocity_correction_method")) {
+        const Node* q;
+        std::string t = variant(x, &q, "InducedVelocityCorrectionMethod");
+        if (t == "NoCorrection") s.induced_velocity_correction = 0;
+        else if (t == "FullCorrection") s.induced_velocity_correction = 1;
+        else throw std::runtime_error("unknown InducedVelocityCorrectionMethod `" + t + "`");
+    }
+}
+
+// Solver (solvers/mod.rs:21-30, simple_iterative.rs:59-84) or, for quasi-steady
+// settings, QuasiSteadySolverBuilder (mod.rs:32-57): default Linearized, and a
+// SimpleIterative builder with 1000 iterations and damping 0.04 by default
+// (simple_iterative.rs:14-45).
+static void parse_solver(SolverConfig& s, const Node* v, bool quasi) {
+    if (quasi) {
+        s.max_iterations = 1000;
+        s.damping = 0.04;
+        if (!v) {
+            s.kind = 1;
+            return;
+        }
+    }
     if (!v) return;
     const Node* p;
     std::string t = variant(v, &p, "Solver");
-    if (t != "SimpleIterative") throw std::runtime_error("Solver `" + t + "` is not supported on the GPU path");
-    allow_only(p, {"max_iterations_per_time_step", "damping_factor", "residual_tolerance_absolute",
-                   "strength_difference_tolerance", "velocity_corrections", "use_raw_circulation_during_iterations",
-                   "start_with_linearized_solution", "correction_factor_initial_solution"},
-               "SimpleIterative");
+    if (t == "Linearized") {
+        parse_linearized(s, p);
+        return;
+    }
+    if (t != "SimpleIterative") throw std::runtime_error("unknown Solver `" + t + "`");
+    if (quasi)
+        allow_only(p, {"max_iterations_per_time_step", "damping_factor", "residual_tolerance_absolute",
+                       "strength_difference_tolerance", "velocity_corrections", "start_with_linearized_solution",
+                       "correction_factor_initial_solution"},
+                   "QuasiSteadySimpleIterativeBuilder");
+    else
+        allow_only(p, {"max_iterations_per_time_step", "damping_factor", "residual_tolerance_absolute",
+                       "strength_difference_tolerance", "velocity_corrections", "use_raw_circulation_during_iterations",
+                       "start_with_linearized_solution", "correction_factor_initial_solution"},
+                   "SimpleIterative");
     if (const Node* x = p->find("max_iterations_per_time_step")) s.max_iterations = (int)number(x, "max_iterations_per_time_step");
     if (const Node* x = p->find("damping_factor")) s.damping = number(x, "damping_factor");
     if (const Node* x = p->find("residual_tolerance_absolute")) s.residual_tol = number(x, "residual_tolerance_absolute");
     if (const Node* x = p->find("strength_difference_tolerance")) s.strength_tol = number(x, "strength_difference_tolerance");
     if (const Node* x = p->find("use_raw_circulation_during_iterations")) boolean(x, "use_raw_circulation_during_iterations");
     if (const Node* x = p->find("start_with_linearized_solution"))
-        if (boolean(x, "start_with_linearized_solution"))
-            throw std::runtime_error("start_with_linearized_solution is not supported on the GPU path");
+        s.start_linearized = boolean(x, "start_with_linearized_solution");
     if (const Node* x = p->find("correction_factor_initial_solution")) s.correction_factor_initial = number(x, "correction_factor_initial_solution");
-    if (const Node* x = p->find("velocity_corrections")) {
-        const Node* q;
-        std::string vt = variant(x, &q, "VelocityCorrections");
-        if (vt == "NoCorrection") s.velocity_correction = 0;
-        else if (vt == "MaxInducedVelocityMagnitudeRatio") {
-            s.velocity_correction = 1;
-            s.velocity_correction_ratio = number(q, "MaxInducedVelocityMagnitudeRatio");
-        } else if (vt == "FixedMagnitudeEqualToFreestream") s.velocity_correction = 2;
-        else throw std::runtime_error("unknown VelocityCorrections `" + vt + "`");
+    if (const Node* x = p->find("velocity_corrections")) parse_velocity_correction(s, x);
+}
+
+static void parse_quasi_wake(QuasiConfig& q, const Node* o) {  // wake/settings.rs:15-58
+    if (!o) return;
+    allow_only(o, {"wake_length_factor", "symmetry_condition", "viscous_core_length"}, "QuasiSteadyWakeSettings");
+    const Node* p;
+    if (const Node* v = o->find("wake_length_factor")) q.wake_length_factor = number(v, "wake_length_factor");
+    if (const Node* v = o->find("symmetry_condition")) {
+        std::string t = variant(v, &p, "SymmetryCondition");
+        if (t == "NoSymmetry") q.symmetry = 0;
+        else if (t == "X") q.symmetry = 1;
+        else if (t == "Y") q.symmetry = 2;
+        else if (t == "Z") q.symmetry = 3;
+        else throw std::runtime_error("unknown SymmetryCondition `" + t + "`");
+    }
+    if (const Node* v = o->find("viscous_core_length")) {
+        std::string t = variant(v, &p, "ViscousCoreLength");
+        if (t == "Relative") {
+            q.core_kind = 0;
+            q.core_value = number(p, "Relative");
+        } else if (t == "Absolute") {
+            q.core_kind = 1;
+            q.core_value = number(p, "Absolute");
+        } else if (t == "NoViscousCore") {
+            q.core_kind = 2;
+            q.core_value = 0.0;
+        } else {
+            throw std::runtime_error("unknown ViscousCoreLength `" + t + "`");
+        }
     }
 }
 
@@ -478,13 +548,24 @@ Setup parse_setup(const char* text) {
     if (!lfm) throw std::runtime_error("SimulationBuilder: missing field `line_force_model`");
     Setup s;
     s.lfm = parse_line_model(lfm);
+    // simulation_builder.rs:42-64: absent settings are QuasiSteadySettings::default()
     const Node* ss = root->find("simulation_settings");
-    if (!ss) throw std::runtime_error("QuasiSteady simulations are not supported on the GPU path");
-    const Node* p;
-    std::string t = variant(ss, &p, "SimulationSettings");
-    if (t != "Dynamic") throw std::runtime_error("SimulationSettings `" + t + "` is not supported on the GPU path");
+    const Node* p = nullptr;
+    std::string t = ss ? variant(ss, &p, "SimulationSettings") : std::string("QuasiSteady");
+    if (t == "QuasiSteady") {
+        if (p) allow_only(p, {"solver", "wake"}, "QuasiSteadySettings");
+        s.quasi = true;
+        parse_solver(s.solver, p ? p->find("solver") : nullptr, true);
+        parse_quasi_wake(s.qs, p ? p->find("wake") : nullptr);
+        s.wake.M = 0;
+        double csum = 0.0;
+        for (double c : s.lfm.chord_len) csum += c;
+        s.wake.rep_chord = csum / (double)s.lfm.chord_len.size();
+        return s;
+    }
+    if (t != "Dynamic") throw std::runtime_error("unknown SimulationSettings `" + t + "`");
     if (p) allow_only(p, {"solver", "wake"}, "DynamicSettings");
-    parse_solver(s.solver, p ? p->find("solver") : nullptr);
+    parse_solver(s.solver, p ? p->find("solver") : nullptr, false);
     parse_wake(s.wake, p ? p->find("wake") : nullptr, s.lfm);
     return s;
 }

@@ -72,15 +72,30 @@ struct WakeConfig {
 };
 
 struct SolverConfig {
+    int kind = 0;  // 0 SimpleIterative, 1 Linearized (solvers/mod.rs:21-24)
     int max_iterations = 40;
     double damping = 0.05, residual_tol = 1e-4, strength_tol = 1e-6;
     int velocity_correction = 0;  // 0 none, 1 max ratio, 2 fixed magnitude
     double velocity_correction_ratio = 0.0;
     double correction_factor_initial = 1.0;
+    bool start_linearized = false;  // start_with_linearized_solution
+    // Linearized settings (linearized.rs:26-48); the velocity correction above applies
+    bool disable_viscous_corrections = false;
+    int induced_velocity_correction = 1;  // 0 NoCorrection, 1 FullCorrection
+};
+
+// Quasi-steady wake (wake/settings.rs:33-58): one horseshoe vortex per strip.
+struct QuasiConfig {
+    double wake_length_factor = 100.0;
+    int symmetry = 0;     // 0 none, 1 X, 2 Y, 3 Z
+    int core_kind = 0;    // 0 Relative, 1 Absolute, 2 NoViscousCore
+    double core_value = 0.1;
 };
 
 struct Setup {
     LineModel lfm;
+    bool quasi = false;   // SimulationSettings::QuasiSteady (no shed wake)
+    QuasiConfig qs;
     WakeConfig wake;
     SolverConfig solver;
 };

@@ -171,8 +171,10 @@ int sb_sim_set_wake_state(sb_sim* s, const double* pts, const double* vel, const
             s->first_done = true;
             check(hipMemset(s->d_prev, 0, sizeof(double) * s->N), "memset");
         }
-        Inst d = s->inst();
-        check(launch_panels(d, d.pts, 0, s->stream), "panels");
+        if (s->nr_panels > 0) {
+            Inst d = s->inst();
+            check(launch_panels(d, d.pts, 0, s->stream), "panels");
+        }
         check(hipStreamSynchronize(s->stream), "sync");
     });
 }
@@ -314,6 +316,11 @@ int sb_debug_libm_device(int fn, const double* x, const double* y, double* out, 
     }
     if (d) (void)hipFree(d);
     return rc;
+}
+
+int sb_debug_linearized_in_shared_memory(long n_strips) {
+    if (n_strips < 1 || n_strips > 1024) return -1;
+    return linearized_in_lds((int)n_strips) ? 1 : 0;
 }
 
 int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n) {

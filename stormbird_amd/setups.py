@@ -117,6 +117,22 @@ def rectangular_wing(aspect_ratio=5.0, cl_zero_angle=0.5, angle_of_attack=0.0, n
     return json.dumps(setup)
 
 
+def wing_sails_quasi_steady(nr_strips=40, solver=None):
+    """Three 10 m wing sails, chord 1 m, at (x, y) = (-5, 5), (0, -5), (5, 5), z from 10 m, with
+    default quasi-steady Linearized settings: the reference's speed test
+    (lifting_line/tests/wing_sails_speed_test.rs:25-78); freestream 1.2 m/s along x."""
+    wings = []
+    for x, y in ((-5.0, 5.0), (0.0, -5.0), (5.0, 5.0)):
+        wings.append({
+            "section_points": [_v(x, y, 10.0), _v(x, y, 20.0)],
+            "chord_vectors": [_v(1, 0, 0), _v(1, 0, 0)],
+            "section_model": foil(cl_zero_angle=0.5),
+            "non_zero_circulation_at_ends": [False, False],
+        })
+    return json.dumps({"line_force_model": {"wing_builders": wings, "nr_sections": int(nr_strips)},
+                       "simulation_settings": quasi_steady_settings(solver=solver)})
+
+
 DEG = math.pi / 180.0
 
 # Section model of the wing-sail workloads: CL = 2*pi*sin(alpha) blended to zero around 15 deg

@@ -1,0 +1,134 @@
+"""GPU parity of the quasi-steady path and the Linearized solver against the oracle.
+
+EXACT mode equals the oracle built with the portable libm bit for bit: the
+horseshoe frozen wake (frozen_wake.rs:68-129), the Linearized solve with its
+Gaussian elimination (linearized.rs:54-222, linalg.rs:41-104) in the shared-memory
+and the global-memory matrix forms, the quasi-steady SimpleIterative solve, the
+start_with_linearized_solution start, and the Linearized solver under dynamic
+settings with a shed wake.
+"""
+import ctypes
+import json
+import math
+
+import numpy as np
+import pytest
+
+import stormbird_amd as sb
+from stormbird_amd import setups as S
+
+from test_gpu_parity import compare_results, same
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(oracle, js, u, steps=1, dt=0.1, motion=None):
+    g = sb.Simulation(js)
+    o = oracle.Simulation(js, "pm")
+    assert g.nr_freestream_points() == o.nr_freestream_points()
+    fs = S.uniform_freestream(g.nr_freestream_points(), u)
+    rg = ro = None
+    for k in range(steps):
+        t = (k + 1) * dt
+        if motion is not None:
+            tr, rot = motion(t)
+            for sim in (g, o):
+                sim.set_translation_and_rotation_with_finite_difference_for_the_velocity(dt, tr, rot)
+        rg = g.do_step(time=t, time_step=dt, freestream_velocity=fs)
+        ro = o.do_step(time=t, time_step=dt, freestream_velocity=fs)
+        compare_results(rg, ro, where=f"step {k}")
+    return g, o, rg, ro
+
+
+def test_default_quasi_steady_rectangular_wing(oracle):
+    """compare_models.rs steady case (AR 5, 64 strips, default QuasiSteady + Linearized)."""
+    js = S.with_settings(S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=1.2, angle_of_attack=2 * S.DEG,
+                                            nr_strips=64), None)
+    g, o, rg, ro = run_pair(oracle, js, [1.2, 0, 0], dt=0.25 / 1.2)
+    assert rg.iterations == 1 and g.nr_panels == 0
+    fg, fo = g.get_frozen_wake(), o.get_frozen_wake()
+    assert same(fg[0], fo[0]) and same(fg[1], fo[1])
+    ff = 0.5 * 13.2 * 1.2 ** 2 * 5.0
+    cl_t = (1.2 + 2 * math.pi * 2 * S.DEG) / (1 + 2 / 5.0)
+    assert abs(rg.integrated_forces_sum()[1] / ff - cl_t) / cl_t < 0.07
+
+
+def test_wing_sails_speed_test_setup(oracle):
+    """wing_sails_speed_test.rs: three wings x 40 strips, Linearized, repeated steps."""
+    run_pair(oracle, S.wing_sails_quasi_steady(), [1.2, 0, 0], steps=3, dt=1.0)
+
+
+@pytest.mark.parametrize("nr_strips", [150, 300])
+def test_linearized_matrix_in_global_memory(oracle, nr_strips):
+    """Matrices beyond the workgroup's shared memory go through global memory."""
+    L = sb.lib()
+    L.sb_debug_linearized_in_shared_memory.argtypes = [ctypes.c_long]
+    assert L.sb_debug_linearized_in_shared_memory(nr_strips) == 0
+    assert L.sb_debug_linearized_in_shared_memory(64) == 1
+    js = S.with_settings(S.rectangular_wing(aspect_ratio=8.0, cl_zero_angle=0.3, angle_of_attack=4 * S.DEG,
+                                            nr_strips=nr_strips), None)
+    run_pair(oracle, js, [7.0, 0.5, 0], steps=2)
+
+
+@pytest.mark.parametrize("solver", [
+    {"Linearized": {"disable_viscous_corrections": True}},
+    {"Linearized": {"induced_velocity_correction_method": "NoCorrection"}},
+    {"Linearized": {"velocity_corrections": {"MaxInducedVelocityMagnitudeRatio": 0.05}}},
+    {"Linearized": {"velocity_corrections": "FixedMagnitudeEqualToFreestream"}},
+])
+def test_linearized_settings(oracle, solver):
+    js = S.wing_sails_quasi_steady(nr_strips=24, solver=solver)
+    run_pair(oracle, js, [6.0, 2.0, 0], steps=2)
+
+
+def test_quasi_steady_simple_iterative_and_linearized_start(oracle):
+    base = {"max_iterations_per_time_step": 3000, "residual_tolerance_absolute": 1e-9,
+            "strength_difference_tolerance": 1e-10}
+    wing = S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=0.4, angle_of_attack=3 * S.DEG, nr_strips=32)
+    _, _, cold, _ = run_pair(oracle, S.with_settings(wing, S.quasi_steady_settings(solver=base)), [5, 0, 0])
+    _, _, warm, _ = run_pair(oracle, S.with_settings(wing, S.quasi_steady_settings(
+        solver=dict(base, start_with_linearized_solution=True))), [5, 0, 0])
+    assert 1 < warm.iterations < cold.iterations
+
+
+def test_quasi_steady_symmetry_and_core_settings(oracle):
+    for sym, core, factor in (("Z", None, None), ("Y", {"Absolute": 0.05}, 40.0), (None, "NoViscousCore", 200.0)):
+        d = json.loads(S.rectangular_wing(aspect_ratio=4.0, cl_zero_angle=0.2, angle_of_attack=5 * S.DEG,
+                                          nr_strips=20))
+        d["simulation_settings"] = S.quasi_steady_settings(symmetry=sym, core=core, wake_length_factor=factor)
+        run_pair(oracle, json.dumps(d), [8.0, 0.3, 0.1], steps=2)
+
+
+def test_quasi_steady_heave_and_roll(oracle):
+    """The reference's translating/rotating quasi-steady cases (dynamic_simulation.rs:119-237),
+    step for step with flow-derivative added mass."""
+    js = S.with_settings(S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=0.0, negative_span_orientation=True),
+                         S.quasi_steady_settings())
+    w = math.pi
+    run_pair(oracle, js, [1.2, 0, 0], steps=19, motion=lambda t: ([0, 0.23 * math.sin(t * w), 0], [0, 0, 0]))
+    run_pair(oracle, js, [10.2, 0, 0], steps=19,
+             motion=lambda t: ([0, 0, 0], [5 * S.DEG * math.sin(w * t), 0, 0]))
+
+
+def test_dynamic_linearized_solver(oracle):
+    """Solver::Linearized under dynamic settings: one direct solve per step, shed wake kept."""
+    js = S.rectangular_wing(aspect_ratio=4.0, cl_zero_angle=0.2, angle_of_attack=3 * S.DEG, nr_strips=16,
+                            nr_panels=30, solver={"Linearized": {}})
+    g, o, rg, _ = run_pair(oracle, js, [6.0, 0, 0], steps=12)
+    assert rg.iterations == 1
+    wg, wo = g.get_wake_state(), o.get_wake_state()
+    for key in ("points", "velocities", "strengths"):
+        assert same(wg[key], wo[key]), key
+
+
+def test_dynamic_start_with_linearized_solution(oracle):
+    js = S.rectangular_wing(aspect_ratio=4.0, cl_zero_angle=0.2, angle_of_attack=3 * S.DEG, nr_strips=16,
+                            nr_panels=30, solver={"start_with_linearized_solution": True})
+    run_pair(oracle, js, [6.0, 0, 0], steps=12)
+
+
+def test_quasi_steady_has_no_wake_api(oracle):
+    g = sb.Simulation(S.wing_sails_quasi_steady(nr_strips=8))
+    assert g.nr_freestream_points() == g.nr_ctrl_points == 24
+    with pytest.raises(RuntimeError, match="quasi-steady"):
+        g.induced_velocities(np.zeros((2, 3)))

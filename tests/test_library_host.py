@@ -89,8 +89,15 @@ def test_unsupported_settings_rejected(oracle):
     import stormbird_amd.setups as S
 
     d = json.loads(S.single_wing_sail())
-    d["simulation_settings"] = {"QuasiSteady": {}}
-    with pytest.raises(ValueError, match="not supported"):
+    d["simulation_settings"] = {"Steady": {}}
+    with pytest.raises(ValueError, match="unknown SimulationSettings"):
+        sb.Simulation(json.dumps(d))
+    d["simulation_settings"] = {"QuasiSteady": {"solver": {"QuasiNewton": {}}}}
+    with pytest.raises(ValueError, match="unknown Solver"):
+        sb.Simulation(json.dumps(d))
+    d["simulation_settings"] = {"QuasiSteady": {"solver": {"SimpleIterative": {
+        "use_raw_circulation_during_iterations": True}}}}
+    with pytest.raises(ValueError, match="unknown field"):
         sb.Simulation(json.dumps(d))
     d = json.loads(S.single_wing_sail())
     d["simulation_settings"]["Dynamic"]["wake"]["bogus"] = 1

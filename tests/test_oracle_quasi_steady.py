@@ -223,3 +223,34 @@ def test_dynamic_linearized_solver(orc):
         assert r.iterations == 1
     assert np.all(np.isfinite(r.force_input.circulation_strength))
     assert np.any(sim.get_wake_state()["strengths"] != 0.0)
+
+
+def test_reference_force_opposes_translation(orc):
+    """dynamic_simulation.rs:119-174: quasi-steady (default settings), heave 0.23 sin(pi t)."""
+    sim = orc.Simulation(S.with_settings(S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=0.0,
+                                                            negative_span_orientation=True),
+                                         S.quasi_steady_settings()))
+    fs = S.uniform_freestream(sim.nr_freestream_points(), [1.2, 0, 0])
+    dt, w, amp = 0.1, 2 * math.pi / 2.0, 0.23
+    ff = 0.5 * 5.0 * 1.2 ** 2 * 13.2
+    for i in range(1, 20):
+        t = i * dt
+        sim.set_translation_and_rotation_with_finite_difference_for_the_velocity(
+            dt, [0, amp * math.sin(t * w), 0], [0, 0, 0])
+        r = sim.do_step(time=t, time_step=dt, freestream_velocity=fs)
+        assert r.integrated_forces_sum()[1] / ff * (amp * w * math.cos(t * w)) <= 0.0
+
+
+def test_reference_moment_opposes_rotation(orc):
+    """dynamic_simulation.rs:180-237: quasi-steady (default settings), roll 5 deg sin(pi t)."""
+    sim = orc.Simulation(S.with_settings(S.rectangular_wing(aspect_ratio=5.0, cl_zero_angle=0.0,
+                                                            negative_span_orientation=True),
+                                         S.quasi_steady_settings()))
+    fs = S.uniform_freestream(sim.nr_freestream_points(), [10.2, 0, 0])
+    dt, w, amp = 0.1, 2 * math.pi / 2.0, 5.0 * S.DEG
+    for i in range(1, 20):
+        t = i * dt
+        sim.set_translation_and_rotation_with_finite_difference_for_the_velocity(
+            dt, [0, 0, 0], [amp * math.sin(w * t), 0, 0])
+        r = sim.do_step(time=t, time_step=dt, freestream_velocity=fs)
+        assert r.integrated_moments_sum()[0] * (w * amp * math.cos(w * t)) <= 0.0
