@@ -402,6 +402,54 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
     return out
 
 
+def quasi_steady(steps=50):
+    """Quasi-steady path (horseshoe frozen wake + Linearized solve, the reference's default
+    settings): the reference's speed-test setup (3 wing sails x 40 strips,
+    wing_sails_speed_test.rs) and a 3 x 340-strip case whose 1020 x 1020 system sits in
+    global memory; per-step GPU time beside the CPU restatement (one thread)."""
+    import stormbird_amd as sb
+    from stormbird_amd import setups as S
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+
+    cases = [("3 wing sails x 40 strips (N = 120), Linearized", S.wing_sails_quasi_steady(), [1.2, 0.0, 0.0], steps),
+             ("3 wing sails x 340 strips (N = 1020), Linearized", S.wing_sails_quasi_steady(nr_strips=340),
+              [1.2, 0.0, 0.0], max(3, steps // 10)),
+             ("3 wing sails x 40 strips (N = 120), quasi-steady SimpleIterative from the Linearized start",
+              S.wing_sails_quasi_steady(solver={"start_with_linearized_solution": True}), [1.2, 0.0, 0.0], steps)]
+    out = []
+    for name, js, u, n in cases:
+        g = sb.Simulation(js)
+        fs = S.uniform_freestream(g.nr_freestream_points(), u)
+        for _ in range(2):
+            g.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
+        dev = []
+        t0 = time.perf_counter()
+        for _ in range(n):
+            r = g.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
+            dev.append(sum(g.last_kernel_ms(k) for k in ("factors", "solve", "forces")))
+        t_gpu = (time.perf_counter() - t0) / n
+        o = oracle_py.Simulation(js, "glibc")
+        o.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
+        n_cpu = max(1, min(n, 5)) if g.nr_ctrl_points > 500 else n
+        t0 = time.perf_counter()
+        for _ in range(n_cpu):
+            o.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
+        t_cpu = (time.perf_counter() - t0) / n_cpu
+        out.append({"workload": name, "n_ctrl_points": g.nr_ctrl_points, "iterations": r.iterations,
+                    "gpu_ms_per_step": 1e3 * t_gpu,
+                    "gpu_device_ms_per_step": float(np.mean(dev)),
+                    "kernel_ms": {k: g.last_kernel_ms(k) for k in ("factors", "solve", "forces")},
+                    "linearized_matrix_in_shared_memory": bool(
+                        sb.lib().sb_debug_linearized_in_shared_memory(g.nr_ctrl_points) == 1),
+                    "cpu_ms_per_step": 1e3 * t_cpu, "cpu": "oracle restatement, 1 thread",
+                    "gpu_vs_cpu": t_cpu / t_gpu})
+        del g, o
+    return {"cases": out, "step_times": "host-to-host do_step (freestream upload and result download included); "
+                                        "device time is the sum of the step's kernel intervals"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -417,7 +465,8 @@ def main():
     ap.add_argument("--no-route", action="store_true")
     ap.add_argument("--route-ships", type=int, default=64)
     ap.add_argument("--train-conditions", type=int, default=16, help="inflow conditions per instance for training")
-    ap.add_argument("--only", choices=("batched", "warm", "route"), default=None,
+    ap.add_argument("--no-quasi", action="store_true")
+    ap.add_argument("--only", choices=("batched", "warm", "route", "quasi"), default=None,
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -439,6 +488,8 @@ def main():
             out = batched_polar_sweep(n_inst=args.batch_instances, steps=args.steps)
         elif args.only == "warm":
             out, _ = warm_start_polar(n_inst=args.batch_instances, train_conditions=args.train_conditions)
+        elif args.only == "quasi":
+            out = quasi_steady()
         else:
             out = route_simulation(n_ships=args.route_ships, steps=args.steps)
         print(json.dumps({"section": args.only, **out}))
@@ -551,6 +602,8 @@ def main():
         "cpu_baseline": cpu,
     }
     mlp = None
+    if not args.no_quasi:
+        line["quasi_steady"] = quasi_steady()
     if not args.no_batch:
         del sim
         line["batched"] = batched_polar_sweep(n_inst=args.batch_instances)

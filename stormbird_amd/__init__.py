@@ -75,6 +75,7 @@ def lib():
         L.sb_wake_induced_velocities.argtypes = [_dp, ctypes.c_size_t, _dp, ctypes.c_size_t, ctypes.c_int,
                                                  ctypes.c_int, _dp]
         L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
+        L.sb_debug_linearized_in_shared_memory.argtypes = [ctypes.c_long]
         L.sb_debug_libm_host.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
         L.sb_batch_create.restype = vp
         L.sb_batch_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_long, ctypes.c_int, ctypes.c_int]

@@ -104,13 +104,14 @@ struct sb_batch {
         fs_off.resize(n + 1);
         out_off.resize(n + 1);
         geo_off[0] = fs_off[0] = out_off[0] = 0;
-        int max_n = 1, sym = 0;
+        int max_n = 1, max_w = 1, sym = 0;
         for (int i = 0; i < n; ++i) {
             const sb_sim& s = *sims[i];
             geo_off[i + 1] = geo_off[i] + s.gl.total;
             fs_off[i + 1] = fs_off[i] + 3L * (s.N + s.nr_points);
             out_off[i + 1] = out_off[i] + OutLayout::total(s.N, s.W);
             max_n = std::max(max_n, s.N);
+            max_w = std::max(max_w, s.W);
             sym |= s.setup.wake.symmetry != 0 ? 1 : 0;
         }
         geo_total = geo_off[n];
@@ -161,7 +162,7 @@ struct sb_batch {
         bd.n = n;
         bd.off = d_off;
         bd.max_threads = std::min(1024, ((max_n + 63) / 64) * 64);
-        bd.solve_shm = sizeof(double) * (2 * (size_t)max_n + 16 + 4);
+        bd.solve_shm = solve_shared_bytes(max_n, max_w);
         bd.counters = d_counters;
         bd.sym = sym;
         bd.fixed_jobs = d_jobs;

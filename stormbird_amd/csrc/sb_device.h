@@ -57,7 +57,8 @@ struct Inst {
     int lin_ivc;              // Linearized: 1 recompute the induced velocity after it
     double hs_wake_length;    // horseshoe trailing-leg length
     double hs_core;           // horseshoe viscous core length
-    double* lin;         // Linearized augmented matrix [N + 1][N] when it does not fit in LDS
+    double* lin;         // Linearized scratch when the matrix does not fit in shared memory
+                         // (linearized_global_doubles(N) doubles)
     double* lin_gamma;   // [N] Linearized circulation that starts SimpleIterative
 
     double* pts;         // current wake points [P*(M+1)][3]
@@ -138,7 +139,9 @@ hipError_t launch_horseshoe_factors(const Inst& d, hipStream_t st);
 // unless linearized_in_lds(d.N).
 hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st);
 bool linearized_in_lds(int N);
+long linearized_global_doubles(int N);
 hipError_t launch_solve(const Inst& d, hipStream_t st);
+size_t solve_shared_bytes(int N, int W);
 hipError_t launch_set_row0(const Inst& d, hipStream_t st);
 hipError_t launch_velocity_copy(const Inst& d, hipStream_t st);
 hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStream_t st);

@@ -287,7 +287,7 @@ struct sb_sim {
         d_span_src = dalloc<int>(2 * P);
         d_foil = dalloc<FoilParams>(W);
         if (setup.solver.kind == 1 || setup.solver.start_linearized) {
-            if (!linearized_in_lds(N)) d_lin = dalloc<double>((size_t)N * (N + 1));
+            if (!linearized_in_lds(N)) d_lin = dalloc<double>((size_t)linearized_global_doubles(N));
             d_lin_gamma = dalloc<double>(N);
         }
         // SB_PAIR_SCHEDULE=plain keeps one workgroup per 256 targets for the pair

@@ -796,13 +796,15 @@ __global__ void k_panels(Inst d, int shift) { panels_body(d, shift, (long)blockI
 // ---------------------------------------------------------------------------
 // Fixed-point solve (simple_iterative.rs:86-212), one workgroup
 // ---------------------------------------------------------------------------
+// The foil stays in memory (a pointer, not a copy): its 18 fields held in
+// registers for every strip push the one-workgroup kernels into spills.
 struct Strip {
     V3 felt, fixed, rel_g, dir_g, chord_g, rel_out;
     double c, local_len;
-    FoilParams foil;
+    const FoilParams* foil;
 };
 
-__device__ inline double strip_cl_pre(const Strip& s, double a) { return foil_cl_pre_dropoff(s.foil, a); }
+__device__ inline double strip_cl_pre(const Strip& s, double a) { return foil_cl_pre_dropoff(*s.foil, a); }
 
 // force_calculations.rs:552-627 (one strip's |CL(Gamma) - CL(alpha)|)
 __device__ inline double residual_term(const Strip& s, double rho, double g, double a, V3 u) {
@@ -817,11 +819,12 @@ __device__ inline double residual_term(const Strip& s, double rho, double g, dou
 // force_calculations.rs:271-292
 __device__ inline double circulation(const Strip& s, int viscous, double a, V3 u) {
     double cl = strip_cl_pre(s, a);
-    if (viscous) cl += foil_cl_post_weighted(s.foil, a);
+    if (viscous) cl += foil_cl_post_weighted(*s.foil, a);
     return -0.5 * s.c * len(u) * cl;
 }
 
-__device__ inline Strip load_strip(const Inst& d, int ic) {
+// Strip ic; its foil from `foils` (a shared-memory copy) when given.
+__device__ inline Strip load_strip(const Inst& d, int ic, const FoilParams* foils = nullptr) {
     const double* geo = d.geo;
     Strip s;
     s.felt = load3(d.felt + 3 * ic);
@@ -832,17 +835,49 @@ __device__ inline Strip load_strip(const Inst& d, int ic) {
     s.rel_out = load3(geo + d.gl.rel_out + 3 * ic);
     s.c = geo[d.gl.chord_len + ic];
     s.local_len = geo[d.gl.local_len + ic];
-    s.foil = d.foil[d.wing_of[ic]];
+    s.foil = (foils ? foils : d.foil) + d.wing_of[ic];
     return s;
 }
 
+// Copies the instance's foils into shared memory (the caller synchronizes).
+// Loop-invariant loads from global memory would be hoisted out of the solver's
+// iteration and held in registers; shared-memory copies are read where used.
+constexpr int FOIL_DOUBLES = (int)(sizeof(FoilParams) / sizeof(double));
+__device__ inline void stage_foils(const Inst& d, FoilParams* dst) {
+    const double* src = (const double*)d.foil;
+    double* o = (double*)dst;
+    for (int q = threadIdx.x; q < d.W * FOIL_DOUBLES; q += blockDim.x) o[q] = src[q];
+}
+
 // frozen_wake.rs:268-301: fixed velocity plus the factor row times G (three
-// running sums in column order, added to the fixed velocity at the end).
+// running sums in column order, added to the fixed velocity at the end).  The
+// factors of MATVEC_U columns are loaded before any of them is used, so that
+// many loads are in flight (the sums keep their order).
+constexpr int MATVEC_U = 8;
 __device__ inline V3 induced_at(const Inst& d, V3 fixed, const double* G, int i) {
     const int N = d.N;
     double sx = 0.0, sy = 0.0, sz = 0.0;
     const double* Fi = d.F + 3 * i;
-    for (int j = 0; j < N; ++j) {
+    int j = 0;
+    for (; j + MATVEC_U <= N; j += MATVEC_U) {
+        double fx[MATVEC_U], fy[MATVEC_U], fz[MATVEC_U], g[MATVEC_U];
+#pragma unroll
+        for (int u = 0; u < MATVEC_U; ++u) {
+            const double* f = Fi + 3L * N * (j + u);
+            fx[u] = f[0];
+            fy[u] = f[1];
+            fz[u] = f[2];
+        }
+#pragma unroll
+        for (int u = 0; u < MATVEC_U; ++u) g[u] = G[j + u];
+#pragma unroll
+        for (int u = 0; u < MATVEC_U; ++u) {
+            sx += fx[u] * g[u];
+            sy += fy[u] * g[u];
+            sz += fz[u] * g[u];
+        }
+    }
+    for (; j < N; ++j) {
         const double* f = Fi + 3L * N * j;
         double gj = G[j];
         sx += f[0] * gj;
@@ -893,10 +928,13 @@ __device__ inline void solve_body(const Inst& d, double* sh) {
     double* R = sh + N;         // [N] residual terms
     double* wm = R + N;         // [16] per-wave maxima
     double* ctl = wm + 16;      // [0] residual, [1] converged flag
+    FoilParams* foils = (FoilParams*)(ctl + 4);  // [W]
     int i = threadIdx.x;
     bool act = i < N;
     int ic = act ? i : N - 1;
-    Strip s = load_strip(d, ic);
+    stage_foils(d, foils);
+    __syncthreads();
+    Strip s = load_strip(d, ic, foils);
     double rho = d.density;
 
     double g = d.prev_gamma[ic];
@@ -964,7 +1002,9 @@ __device__ inline void solve_body(const Inst& d, double* sh) {
         d.out[OutLayout::residual(N, d.W)] = ctl[0];
     }
 }
-__global__ __launch_bounds__(1024) void k_solve(Inst d) {
+// MAXT = 512 covers N <= 512 with twice the registers per thread (no spills).
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void k_solve(Inst d) {
     extern __shared__ double sh[];
     solve_body(d, sh);
 }
@@ -1013,157 +1053,106 @@ __global__ void k_horseshoe_factors(Inst d) {
     horseshoe_factors_body(d, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Shared-memory doubles of the Linearized solve besides the matrix.
-SB_HD inline long linearized_aux(int N) { return 4L * N + 1 + 16 + 16 + 4; }
+// Linearized::solve (linearized.rs:54-222).  The augmented matrix [A | b] is
+// kept column-major (column j at aug + j*N), so a pivot column is contiguous
+// and trailing updates are coalesced.  Gaussian elimination with partial
+// pivoting follows linalg.rs:41-104 element by element: every entry receives
+// the same sequence of a - f*p updates (f and p rounded as there), the pivot is
+// the first row of largest magnitude (a NaN diagonal is kept), and back
+// substitution sums in ascending column order.  Entries below the diagonal of
+// an eliminated column are never read again; they are not updated (or hold the
+// factors, in the blocked form).  A pivot below 1e-12 in magnitude (where the
+// reference's unwrap panics) records its row + 1 in scal[4] and the circulation
+// is then zero.  as_start = 1: the default settings, and the circulation goes to
+// lin_gamma only (the start of a SimpleIterative solve).
 
-// Linearized::solve (linearized.rs:54-222), one workgroup of T >= N threads.
-// The augmented matrix [A | b] is kept column-major (column j at aug + j*N),
-// so that a pivot column is contiguous and the trailing update is coalesced.
-// Gaussian elimination with partial pivoting follows linalg.rs:41-104 element
-// by element: every entry receives the same sequence of a - f*p updates (f and
-// p rounded as there), the pivot is the first row of largest magnitude (a NaN
-// diagonal is kept), and back substitution sums in ascending column order.
-// Sub-diagonal entries of eliminated columns are never read again, so they are
-// neither updated nor swapped.  A pivot below 1e-12 in magnitude (where the
-// reference's unwrap panics) records its row + 1 in scal[4]; the circulation is
-// then zero.  as_start = 1: default settings, circulation to lin_gamma only.
-template <bool LDS_MATRIX>
-__device__ inline void linearized_body(const Inst& d, double* sh, int as_start) {
+// linearized.rs:75-131: what row i of the system needs from strip i.
+struct LinRow {
+    V3 ndir;   // normal direction: velocity direction x span direction
+    double ut, dcl, cll, c;
+};
+__device__ inline LinRow lin_row(const Strip& s) {
+    V3 u = s.felt + s.fixed;
+    u = u - project(u, s.rel_g);  // remove_span_velocity
+    V3 vdir = normalize(u);
+    LinRow q;
+    q.ndir = cross(vdir, normalize(s.rel_g));
+    double a = angle_of_attack(s, u);
+    q.cll = foil_cl_linear(*s.foil, a);
+    q.dcl = s.foil->cl_initial_slope;
+    q.ut = dot(u, vdir);
+    q.c = s.c;
+    return q;
+}
+__device__ inline double lin_entry(const Inst& d, const LinRow& q, int r, int c) {
+    V3 f = load3(d.F + 3L * ((long)c * d.N + r));
+    double v = -q.dcl * dot(f, q.ndir) / q.ut;
+    if (c == r) v += 2.0 / (q.c * q.ut);
+    return v;
+}
+
+// One wave's argmax of |colk[r]| over r in [k, N): larger wins, ties go to the
+// lower row, NaN never wins; every lane returns the same row (N if none).
+__device__ inline int wave_pivot(const double* colk, int k, int N, int lane) {
+    double bv = -1.0;
+    int bi = N;
+    for (int r = k + lane; r < N; r += WAVE) {
+        double v = fabs(colk[r]);
+        if (v > bv) {
+            bv = v;
+            bi = r;
+        }
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        double ov = __shfl_xor(bv, off, WAVE);
+        int oi = __shfl_xor(bi, off, WAVE);
+        if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    return bi;
+}
+
+// linalg.rs:92-101 on one lane: x[i] = (b[i] - sum_{j>i} a[i][j] x[j]) / a[i][i].
+__device__ inline void back_substitute(const double* aug, int N, const double* diag, long dstride, double* x) {
+    for (int ii = N - 1; ii >= 0; --ii) {
+        double sum = 0.0;
+        int j = ii + 1;
+        for (; j + 4 <= N; j += 4) {
+            double a0 = aug[(long)j * N + ii], a1 = aug[(long)(j + 1) * N + ii];
+            double a2 = aug[(long)(j + 2) * N + ii], a3 = aug[(long)(j + 3) * N + ii];
+            double x0 = x[j], x1 = x[j + 1], x2 = x[j + 2], x3 = x[j + 3];
+            sum += a0 * x0;
+            sum += a1 * x1;
+            sum += a2 * x2;
+            sum += a3 * x3;
+        }
+        for (; j < N; ++j) sum += aug[(long)j * N + ii] * x[j];
+        x[ii] = (aug[(long)N * N + ii] - sum) / diag[(long)ii * dstride];
+    }
+}
+
+// Everything after the linear solve (linearized.rs:141-221): induced velocity,
+// viscous correction of the circulation, induced velocity again, residual and
+// outputs.  G holds the solution; R is [N] shared scratch.
+__device__ inline void lin_finish(const Inst& d, const Strip& s, double* G, double* R, int as_start) {
     const int N = d.N;
-    const int T = blockDim.x;
     const int t = threadIdx.x;
-    double* G = sh;            // [N] solution, then circulation
-    double* fac = G + N;       // [N] factors of the current pivot column
-    double* prow = fac + N;    // [N + 1] current pivot row
-    double* R = prow + N + 1;  // [N] residual terms
-    double* rv = R + N;        // [16] per-wave largest pivot magnitude
-    double* rix = rv + 16;     // [16] and its row
-    double* ctl = rix + 16;    // [0] pivot row, [1] singular row + 1, [2] residual
-    double* aug = LDS_MATRIX ? ctl + 4 : d.lin;
+    const bool act = t < N;
+    const int ic = act ? t : N - 1;
     const int vc = as_start ? 0 : d.vc_mode;
     const double vc_ratio = d.vc_ratio;
     const int viscous = as_start ? 1 : !d.lin_disable_viscous;
     const int ivc = as_start ? 1 : d.lin_ivc;
-
-    bool act = t < N;
-    int ic = act ? t : N - 1;
-    Strip s = load_strip(d, ic);
-    {
-        V3 u = s.felt + s.fixed;
-        u = u - project(u, s.rel_g);  // remove_span_velocity
-        V3 vdir = normalize(u);
-        V3 ndir = cross(vdir, normalize(s.rel_g));
-        double a = angle_of_attack(s, u);
-        double cll = foil_cl_linear(s.foil, a);
-        double dcl = s.foil.cl_initial_slope;
-        double ut = dot(u, vdir);
-        if (act) {  // row t of the equation system (linearized.rs:107-131)
-            const double* Fi = d.F + 3L * t;
-            for (int c = 0; c < N; ++c) {
-                V3 f = load3(Fi + 3L * N * c);
-                double v = -dcl * dot(f, ndir) / ut;
-                if (c == t) v += 2.0 / (s.c * ut);
-                aug[(long)c * N + t] = v;
-            }
-            aug[(long)N * N + t] = -cll;
-        }
-    }
-    if (t == 0) ctl[1] = 0.0;
-    __syncthreads();
-
-    const int nw = (T + WAVE - 1) / WAVE;
-    for (int k = 0; k < N; ++k) {
-        const double* colk = aug + (long)k * N;
-        double bv = -1.0;
-        int bi = N;
-        for (int r = k + t; r < N; r += T) {
-            double v = fabs(colk[r]);
-            if (v > bv) {  // a NaN never wins
-                bv = v;
-                bi = r;
-            }
-        }
-        for (int off = WAVE / 2; off > 0; off >>= 1) {  // larger wins, ties to the lower row
-            double ov = __shfl_xor(bv, off, WAVE);
-            int oi = __shfl_xor(bi, off, WAVE);
-            if (ov > bv || (ov == bv && oi < bi)) {
-                bv = ov;
-                bi = oi;
-            }
-        }
-        if ((t % WAVE) == 0) {
-            rv[t / WAVE] = bv;
-            rix[t / WAVE] = (double)bi;
-        }
-        __syncthreads();
-        if (t == 0) {
-            double best = rv[0];
-            int p = (int)rix[0];
-            for (int w = 1; w < nw; ++w) {
-                double ov = rv[w];
-                int oi = (int)rix[w];
-                if (ov > best || (ov == best && oi < p)) {
-                    best = ov;
-                    p = oi;
-                }
-            }
-            double akk = colk[k];
-            if (akk != akk || p >= N) p = k;
-            ctl[0] = (double)p;
-            if (fabs(colk[p]) < 1e-12) ctl[1] = (double)(k + 1);
-        }
-        __syncthreads();
-        if (ctl[1] != 0.0) break;
-        const int p = (int)ctl[0];
-        for (int j = k + t; j <= N; j += T) {
-            double* cj = aug + (long)j * N;
-            double a0 = cj[k], a1 = cj[p];
-            if (p != k) {
-                cj[k] = a1;
-                cj[p] = a0;
-            }
-            prow[j] = p != k ? a1 : a0;
-        }
-        __syncthreads();
-        for (int r = k + 1 + t; r < N; r += T) fac[r] = colk[r] / prow[k];
-        __syncthreads();
-        const int rows = N - k - 1;
-        if (rows > 0) {
-            const int per = T / rows;  // columns per pass; T >= N > rows
-            const int r = t % rows, c0 = t / rows;
-            if (c0 < per) {
-                const double f = fac[k + 1 + r];
-                for (int j = k + 1 + c0; j <= N; j += per) {
-                    double* e = aug + (long)j * N + (k + 1 + r);
-                    *e = *e - f * prow[j];
-                }
-            }
-        }
-        __syncthreads();
-    }
-    const bool singular = ctl[1] != 0.0;
-    if (t == 0) {
-        if (singular) {
-            for (int q = 0; q < N; ++q) G[q] = 0.0;
-            d.scal[4] = ctl[1];
-        } else {
-            for (int ii = N - 1; ii >= 0; --ii) {
-                double sum = 0.0;
-                for (int j = ii + 1; j < N; ++j) sum += aug[(long)j * N + ii] * G[j];
-                G[ii] = (aug[(long)N * N + ii] - sum) / aug[(long)ii * N + ii];
-            }
-        }
-    }
-    __syncthreads();
-
     double g = G[ic];
     const V3 ind1 = induced_at(d, s.fixed, G, ic);
     V3 u = corrected_velocity(vc, vc_ratio, s.felt, ind1);
     double a;
     if (viscous) {  // linearized.rs:156-197
         a = angle_of_attack(s, u);
-        double clf = foil_cl_pre_dropoff(s.foil, a);
-        double cll = foil_cl_linear(s.foil, a);
+        double clf = foil_cl_pre_dropoff(*s.foil, a);
+        double cll = foil_cl_linear(*s.foil, a);
         if (fabs(cll) > MIN_POSITIVE) g *= clf / cll;
         __syncthreads();
         if (act) G[t] = g;
@@ -1197,10 +1186,476 @@ __device__ inline void linearized_body(const Inst& d, double* sh, int as_start) 
         d.out[OutLayout::residual(N, d.W)] = sum / (double)N;
     }
 }
-template <bool LDS_MATRIX>
-__global__ __launch_bounds__(1024) void k_linearized(Inst d, int as_start) {
+
+// ---- one-workgroup form (small N) -------------------------------------------
+// T threads; Q = min(16, T / N) threads per row, thread (r, q) holds the entries
+// of row r in the columns j = q + c Q, c < MAXC (the right-hand side is column
+// N) in registers for the whole elimination.  Row exchanges are not carried
+// out: every wave keeps the logical position of every row (lanes hold rows l,
+// l + 64, ...), and the rows already used as pivots.  Per pivot k:
+//   - every wave finds the pivot row from column k (published in shared memory
+//     by its owners): the largest magnitude, ties to the lower logical position,
+//     NaN never chosen unless the row at position k holds NaN (then that row);
+//   - the pivot row's threads publish their entries (prow);
+//   - every other row still in play subtracts f * prow[j], f = a[r][k] / pivot,
+//     in blocks of 8 entries; blocks entirely left of column k + 1 are skipped,
+//     and the other entries left of it (never read again) take harmless updates;
+//     the owners of column k + 1 publish the new column.
+// Two barriers per pivot.  The upper triangle is then written row-major in
+// logical order for the back substitution.
+constexpr int LINR_MAXS = 3;  // rows per lane in the pivot search (N <= 192)
+
+template <int T>
+SB_HD inline int linr_q(int N) { return T / N < 16 ? T / N : 16; }
+template <int T, int MAXC>
+SB_HD inline bool linr_fits(int N) {
+    if (N < 1 || N > 64 * LINR_MAXS) return false;
+    const int q = linr_q<T>(N);
+    return q >= 1 && (N + 1 + q - 1) / q <= MAXC;
+}
+// Shared-memory doubles: U [N][N + 1] (+8 pad), G (+8 pad), R, two column
+// buffers, position map, control, prow [max(N + 1, Q MAXC)].
+template <int T, int MAXC>
+SB_HD inline long linr_shared_doubles(int N) {
+    const long q = linr_q<T>(N);
+    const long pr = q * MAXC > N + 1 ? q * MAXC : N + 1;
+    return (long)N * (N + 1) + 8 + (N + 8) + N + 2L * N + N + 8 + pr;
+}
+
+// linalg.rs:92-101 on one lane over a row-major upper triangle.  U and x are
+// padded by 8 doubles so that every batch of 8 terms loads unconditionally;
+// terms past the row end are replaced by +0.0, which leaves the sum unchanged
+// (it starts at +0.0 and cannot become -0.0).  The next batch is loaded before
+// the current one is summed, in ascending j.
+__device__ inline void back_substitute_rows(const double* U, int N, double* x) {
+    const int ld = N + 1;
+    for (int ii = N - 1; ii >= 0; --ii) {
+        const double* row = U + (long)ii * ld;
+        double sum = 0.0;
+        int j = ii + 1;
+        double ua[8], xa[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            ua[u] = row[j + u];
+            xa[u] = x[j + u];
+        }
+        while (j < N) {
+            const int jn = j + 8;
+            double ub[8], xb[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ub[u] = row[jn + u];
+                xb[u] = x[jn + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double p = ua[u] * xa[u];
+                sum += (j + u < N) ? p : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ua[u] = ub[u];
+                xa[u] = xb[u];
+            }
+            j = jn;
+        }
+        x[ii] = (row[N] - sum) / row[ii];
+    }
+}
+
+// One wave's argmax of (magnitude, key): larger magnitude wins, then the smaller
+// key.  Four DPP exchange levels inside each row of 16 lanes, then the four row
+// results by readlane; every lane returns the same pair.
+template <int CTRL>
+__device__ inline void argmax_level(double& bm, int& bkey) {
+    const int lo = __double2loint(bm), hi = __double2hiint(bm);
+    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    const int ok = __builtin_amdgcn_update_dpp(bkey, bkey, CTRL, 0xf, 0xf, false);
+    const double om = __hiloint2double(ohi, olo);
+    if (om > bm || (om == bm && ok < bkey)) {
+        bm = om;
+        bkey = ok;
+    }
+}
+__device__ inline void wave_argmax(double& bm, int& bkey) {
+    argmax_level<0xB1>(bm, bkey);   // quad_perm [1,0,3,2]
+    argmax_level<0x4E>(bm, bkey);   // quad_perm [2,3,0,1]
+    argmax_level<0x141>(bm, bkey);  // row_half_mirror
+    argmax_level<0x140>(bm, bkey);  // row_mirror
+    const int lo = __double2loint(bm), hi = __double2hiint(bm);
+    double m = bm;
+    int key = bkey;
+#pragma unroll
+    for (int rw = 0; rw < 4; ++rw) {
+        const double om = __hiloint2double(__builtin_amdgcn_readlane(hi, 16 * rw), __builtin_amdgcn_readlane(lo, 16 * rw));
+        const int ok = __builtin_amdgcn_readlane(bkey, 16 * rw);
+        if (rw == 0 || om > m || (om == m && ok < key)) {
+            m = om;
+            key = ok;
+        }
+    }
+    bm = m;
+    bkey = key;
+}
+
+template <int T, int MAXC>
+__device__ inline void linearized_reg_body(const Inst& d, double* sh, int as_start) {
+    static_assert(MAXC % 8 == 0, "MAXC in blocks of 8");
+    const int N = d.N;
+    const int t = threadIdx.x;
+    const int lane = t % WAVE;
+    const int Q = linr_q<T>(N);
+    const int r = t / Q, q = t - r * Q;
+    const bool own = r < N;
+    double* U = sh;                           // [N][N + 1] row-major, logical rows (+8 pad)
+    double* G = U + (long)N * (N + 1) + 8;    // [N] (+8 pad)
+    double* R = G + N + 8;                    // [N]
+    double* cbuf = R + N;                     // [2][N]
+    double* posd = cbuf + 2 * N;              // [N] logical position of each row
+    double* ctl = posd + N;                   // [8]
+    double* prow = ctl + 8;                   // [max(N + 1, Q MAXC)]
+
+    double a[MAXC];
+    {
+        const LinRow qr = lin_row(load_strip(d, own ? r : N - 1));
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            const int j = q + c * Q;
+            double v = 0.0;
+            if (own && j < N) v = lin_entry(d, qr, r, j);
+            else if (own && j == N) v = -qr.cll;
+            a[c] = v;
+        }
+    }
+    if (own && q == 0) cbuf[r] = a[0];
+    int wpos[LINR_MAXS];
+    bool wdone[LINR_MAXS];
+#pragma unroll
+    for (int sl = 0; sl < LINR_MAXS; ++sl) {
+        wpos[sl] = lane + WAVE * sl;
+        wdone[sl] = false;
+    }
+    bool done = false;
+    int singular = 0;
+    __syncthreads();
+    for (int k = 0; k < N; ++k) {
+        const double* cb = cbuf + (k & 1) * N;
+        double bm = -1.0;
+        int bkey = 0x7fffffff;  // logical position * 1024 + row
+        int rk = -1;            // row at logical position k
+        bool nan_k = false;
+#pragma unroll
+        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+            const int row = lane + WAVE * sl;
+            if (row < N && !wdone[sl]) {
+                const double v = cb[row];
+                if (wpos[sl] == k) {
+                    rk = row;
+                    nan_k = v != v;
+                }
+                const double m = fabs(v);
+                const int key = wpos[sl] * 1024 + row;
+                if (m > bm || (m == bm && key < bkey)) {  // NaN never wins
+                    bm = m;
+                    bkey = key;
+                }
+            }
+        }
+        wave_argmax(bm, bkey);
+        {
+            const unsigned long long hit = __ballot(rk >= 0);
+            const int src = (int)__builtin_ctzll(hit);
+            rk = __builtin_amdgcn_readlane(rk, src);
+            nan_k = __builtin_amdgcn_readlane((int)nan_k, src) != 0;
+        }
+        int pr = bkey & 1023, ppos = bkey >> 10;
+        if (nan_k || bkey == 0x7fffffff) {
+            pr = rk;
+            ppos = k;
+        }
+        const double piv = cb[pr];
+        if (fabs(piv) < 1e-12) {
+            singular = k + 1;
+            break;
+        }
+#pragma unroll
+        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+            const int row = lane + WAVE * sl;
+            if (row == pr) {
+                wpos[sl] = k;
+                wdone[sl] = true;
+            } else if (row == rk) {
+                wpos[sl] = ppos;
+            }
+        }
+        if (own && r == pr) {
+            done = true;
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) prow[q + c * Q] = a[c];
+        }
+        __syncthreads();
+        if (own && !done) {
+            const double f = cb[r] / piv;
+#pragma unroll
+            for (int b8 = 0; b8 < MAXC / 8; ++b8) {
+                if (q + (8 * b8 + 7) * Q > k && q + 8 * b8 * Q <= N) {
+                    double w[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) w[u] = prow[q + (8 * b8 + u) * Q];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[8 * b8 + u] = a[8 * b8 + u] - f * w[u];
+                }
+            }
+            const int k1 = k + 1;
+            if (k1 < N && q == k1 % Q) {
+                const int c1 = k1 / Q;
+                double v = 0.0;
+#pragma unroll
+                for (int b8 = 0; b8 < MAXC / 8; ++b8) {
+                    if ((c1 >> 3) == b8) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if ((c1 & 7) == u) v = a[8 * b8 + u];
+                    }
+                }
+                cbuf[(k1 & 1) * N + r] = v;
+            }
+        }
+        __syncthreads();
+    }
+    if (singular) {
+        if (t == 0) {
+            for (int qq = 0; qq < N; ++qq) G[qq] = 0.0;
+            d.scal[4] = (double)singular;
+        }
+    } else {
+        if (t < WAVE) {
+#pragma unroll
+            for (int sl = 0; sl < LINR_MAXS; ++sl) {
+                const int row = lane + WAVE * sl;
+                if (row < N) posd[row] = (double)wpos[sl];
+            }
+        }
+        __syncthreads();
+        if (own) {
+            double* urow = U + (long)posd[r] * (N + 1);
+#pragma unroll
+            for (int c = 0; c < MAXC; ++c) {
+                const int j = q + c * Q;
+                if (j <= N) urow[j] = a[c];
+            }
+        }
+        __syncthreads();
+        if (t == 0) back_substitute_rows(U, N, G);
+    }
+    __syncthreads();
+    lin_finish(d, load_strip(d, t < N ? t : N - 1), G, R, as_start);
+}
+template <int T, int MAXC>
+__global__ __launch_bounds__(T) void k_linearized(Inst d, int as_start) {
     extern __shared__ double sh[];
-    linearized_body<LDS_MATRIX>(d, sh, as_start);
+    linearized_reg_body<T, MAXC>(d, sh, as_start);
+}
+
+// ---- blocked form (matrix in global memory, d.lin) -------------------------
+// d.lin: [N + 1][N] matrix, then [N][8] row data (LinRow), then [N] ints: the
+// source position of every row after the current panel's exchanges.
+__device__ inline double* lin_rows(const Inst& d) { return d.lin + (long)d.N * (d.N + 1); }
+__device__ inline int* lin_perm(const Inst& d) { return (int*)(lin_rows(d) + 8L * d.N); }
+
+__global__ void k_lin_rows(Inst d) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= d.N) return;
+    LinRow q = lin_row(load_strip(d, i));
+    double* o = lin_rows(d) + 8L * i;
+    o[0] = q.ndir.x;
+    o[1] = q.ndir.y;
+    o[2] = q.ndir.z;
+    o[3] = q.ut;
+    o[4] = q.dcl;
+    o[5] = q.cll;
+    o[6] = q.c;
+}
+
+__global__ void k_lin_assemble(Inst d) {
+    const long N = d.N;
+    long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N * (N + 1)) return;
+    int c = (int)(idx / N), r = (int)(idx - (long)c * N);
+    const double* o = lin_rows(d) + 8L * r;
+    LinRow q;
+    q.ndir = V3{o[0], o[1], o[2]};
+    q.ut = o[3];
+    q.dcl = o[4];
+    q.cll = o[5];
+    q.c = o[6];
+    d.lin[idx] = c < N ? lin_entry(d, q, r, c) : -q.cll;
+}
+
+// Panel factorization of columns [k0, k0 + b): one workgroup, the panel's rows
+// [k0, N) staged in shared memory.  Row exchanges cover the whole panel width;
+// the factors replace the entries below the diagonal; the source position of
+// every row after the panel's exchanges goes to lin_perm for the trailing update.
+__global__ __launch_bounds__(1024) void k_lin_panel(Inst d, int k0, int b) {
+    extern __shared__ double sh[];
+    if (d.scal[4] != 0.0) return;
+    const int N = d.N, T = blockDim.x, t = threadIdx.x;
+    const int nr = N - k0;
+    double* P = sh;                   // [b][nr]
+    double* rv = P + (long)b * nr;    // [16]
+    double* rix = rv + 16;            // [16]
+    double* ctl = rix + 16;           // [4]
+    int* src = (int*)(ctl + 4);       // [nr]
+    for (long idx = t; idx < (long)b * nr; idx += T) {
+        int c = (int)(idx / nr), rr = (int)(idx - (long)c * nr);
+        P[idx] = d.lin[(long)(k0 + c) * N + k0 + rr];
+    }
+    for (int q = t; q < nr; q += T) src[q] = q;
+    if (t == 0) ctl[1] = 0.0;
+    __syncthreads();
+    const int nw = (T + WAVE - 1) / WAVE;
+    for (int kk = 0; kk < b; ++kk) {
+        double* col = P + (long)kk * nr;
+        double bv = -1.0;
+        int bi = nr;
+        for (int r = kk + t; r < nr; r += T) {
+            double v = fabs(col[r]);
+            if (v > bv) {
+                bv = v;
+                bi = r;
+            }
+        }
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            double ov = __shfl_xor(bv, off, WAVE);
+            int oi = __shfl_xor(bi, off, WAVE);
+            if (ov > bv || (ov == bv && oi < bi)) {
+                bv = ov;
+                bi = oi;
+            }
+        }
+        if ((t % WAVE) == 0) {
+            rv[t / WAVE] = bv;
+            rix[t / WAVE] = (double)bi;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double best = rv[0];
+            int p = (int)rix[0];
+            for (int w = 1; w < nw; ++w) {
+                double ov = rv[w];
+                int oi = (int)rix[w];
+                if (ov > best || (ov == best && oi < p)) {
+                    best = ov;
+                    p = oi;
+                }
+            }
+            double akk = col[kk];
+            if (akk != akk || p >= nr) p = kk;
+            ctl[0] = (double)p;
+            if (fabs(col[p]) < 1e-12) ctl[1] = (double)(k0 + kk + 1);
+            if (p != kk) {
+                int tmp = src[kk];
+                src[kk] = src[p];
+                src[p] = tmp;
+            }
+        }
+        __syncthreads();
+        if (ctl[1] != 0.0) {
+            if (t == 0) d.scal[4] = ctl[1];
+            return;
+        }
+        const int p = (int)ctl[0];
+        if (p != kk) {
+            for (int c = t; c < b; c += T) {
+                double a0 = P[(long)c * nr + kk];
+                P[(long)c * nr + kk] = P[(long)c * nr + p];
+                P[(long)c * nr + p] = a0;
+            }
+        }
+        __syncthreads();
+        const double piv = col[kk];
+        for (int r = kk + 1 + t; r < nr; r += T) {
+            const double f = col[r] / piv;
+            col[r] = f;
+            for (int c = kk + 1; c < b; ++c) P[(long)c * nr + r] = P[(long)c * nr + r] - f * P[(long)c * nr + kk];
+        }
+        __syncthreads();
+    }
+    for (long idx = t; idx < (long)b * nr; idx += T) {
+        int c = (int)(idx / nr), rr = (int)(idx - (long)c * nr);
+        d.lin[(long)(k0 + c) * N + k0 + rr] = P[idx];
+    }
+    int* gp = lin_perm(d);
+    for (int q = t; q < nr; q += T) gp[q] = src[q];
+}
+
+// Trailing update after the panel [k0, k0 + b): one workgroup per column j >=
+// k0 + b (the right-hand side included).  The panel's exchanges are applied as
+// one gather; the panel rows' entries take their updates from the earlier
+// pivots in order (these are the pivot rows' values), then every lower entry
+// takes the b updates a - l*u in pivot order.
+constexpr int LIN_TRAIL_THREADS = 256;
+__global__ __launch_bounds__(LIN_TRAIL_THREADS) void k_lin_trailing(Inst d, int k0, int b) {
+    __shared__ double u[16];
+    if (d.scal[4] != 0.0) return;
+    const int N = d.N, t = threadIdx.x;
+    const int nr = N - k0;
+    const int j = k0 + b + blockIdx.x;
+    if (j > N) return;
+    double* cj = d.lin + (long)j * N + k0;  // rows from k0
+    const int* src = lin_perm(d);
+    constexpr int PER = 1024 / LIN_TRAIL_THREADS;
+    double v[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        int q = t + m * LIN_TRAIL_THREADS;
+        v[m] = (q < nr && src[q] != q) ? cj[src[q]] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        int q = t + m * LIN_TRAIL_THREADS;
+        if (q < nr && src[q] != q) cj[q] = v[m];
+    }
+    __syncthreads();
+    if (t == 0) {
+        double uu[16];
+        for (int kk = 0; kk < b; ++kk) uu[kk] = cj[kk];
+        for (int kk = 1; kk < b; ++kk) {
+            double a = uu[kk];
+            for (int m = 0; m < kk; ++m) a = a - d.lin[(long)(k0 + m) * N + k0 + kk] * uu[m];
+            uu[kk] = a;
+        }
+        for (int kk = 0; kk < b; ++kk) {
+            u[kk] = uu[kk];
+            cj[kk] = uu[kk];
+        }
+    }
+    __syncthreads();
+    for (int q = b + t; q < nr; q += LIN_TRAIL_THREADS) {
+        double a = cj[q];
+        for (int kk = 0; kk < b; ++kk) a = a - d.lin[(long)(k0 + kk) * N + k0 + q] * u[kk];
+        cj[q] = a;
+    }
+}
+
+// Back substitution and the rest of the solve after the blocked factorization.
+__global__ __launch_bounds__(1024) void k_lin_finish(Inst d, int as_start) {
+    extern __shared__ double sh[];
+    const int N = d.N, t = threadIdx.x;
+    double* G = sh;
+    double* R = G + N;
+    const Strip s = load_strip(d, t < N ? t : N - 1);
+    if (t == 0) {
+        if (d.scal[4] != 0.0) {
+            for (int q = 0; q < N; ++q) G[q] = 0.0;
+        } else {
+            back_substitute(d.lin, N, d.lin, (long)N + 1, G);
+        }
+    }
+    __syncthreads();
+    lin_finish(d, s, G, R, as_start);
 }
 
 // ---------------------------------------------------------------------------
@@ -1388,7 +1843,8 @@ __global__ void kb_velocity_copy(const Inst* I, const long* off, int n) {
     if (q < 3L * d.P * (d.M + 1)) d.vel[q] = d.fs[3L * d.N + q];
 }
 
-__global__ __launch_bounds__(1024) void kb_solve(const Inst* I) {
+template <int MAXT>
+__global__ __launch_bounds__(MAXT) void kb_solve(const Inst* I) {
     extern __shared__ double sh[];
     const Inst d = I[blockIdx.x];
     solve_body(d, sh);
@@ -1766,30 +2222,67 @@ static long lds_limit_bytes() {
     return cached;
 }
 
-bool linearized_in_lds(int N) {
-    long bytes = (long)sizeof(double) * (linearized_aux(N) + (long)N * (N + 1));
-    return bytes <= lds_limit_bytes();
+// One-workgroup variants: 256 threads x 72 entries (N <= 128), 512 x 48.
+static int linr_variant(int N) {
+    const long lim = lds_limit_bytes();
+    if (linr_fits<256, 72>(N) && N <= 128 && (long)sizeof(double) * linr_shared_doubles<256, 72>(N) <= lim) return 1;
+    if (linr_fits<512, 48>(N) && (long)sizeof(double) * linr_shared_doubles<512, 48>(N) <= lim) return 2;
+    return 0;
+}
+
+bool linearized_in_lds(int N) { return linr_variant(N) != 0; }
+
+static hipError_t allow_shared(const void* fn, size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+// Panel width of the blocked form: up to 16 columns of N rows in shared memory.
+static int linearized_panel_width(int N) {
+    long room = lds_limit_bytes() - (long)sizeof(double) * 40 - (long)sizeof(int) * N;
+    long b = room / ((long)sizeof(double) * N);
+    return (int)(b < 16 ? b : 16);
 }
 
 hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
-    int threads = ((d.N + WAVE - 1) / WAVE) * WAVE;
-    if (threads < 256) threads = 256;
-    if (threads > 1024 || d.N < 1) return hipErrorInvalidValue;
-    if (linearized_in_lds(d.N)) {
-        size_t shm = sizeof(double) * (linearized_aux(d.N) + (size_t)d.N * (d.N + 1));
-        if (shm > 65536) {
-            hipError_t e = hipFuncSetAttribute((const void*)k_linearized<true>,
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
-            if (e != hipSuccess) return e;
-        }
-        hipLaunchKernelGGL(k_linearized<true>, dim3(1), dim3(threads), shm, st, d, as_start);
-    } else {
-        if (!d.lin) return hipErrorInvalidValue;
-        size_t shm = sizeof(double) * linearized_aux(d.N);
-        hipLaunchKernelGGL(k_linearized<false>, dim3(1), dim3(threads), shm, st, d, as_start);
+    const int N = d.N;
+    if (N < 1 || N > 1024) return hipErrorInvalidValue;
+    const int v = linr_variant(N);
+    if (v == 1) {
+        size_t shm = sizeof(double) * (size_t)linr_shared_doubles<256, 72>(N);
+        hipError_t e = allow_shared((const void*)k_linearized<256, 72>, shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_linearized<256, 72>), dim3(1), dim3(256), shm, st, d, as_start);
+        return hipGetLastError();
     }
+    if (v == 2) {
+        size_t shm = sizeof(double) * (size_t)linr_shared_doubles<512, 48>(N);
+        hipError_t e = allow_shared((const void*)k_linearized<512, 48>, shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_linearized<512, 48>), dim3(1), dim3(512), shm, st, d, as_start);
+        return hipGetLastError();
+    }
+    if (!d.lin) return hipErrorInvalidValue;
+    const int b = linearized_panel_width(N);
+    if (b < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_lin_rows, dim3(blocks_for(N, 256)), dim3(256), 0, st, d);
+    hipLaunchKernelGGL(k_lin_assemble, dim3(blocks_for((long)N * (N + 1), 256)), dim3(256), 0, st, d);
+    for (int k0 = 0; k0 < N; k0 += b) {
+        const int bb = N - k0 < b ? N - k0 : b;
+        const int nr = N - k0;
+        size_t shm = sizeof(double) * ((size_t)bb * nr + 36) + sizeof(int) * (size_t)nr;
+        hipError_t e = allow_shared((const void*)k_lin_panel, shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_lin_panel, dim3(1), dim3(1024), shm, st, d, k0, bb);
+        const unsigned cols = (unsigned)(N + 1 - k0 - bb);
+        hipLaunchKernelGGL(k_lin_trailing, dim3(cols), dim3(LIN_TRAIL_THREADS), 0, st, d, k0, bb);
+    }
+    hipLaunchKernelGGL(k_lin_finish, dim3(1), dim3(1024), sizeof(double) * 2 * (size_t)N, st, d, as_start);
     return hipGetLastError();
 }
+
+// Doubles of d.lin for the blocked form: matrix, row data, exchange sources.
+long linearized_global_doubles(int N) { return (long)N * (N + 1) + 8L * N + N; }
 
 hipError_t launch_factors(const Inst& d, hipStream_t st) {
     long n = (long)d.N * d.N;
@@ -1797,10 +2290,16 @@ hipError_t launch_factors(const Inst& d, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Shared memory of the one-workgroup fixed-point solve (solve_body).
+size_t solve_shared_bytes(int N, int W) {
+    return sizeof(double) * (2 * (size_t)N + 16 + 4) + sizeof(FoilParams) * (size_t)W;
+}
+
 hipError_t launch_solve(const Inst& d, hipStream_t st) {
     int threads = ((d.N + WAVE - 1) / WAVE) * WAVE;
-    size_t shm = sizeof(double) * (2 * (size_t)d.N + 16 + 4);
-    hipLaunchKernelGGL(k_solve, dim3(1), dim3(threads), shm, st, d);
+    size_t shm = solve_shared_bytes(d.N, d.W);
+    if (threads <= 512) hipLaunchKernelGGL(k_solve<512>, dim3(1), dim3(threads), shm, st, d);
+    else hipLaunchKernelGGL(k_solve<1024>, dim3(1), dim3(threads), shm, st, d);
     return hipGetLastError();
 }
 
@@ -1984,7 +2483,8 @@ hipError_t launch_batch_factors(const BatchDev& b, hipStream_t st) {
 }
 
 hipError_t launch_batch_solve(const BatchDev& b, hipStream_t st) {
-    hipLaunchKernelGGL(kb_solve, dim3(b.n), dim3(b.max_threads), b.solve_shm, st, b.inst);
+    if (b.max_threads <= 512) hipLaunchKernelGGL(kb_solve<512>, dim3(b.n), dim3(b.max_threads), b.solve_shm, st, b.inst);
+    else hipLaunchKernelGGL(kb_solve<1024>, dim3(b.n), dim3(b.max_threads), b.solve_shm, st, b.inst);
     return hipGetLastError();
 }
 
