@@ -446,8 +446,57 @@ def quasi_steady(steps=50):
                     "cpu_ms_per_step": 1e3 * t_cpu, "cpu": "oracle restatement, 1 thread",
                     "gpu_vs_cpu": t_cpu / t_gpu})
         del g, o
-    return {"cases": out, "step_times": "host-to-host do_step (freestream upload and result download included); "
-                                        "device time is the sum of the step's kernel intervals"}
+    # batched polar: many quasi-steady configurations, one workgroup per instance
+    n_inst, seed = 1024, 4
+    rng = np.random.default_rng(seed)
+    setups, pcases = zip(*[S.quasi_steady_polar_instance(rng) for _ in range(n_inst)])
+    batch = sb.Batch(list(setups))
+    fs_all = np.concatenate([S.uniform_freestream(inst.nr_freestream_points(), c["velocity"])
+                             for inst, c in zip(batch.instances, pcases)])
+    dts = np.array([c["time_step"] for c in pcases])
+    for k in range(2):
+        batch.do_step(times=k * dts, time_steps=dts, freestream_velocity=fs_all)
+    reps = 5
+    dev = []
+    t0 = time.perf_counter()
+    for k in range(reps):
+        batch.do_step(times=k * dts, time_steps=dts, freestream_velocity=fs_all)
+        dev.append(batch.last_step_ms())
+    t_b = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for k in range(reps):
+        batch.do_step(times=k * dts, time_steps=dts, freestream_velocity=fs_all, want_result=False)
+    t_nr = (time.perf_counter() - t0) / reps
+    sample = 64
+    alone = [sb.Simulation(js) for js in setups[:sample]]
+    parts = [S.uniform_freestream(a.nr_freestream_points(), c["velocity"]) for a, c in zip(alone, pcases)]
+    for a, f, dt in zip(alone, parts, dts):
+        a.do_step(time=0.0, time_step=dt, freestream_velocity=f)
+    t0 = time.perf_counter()
+    for a, f, dt in zip(alone, parts, dts):
+        a.do_step(time=0.0, time_step=dt, freestream_velocity=f)
+    t_plain = (time.perf_counter() - t0) / sample
+    orcs = [oracle_py.Simulation(js, "glibc") for js in setups[:sample]]
+    t0 = time.perf_counter()
+    for o, f, dt in zip(orcs, parts, dts):
+        o.do_step(time=0.0, time_step=dt, freestream_velocity=f)
+    t_cpu = (time.perf_counter() - t0) / sample
+    n_tot = int(sum(inst.nr_ctrl_points for inst in batch.instances))
+    polar = {"workload": (f"{n_inst} quasi-steady polar instances (seed {seed}): 1-4 sails, 8-32 strips per sail, "
+                          "chords 5-15 m, AR 2-6, AWA 20-180 deg, AWS 6-18 m/s, default settings (horseshoe wake, "
+                          "Linearized solve)"),
+             "n_instances": n_inst, "n_ctrl_points_total": n_tot,
+             "batched_ms_per_step": 1e3 * t_b, "batched_device_ms_per_step": float(np.mean(dev)),
+             "batched_ms_per_step_results_left_on_device": 1e3 * t_nr,
+             "batched_instance_steps_per_s": n_inst / t_b,
+             "plain_gpu_ms_per_instance_step": 1e3 * t_plain, "plain_gpu_instance_steps_per_s": 1.0 / t_plain,
+             "cpu_ms_per_instance_step": 1e3 * t_cpu, "cpu_instance_steps_per_s": 1.0 / t_cpu,
+             "cpu": f"oracle restatement, 1 thread, first {sample} instances",
+             "batched_vs_plain_gpu": t_plain * n_inst / t_b, "batched_vs_cpu_1_thread": t_cpu * n_inst / t_b}
+    del batch, alone, orcs
+    return {"cases": out, "batched_polar": polar,
+            "step_times": "host-to-host do_step (freestream upload and result download included); "
+                          "device time is the sum of the step's kernel intervals"}
 
 
 def main():

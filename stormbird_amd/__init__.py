@@ -18,7 +18,7 @@ import os
 
 import numpy as np
 
-from .results import StepBuffers, StepResultC, SimulationResult  # noqa: F401
+from .results import FIELDS, BatchResults, StepBuffers, StepResultC, SimulationResult  # noqa: F401
 
 __all__ = ["Simulation", "SimulationResult", "MODE_EXACT", "MODE_FAST", "library_path", "lib",
            "wake_induced_velocities"]
@@ -394,8 +394,24 @@ class Batch:
         if L.sb_batch_freestream_offsets(self._h, offs, n + 1) != n + 1:
             _err("sb_batch_freestream_offsets", -1)
         self.freestream_offsets = np.array(list(offs), dtype=np.int64)
+        # batch-wide output arrays; every instance's buffers are views into them
+        ns = [s.nr_ctrl_points for s in self.instances]
+        ws = [s.nr_wings for s in self.instances]
+        self._strip_off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+        self._wing_off = np.concatenate([[0], np.cumsum(ws)]).astype(np.int64)
+        self._out = {}
+        for name, per_strip, tail in FIELDS:
+            per = int(np.prod(tail)) if tail else 1
+            total = int(self._strip_off[-1] if per_strip else self._wing_off[-1])
+            self._out[name] = np.zeros(total * per)
         self._res = (StepResultC * n)()
         for i, sim in enumerate(self.instances):
+            storage = {}
+            for name, per_strip, tail in FIELDS:
+                per = int(np.prod(tail)) if tail else 1
+                off = self._strip_off if per_strip else self._wing_off
+                storage[name] = self._out[name][off[i] * per:off[i + 1] * per]
+            sim._buf = StepBuffers(ns[i], ws[i], storage)
             self._res[i] = sim._buf.c
 
     def __del__(self):
@@ -417,13 +433,14 @@ class Batch:
         ts = _f64(np.broadcast_to(np.asarray(0.0 if times is None else times, dtype=np.float64), (self.n,)).copy())
         return ts, dts
 
+    # sb_step_result as a numpy record: time, iterations, residual, then eight pointers
+    _RES_DTYPE = np.dtype([("time", "<f8"), ("iterations", "<i8"), ("residual", "<f8")] +
+                          [(f, "<u8") for f, _ in StepResultC._fields_[3:]])
+
     def _collect(self):
-        out = []
-        for i, s in enumerate(self.instances):
-            c = self._res[i]
-            s._buf.c.time, s._buf.c.iterations, s._buf.c.residual = c.time, c.iterations, c.residual
-            out.append(s._buf.snapshot())
-        return out
+        arr = np.frombuffer(self._res, dtype=self._RES_DTYPE)
+        return BatchResults(arr["time"].copy(), arr["iterations"].copy(), arr["residual"].copy(),
+                            {k: v.copy() for k, v in self._out.items()}, self._strip_off, self._wing_off)
 
     def do_step(self, *, times, time_steps, freestream_velocity, want_result: bool = True):
         fs = _f64(freestream_velocity, (-1, 3))

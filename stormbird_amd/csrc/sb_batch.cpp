@@ -14,8 +14,14 @@
 // Every instance computes exactly what its own single-instance step computes,
 // bit for bit: the kernel bodies are shared, and the batched pair sums keep
 // each target's ascending panel order.
+//
+// A batch of quasi-steady instances (no shed wake) runs its own short step:
+// prepare, horseshoe factors, one-workgroup Linearized solve (and/or the
+// fixed-point solve), forces.
 #include <algorithm>
+#include <exception>
 #include <numeric>
+#include <thread>
 
 #include "sb_host.h"
 
@@ -30,6 +36,34 @@ T* halloc(size_t n) {
 }
 
 constexpr int kStages = 4;  // before the wake step, after it, after the strength shift, after the solve
+
+// fn(i) for i in [0, n) over up to 16 host threads (contiguous chunks); the
+// first exception is rethrown here.  Used for per-instance host work, which
+// touches only that instance.
+template <class F>
+void parallel_for(int n, F&& fn) {
+    unsigned hw = std::thread::hardware_concurrency();
+    int nt = (int)std::min<unsigned>(hw ? hw : 1, 16u);
+    nt = std::min(nt, std::max(1, n / 32));
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    std::vector<std::exception_ptr> err(nt);
+    for (int w = 0; w < nt; ++w) {
+        pool.emplace_back([&, w] {
+            try {
+                for (int i = (int)((long)n * w / nt); i < (int)((long)n * (w + 1) / nt); ++i) fn(i);
+            } catch (...) {
+                err[w] = std::current_exception();
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
+}
 
 }  // namespace
 
@@ -63,16 +97,21 @@ struct sb_batch {
     float *d_feat = nullptr, *d_scale = nullptr, *d_pred = nullptr;
     unsigned *d_fallbacks = nullptr, *h_fallbacks = nullptr;
     bool last_applied = false;
+    // quasi-steady batches
+    bool quasi = false;
+    int lin_variant = 0;
+    size_t lin_shm = 0;
+    double *d_singular = nullptr, *h_singular = nullptr;  // [n]
 
     ~sb_batch() {
         (void)hipSetDevice(device);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         void* dev[] = {d_geo, d_fs, d_out, d_inst, d_off, d_jobs, d_counters, d_strip_off, d_feat, d_scale, d_pred,
-                       d_fallbacks};
+                       d_fallbacks, d_singular};
         for (void* p : dev)
             if (p) (void)hipFree(p);
-        void* host[] = {h_geo, h_out, h_inst, h_jobs, h_fallbacks};
+        void* host[] = {h_geo, h_out, h_inst, h_jobs, h_fallbacks, h_singular};
         for (void* p : host)
             if (p) (void)hipHostFree(p);
         for (auto& e : ev)
@@ -92,14 +131,31 @@ struct sb_batch {
             auto s = std::make_unique<sb_sim>();
             s->create(jsons[i], dev, md);
             if (s->N > 1024) throw std::invalid_argument("batched instances support at most 1024 strips");
-            if (s->setup.quasi || s->setup.solver.kind != 0 || s->setup.solver.start_linearized)
+            if (!s->setup.quasi && (s->setup.solver.kind != 0 || s->setup.solver.start_linearized))
                 throw std::invalid_argument(
-                    "batched instances run dynamic SimpleIterative simulations (no quasi-steady wake, no Linearized "
-                    "solver, no start_with_linearized_solution)");
+                    "batched dynamic instances use the SimpleIterative solver without "
+                    "start_with_linearized_solution");
+            if (i > 0 && s->setup.quasi != sims[0]->setup.quasi)
+                throw std::invalid_argument("a batch holds either quasi-steady or dynamic instances, not both");
             sims.push_back(std::move(s));
         }
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        quasi = sims[0]->setup.quasi;
+        bool any_lin = false;
+        for (auto& s : sims) any_lin = any_lin || s->setup.solver.kind == 1 || s->setup.solver.start_linearized;
+        if (any_lin) {
+            std::vector<int> ns(n);
+            int mx = 1;
+            for (int i = 0; i < n; ++i) mx = std::max(mx, ns[i] = sims[i]->N);
+            lin_variant = batch_linearized_variant(mx);
+            lin_shm = lin_variant ? batch_linearized_shared(ns.data(), n, lin_variant) : 0;
+            if (!lin_variant || !lin_shm)
+                throw std::invalid_argument("batched Linearized solves need every instance's system in one "
+                                            "workgroup's shared memory (" + std::to_string(mx) + " strips is too many)");
+        }
+        d_singular = dalloc<double>(n);
+        h_singular = halloc<double>(n);
         geo_off.resize(n + 1);
         fs_off.resize(n + 1);
         out_off.resize(n + 1);
@@ -221,9 +277,13 @@ struct sb_batch {
         }
         const bool first = !first_done;
         // host: line-force-model update and packed geometry of every instance
-        for (int i = 0; i < n; ++i) {
+        parallel_for(n, [&](int i) {
             sims[i]->setup.lfm.update_global();
             sims[i]->pack_geometry(h_geo + geo_off[i]);
+        });
+        if (quasi) {
+            step_quasi(first, times, dts, fs, fs_on_device, results);
+            return;
         }
         // stage descriptors (the host state transitions of sb_sim::step)
         for (int i = 0; i < n; ++i) {
@@ -298,14 +358,78 @@ struct sb_batch {
         (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
         last_ms = ms;
         if (first) first_done = true;
-        for (int i = 0; i < n; ++i) {
+        parallel_for(n, [&](int i) {
             sb_sim& s = *sims[i];
             s.first_done = true;
             s.steps_completed += 1;
             std::copy(h_out + out_off[i], h_out + out_off[i + 1], s.h_out.begin());
             if (results) s.write_result(times ? times[i] : 0.0, results + i);
-            if (s.setup.wake.write_files) write_wake_step_file(s, times ? times[i] : 0.0, dts[i]);
+        });
+        for (int i = 0; i < n; ++i)
+            if (sims[i]->setup.wake.write_files) write_wake_step_file(*sims[i], times ? times[i] : 0.0, dts[i]);
+    }
+
+    // simulation.rs:107-203 with WakeData::QuasiSteady for every instance.
+    void step_quasi(bool first, const double* times, const double* dts, const double* fs, bool fs_on_device,
+                    sb_step_result* results) {
+        if (mlp) throw std::invalid_argument("the learned warm start runs on dynamic batches");
+        bool any_lin_start = false, any_lin_solver = false, any_iter = false;
+        parallel_for(n, [&](int i) {
+            sims[i]->update_horseshoe_core();
+            h_inst[i] = stage_inst(i, dts[i]);
+        });
+        for (int i = 0; i < n; ++i) {
+            sb_sim& s = *sims[i];
+            any_lin_solver = any_lin_solver || s.setup.solver.kind == 1;
+            any_lin_start = any_lin_start || (s.setup.solver.kind == 0 && s.setup.solver.start_linearized);
+            any_iter = any_iter || s.setup.solver.kind == 0;
         }
+        if (any_lin_solver && any_iter)
+            throw std::invalid_argument("a quasi-steady batch uses one solver (Linearized or SimpleIterative)");
+        check(hipMemcpyAsync(d_fs, fs, sizeof(double) * fs_total,
+                             fs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
+              "upload freestream");
+        check(hipEventRecord(ev[0], stream), "event");
+        check(hipMemcpyAsync(d_geo, h_geo, sizeof(double) * geo_total, hipMemcpyHostToDevice, stream), "upload geometry");
+        check(hipMemcpyAsync(d_inst, h_inst, sizeof(Inst) * n, hipMemcpyHostToDevice, stream), "upload");
+        BatchDev b = bd;
+        b.inst = d_inst;
+        check(launch_batch_prepare(b, first ? 1 : 0, stream), "prepare");
+        check(launch_batch_horseshoe_factors(b, stream), "horseshoe factors");
+        if (any_lin_solver) {
+            check(launch_batch_linearized(b, lin_variant, lin_shm, 0, stream), "linearized solve");
+        } else {
+            if (any_lin_start) check(launch_batch_linearized(b, lin_variant, lin_shm, 1, stream), "linearized start");
+            check(launch_batch_solve(b, stream), "solve");
+        }
+        check(launch_batch_forces(b, stream), "forces");
+        if (any_lin_solver || any_lin_start) {
+            check(launch_batch_take_singular(b, d_singular, stream), "singular flags");
+            check(hipMemcpyAsync(h_singular, d_singular, sizeof(double) * n, hipMemcpyDeviceToHost, stream), "download");
+        }
+        check(hipMemcpyAsync(h_out, d_out, sizeof(double) * out_total, hipMemcpyDeviceToHost, stream), "download");
+        check(hipEventRecord(ev[1], stream), "event");
+        check(hipStreamSynchronize(stream), "batch step");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+        last_ms = ms;
+        if (first) first_done = true;
+        for (int i = 0; i < n; ++i) {
+            sb_sim& s = *sims[i];
+            s.first_done = true;
+            s.steps_completed += 1;
+        }
+        if (any_lin_solver || any_lin_start)
+            for (int i = 0; i < n; ++i)
+                if (h_singular[i] != 0.0)  // linalg.rs:77-81, unwrapped at linearized.rs:139
+                    throw std::runtime_error("instance " + std::to_string(i) +
+                                             ": Linearized solver: matrix is singular or nearly singular at row " +
+                                             std::to_string((long)h_singular[i] - 1));
+        parallel_for(n, [&](int i) {
+            sb_sim& s = *sims[i];
+            std::copy(h_out + out_off[i], h_out + out_off[i + 1], s.h_out.begin());
+            if (results) s.write_result(times ? times[i] : 0.0, results + i);
+        });
     }
 
     // The launches of one step (captured into the graph after the first step).
@@ -394,6 +518,10 @@ int sb_batch_reset(sb_batch* b) {
 int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode) {
     if (!b) return -1;
     if (mode < 0 || mode > 3 || (mode > 0 && !m)) return -4;
+    if (b->quasi && mode > 0) {
+        g_error = "the learned warm start runs on dynamic batches";
+        return -4;
+    }
     return guarded([&] {
         if (m) mlp_reserve(m, b->n_strips);
         b->mlp = mode > 0 ? m : nullptr;
@@ -403,6 +531,10 @@ int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode) {
 
 int sb_batch_set_record_features(sb_batch* b, int on) {
     if (!b) return -1;
+    if (b->quasi && on) {
+        g_error = "features are recorded on dynamic batches";
+        return -4;
+    }
     b->record_features = on != 0;
     return 0;
 }

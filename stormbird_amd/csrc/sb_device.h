@@ -206,6 +206,16 @@ hipError_t launch_batch_warm_start(const BatchDev& b, sb_mlp* mlp, int apply, hi
 // Per instance, the better of the default guess and the prediction by the
 // solver's first evaluation (after the factors, before the solve).
 hipError_t launch_batch_warm_judge(const BatchDev& b, hipStream_t st);
+// Quasi-steady batches: horseshoe factors of every instance, and the
+// one-workgroup Linearized solve per instance (variant 1: N <= 128, 2: larger;
+// 0 when max_n needs the blocked form, which batches do not run).
+hipError_t launch_batch_horseshoe_factors(const BatchDev& b, hipStream_t st);
+int batch_linearized_variant(int max_n);
+// Shared memory of the variant for instances of ns[0..n) strips (0: one does not fit).
+size_t batch_linearized_shared(const int* ns, int n, int variant);
+hipError_t launch_batch_linearized(const BatchDev& b, int variant, size_t shm, int as_start, hipStream_t st);
+// Singular-pivot flags of every instance into flags[n] (device), cleared in the instances.
+hipError_t launch_batch_take_singular(const BatchDev& b, double* flags, hipStream_t st);
 // Resident workgroups of the persistent pair-sum kernels (0 fixed, 1 self-induction).
 int batch_resident_groups(int which, int mode, int sym);
 

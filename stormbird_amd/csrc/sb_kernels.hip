@@ -1835,6 +1835,30 @@ __global__ void kb_factors(const Inst* I, const long* off, int n) {
     factors_body(d, (b - off[i]) * blockDim.x + threadIdx.x);
 }
 
+__global__ void kb_horseshoe_factors(const Inst* I, const long* off, int n) {
+    long b = blockIdx.x;
+    int i = owner_of(off, n, b);
+    const Inst d = I[i];
+    horseshoe_factors_body(d, (b - off[i]) * blockDim.x + threadIdx.x);
+}
+
+// One workgroup per instance (the one-workgroup Linearized form).
+template <int T, int MAXC>
+__global__ __launch_bounds__(T) void kb_linearized(const Inst* I, int as_start) {
+    extern __shared__ double sh[];
+    const Inst d = I[blockIdx.x];
+    linearized_reg_body<T, MAXC>(d, sh, as_start);
+}
+
+// Singular-pivot flags of every instance (scal[4]) into flags[n], then cleared.
+__global__ void kb_take_singular(const Inst* I, int n, double* flags) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double* sc = I[i].scal;
+    flags[i] = sc[4];
+    sc[4] = 0.0;
+}
+
 __global__ void kb_velocity_copy(const Inst* I, const long* off, int n) {
     long b = blockIdx.x;
     int i = owner_of(off, n, b);
@@ -2479,6 +2503,48 @@ hipError_t launch_batch_fixed(const BatchDev& b, int mode, hipStream_t st) {
 hipError_t launch_batch_factors(const BatchDev& b, hipStream_t st) {
     hipLaunchKernelGGL(kb_factors, dim3((unsigned)b.blocks[BK_FACTORS]), dim3(256), 0, st, b.inst,
                        boff(b, BK_FACTORS), b.n);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_horseshoe_factors(const BatchDev& b, hipStream_t st) {
+    hipLaunchKernelGGL(kb_horseshoe_factors, dim3((unsigned)b.blocks[BK_FACTORS]), dim3(256), 0, st, b.inst,
+                       boff(b, BK_FACTORS), b.n);
+    return hipGetLastError();
+}
+
+int batch_linearized_variant(int max_n) {
+    const int v = linr_variant(max_n);
+    return v;
+}
+
+size_t batch_linearized_shared(const int* ns, int n, int variant) {
+    long m = 0;
+    for (int i = 0; i < n; ++i) {
+        long x = variant == 1 ? linr_shared_doubles<256, 72>(ns[i]) : linr_shared_doubles<512, 48>(ns[i]);
+        bool fits = variant == 1 ? linr_fits<256, 72>(ns[i]) : linr_fits<512, 48>(ns[i]);
+        if (!fits) return 0;
+        m = x > m ? x : m;
+    }
+    return sizeof(double) * (size_t)m;
+}
+
+hipError_t launch_batch_linearized(const BatchDev& b, int variant, size_t shm, int as_start, hipStream_t st) {
+    if (variant == 1) {
+        hipError_t e = allow_shared((const void*)kb_linearized<256, 72>, shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((kb_linearized<256, 72>), dim3(b.n), dim3(256), shm, st, b.inst, as_start);
+    } else if (variant == 2) {
+        hipError_t e = allow_shared((const void*)kb_linearized<512, 48>, shm);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((kb_linearized<512, 48>), dim3(b.n), dim3(512), shm, st, b.inst, as_start);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_take_singular(const BatchDev& b, double* flags, hipStream_t st) {
+    hipLaunchKernelGGL(kb_take_singular, dim3(blocks_for(b.n, 256)), dim3(256), 0, st, b.inst, b.n, flags);
     return hipGetLastError();
 }
 

@@ -40,19 +40,33 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
-class StepBuffers:
-    """Caller-owned output arrays for one step (N strips, W wings)."""
+# Output fields of one step: name, per-strip (True) or per-wing, shape after the count.
+FIELDS = (("circulation_strength", True, ()), ("angles_of_attack", True, ()),
+          ("ctrl_points_velocity", True, (3,)), ("solver_input_velocity", True, (3,)),
+          ("ctrl_points", True, (3,)), ("sectional_forces", True, (6, 3)),
+          ("integrated_forces", False, (6, 3)), ("integrated_moments", False, (6, 3)))
 
-    def __init__(self, n: int, w: int):
+
+def _field_shape(per_strip, tail, n, w):
+    count = n if per_strip else w
+    if len(tail) == 2:  # [6][count][3]
+        return (6, count, 3)
+    return (count,) + tail
+
+
+class StepBuffers:
+    """Caller-owned output arrays for one step (N strips, W wings).  `storage` maps each
+    field to a flat array to use (views into batch-wide arrays); default: own arrays."""
+
+    def __init__(self, n: int, w: int, storage=None):
         self.n, self.w = n, w
-        self.circulation_strength = np.zeros(n)
-        self.angles_of_attack = np.zeros(n)
-        self.ctrl_points_velocity = np.zeros((n, 3))
-        self.solver_input_velocity = np.zeros((n, 3))
-        self.ctrl_points = np.zeros((n, 3))
-        self.sectional_forces = np.zeros((6, n, 3))
-        self.integrated_forces = np.zeros((6, w, 3))
-        self.integrated_moments = np.zeros((6, w, 3))
+        for name, per_strip, tail in FIELDS:
+            shape = _field_shape(per_strip, tail, n, w)
+            if storage is None:
+                arr = np.zeros(shape)
+            else:
+                arr = storage[name].reshape(shape)
+            setattr(self, name, arr)
         self.c = StepResultC()
         self.c.circulation_strength = _ptr(self.circulation_strength)
         self.c.angles_of_attack = _ptr(self.angles_of_attack)
@@ -93,7 +107,8 @@ class SectionalForcesInput:
 
 
 class SimulationResult:
-    """One step's result (same field names as Stormbird's SimulationResult)."""
+    """One step's result (same field names as Stormbird's SimulationResult).  The nested
+    views (force_input, sectional_forces, per-wing integrated forces) are built on first use."""
 
     def __init__(self, *, time, iterations, residual, circulation_strength, angles_of_attack,
                  velocity, solver_input_velocity, ctrl_points, sectional, integrated_forces,
@@ -103,13 +118,36 @@ class SimulationResult:
         self.residual = residual
         self.ctrl_points = ctrl_points
         self.solver_input_ctrl_points_velocity = solver_input_velocity
-        self.force_input = SectionalForcesInput(circulation_strength, velocity, angles_of_attack)
-        self.sectional_forces = _Fields(sectional)
-        self.integrated_forces = [_Fields(integrated_forces[:, w, :]) for w in range(integrated_forces.shape[1])]
-        self.integrated_moments = [_Fields(integrated_moments[:, w, :]) for w in range(integrated_moments.shape[1])]
+        self._gamma, self._vel, self._alpha, self._sect = circulation_strength, velocity, angles_of_attack, sectional
         self._if = integrated_forces
         self._im = integrated_moments
-        self.input_power = [0.0] * integrated_forces.shape[1]
+        self._cache = {}
+
+    def _lazy(self, key, make):
+        v = self._cache.get(key)
+        if v is None:
+            v = self._cache[key] = make()
+        return v
+
+    @property
+    def force_input(self):
+        return self._lazy("fi", lambda: SectionalForcesInput(self._gamma, self._vel, self._alpha))
+
+    @property
+    def sectional_forces(self):
+        return self._lazy("sf", lambda: _Fields(self._sect))
+
+    @property
+    def integrated_forces(self):
+        return self._lazy("if", lambda: [_Fields(self._if[:, w, :]) for w in range(self._if.shape[1])])
+
+    @property
+    def integrated_moments(self):
+        return self._lazy("im", lambda: [_Fields(self._im[:, w, :]) for w in range(self._im.shape[1])])
+
+    @property
+    def input_power(self):
+        return [0.0] * self._if.shape[1]
 
     def integrated_forces_sum(self) -> np.ndarray:
         """Sum of the per-wing total forces (results/simulation.rs:61-69)."""
@@ -127,10 +165,56 @@ class SimulationResult:
     def as_arrays(self) -> dict:
         """Every numeric output as a flat dict (used by parity checks)."""
         return {
-            "circulation_strength": self.force_input.circulation_strength,
-            "angles_of_attack": self.force_input.angles_of_attack,
-            "velocity": self.force_input.velocity,
-            "sectional_forces": np.stack([getattr(self.sectional_forces, k) for k in FORCE_KINDS]),
+            "circulation_strength": self._gamma,
+            "angles_of_attack": self._alpha,
+            "velocity": self._vel,
+            "sectional_forces": self._sect,
             "integrated_forces": self._if,
             "integrated_moments": self._im,
         }
+
+
+class BatchResults:
+    """The results of one batched step: a sequence of per-instance SimulationResult built on
+    access from one copy of the batch-wide output arrays, plus whole-batch arrays."""
+
+    def __init__(self, time, iterations, residual, arrays, strip_off, wing_off):
+        self.time = time                # [n]
+        self.iterations = iterations    # [n]
+        self.residual = residual        # [n]
+        self._a = arrays                # field -> flat array over all instances
+        self._so, self._wo = strip_off, wing_off
+        self._items = {}
+
+    def __len__(self):
+        return len(self.iterations)
+
+    def field(self, name: str, i: int) -> np.ndarray:
+        per_strip, tail = next((p, t) for f, p, t in FIELDS if f == name)
+        n, w = self._so[i + 1] - self._so[i], self._wo[i + 1] - self._wo[i]
+        shape = _field_shape(per_strip, tail, n, w)
+        size = int(np.prod(shape))
+        start = (self._so[i] if per_strip else self._wo[i]) * size // (n if per_strip else w)
+        return self._a[name][start:start + size].reshape(shape)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[k] for k in range(*i.indices(len(self)))]
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        r = self._items.get(i)
+        if r is None:
+            f = lambda name: self.field(name, i)  # noqa: E731
+            r = self._items[i] = SimulationResult(
+                time=float(self.time[i]), iterations=int(self.iterations[i]), residual=float(self.residual[i]),
+                circulation_strength=f("circulation_strength"), angles_of_attack=f("angles_of_attack"),
+                velocity=f("ctrl_points_velocity"), solver_input_velocity=f("solver_input_velocity"),
+                ctrl_points=f("ctrl_points"), sectional=f("sectional_forces"),
+                integrated_forces=f("integrated_forces"), integrated_moments=f("integrated_moments"))
+        return r
+
+    def __iter__(self):
+        for i in range(len(self)):
+            yield self[i]

@@ -228,6 +228,39 @@ def polar_sweep_instance(rng: np.random.Generator, max_panels=3000, min_panels=2
     return json.dumps(setup), case
 
 
+def quasi_steady_polar_instance(rng: np.random.Generator, solver=None):
+    """One instance of the quasi-steady polar sweep (default QuasiSteady settings: horseshoe
+    wake, Linearized solve).
+
+    W ~ U{1..4} sails, strips/sail ~ U{8..32} (N <= 128), chord ~ U[5,15] m, AR ~ U[2,6],
+    AWA ~ U[20,180] deg, AWS ~ U[6,18] m/s; sails 2.5 chords apart along x, chords 8 deg
+    from the wind (the sail foil of the wing-sail workloads).
+    """
+    w = int(rng.integers(1, 5))
+    strips = int(rng.integers(8, 33))
+    c = float(rng.uniform(5.0, 15.0))
+    ar = float(rng.uniform(2.0, 6.0))
+    awa = float(rng.uniform(20.0, 180.0)) * DEG
+    aws = float(rng.uniform(6.0, 18.0))
+    u = [-aws * math.cos(awa), -aws * math.sin(awa), 0.0]
+    chord = rotate_around_axis([c * u[0] / aws, c * u[1] / aws, 0.0], -8.0 * DEG, [0.0, 0.0, 1.0])
+    span = ar * c
+    wings = []
+    for k in range(w):
+        x = (k - 0.5 * (w - 1)) * 2.5 * c
+        wings.append({
+            "section_points": [_v(x, 0, 0), _v(x, 0, span)],
+            "chord_vectors": [_v(*chord), _v(*chord)],
+            "section_model": foil(**SAIL_FOIL),
+        })
+    setup = {
+        "line_force_model": {"wing_builders": wings, "nr_sections": strips},
+        "simulation_settings": quasi_steady_settings(solver=solver),
+    }
+    case = {"velocity": u, "time_step": 0.25 * c / aws, "wings": w, "strips": strips}
+    return json.dumps(setup), case
+
+
 def uniform_freestream(n_points: int, velocity) -> np.ndarray:
     out = np.empty((n_points, 3))
     out[:] = np.asarray(velocity, dtype=np.float64)

@@ -120,3 +120,41 @@ def test_route_simulation_batch_equals_alone():
     assert route.batch.last_warm_taken() + route.batch.last_fallbacks() <= 3
     for r in res:
         assert np.all(np.isfinite(r.force_input.circulation_strength))
+
+
+@pytest.mark.parametrize("solver", [None, {"start_with_linearized_solution": True,
+                                           "max_iterations_per_time_step": 400}])
+def test_quasi_steady_batch_matches_single_instances(oracle, solver):
+    """A batch of quasi-steady instances (horseshoe wake; Linearized, or SimpleIterative from
+    the Linearized start) equals every instance stepped alone, bit for bit, over several steps;
+    two instances are held to the oracle too."""
+    rng = np.random.default_rng(11)
+    setups, cases = zip(*[S.quasi_steady_polar_instance(rng, solver=solver) for _ in range(12)])
+    batch = sb.Batch(list(setups))
+    alone = [sb.Simulation(js) for js in setups]
+    orcs = {0: oracle.Simulation(setups[0], "pm"), 5: oracle.Simulation(setups[5], "pm")}
+    assert batch.nr_freestream_points == sum(s.nr_ctrl_points for s in alone)
+    dts = np.array([c["time_step"] for c in cases])
+    for k in range(3):
+        print(f"quasi batch step {k}", flush=True)
+        parts = [S.uniform_freestream(s.nr_freestream_points(), np.array(c["velocity"]) * (1.0 + 0.05 * k))
+                 for s, c in zip(alone, cases)]
+        rb = batch.do_step(times=k * dts, time_steps=dts, freestream_velocity=np.concatenate(parts))
+        for i, sim in enumerate(alone):
+            ra = sim.do_step(time=k * dts[i], time_step=dts[i], freestream_velocity=parts[i])
+            compare_results(rb[i], ra, exact=True, where=f"step {k} instance {i}")
+            if i in orcs:
+                ro = orcs[i].do_step(time=k * dts[i], time_step=dts[i], freestream_velocity=parts[i])
+                compare_results(rb[i], ro, exact=True, where=f"oracle step {k} instance {i}")
+
+
+def test_batch_rejects_mixed_and_warm_quasi():
+    rng = np.random.default_rng(3)
+    q, _ = S.quasi_steady_polar_instance(rng)
+    d, _ = S.polar_sweep_instance(rng, max_panels=10, min_panels=5)
+    with pytest.raises(ValueError, match="either quasi-steady or dynamic"):
+        sb.Batch([q, d])
+    b = sb.Batch([q, q])
+    m = sb.Mlp(seed=1)
+    with pytest.raises(RuntimeError, match="dynamic batches"):
+        b.set_warm_start(m, 1)
