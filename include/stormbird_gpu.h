@@ -76,6 +76,12 @@ const char* sb_last_error(void);
  * (linearized.rs:54-222; a singular equation system fails the step with -4
  * where the reference's unwrap panics). */
 sb_sim* sb_sim_create(const char* setup_json);
+/* Parses and checks a setup string without touching a GPU (what
+ * SimulationBuilder's deserialization accepts, simulation_builder.rs:42-66, plus
+ * this library's limits).  out[0..7): strips, wings, span points, wake rows,
+ * freestream points, wake panels, 1 for quasi-steady.  No counterpart in the
+ * reference (there, building the simulation is the check). */
+int sb_setup_info(const char* setup_json, long* out, size_t cap);
 /* Same, on GPU `device` with a Biot-Savart arithmetic mode (SB_MODE_*). */
 sb_sim* sb_sim_create_ex(const char* setup_json, int device, int mode);
 /* Mirrors complete_sail_model_drop (interfaces/cstormbird/src/lifting_line/complete_sail_model.rs:59-66). */

@@ -76,6 +76,7 @@ def lib():
                                                  ctypes.c_int, _dp]
         L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
         L.sb_debug_linearized_in_shared_memory.argtypes = [ctypes.c_long]
+        L.sb_setup_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), ctypes.c_size_t]
         L.sb_debug_libm_host.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
         L.sb_batch_create.restype = vp
         L.sb_batch_create.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_long, ctypes.c_int, ctypes.c_int]
@@ -351,6 +352,17 @@ def wake_induced_velocities(panel_records, targets, symmetry: int = 0, mode: int
     if rc != 0:
         _err("wake_induced_velocities", rc)
     return out
+
+
+def setup_info(setup_string: str) -> dict:
+    """Parses a setup string without a GPU (sb_setup_info); ValueError on a bad setup."""
+    out = (ctypes.c_long * 7)()
+    rc = lib().sb_setup_info(setup_string.encode(), out, 7)
+    if rc != 0:
+        raise ValueError(lib().sb_last_error().decode())
+    keys = ("nr_ctrl_points", "nr_wings", "nr_span_points", "nr_wake_rows", "nr_freestream_points",
+            "nr_panels", "quasi_steady")
+    return dict(zip(keys, list(out)))
 
 
 def libm_device(fn: int, x, y=None) -> np.ndarray:

@@ -6,6 +6,7 @@
 // stormath/src/{rigid_body_motion.rs, spatial_vector/transformations.rs, interpolation.rs}.
 #include "sb_model.h"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -133,6 +134,12 @@ static double number(const Node* n, const char* what) {
     if (!n || n->t != Node::Num) throw std::runtime_error(std::string(what) + ": expected a number");
     return n->n;
 }
+// usize fields: serde reads them from non-negative integer literals only.
+static long count(const Node* n, const char* what) {
+    if (!n || n->t != Node::Num || !n->is_int || n->n < 0.0 || n->n > 9.0e15)
+        throw std::runtime_error(std::string(what) + ": expected a non-negative integer");
+    return (long)n->n;
+}
 static bool boolean(const Node* n, const char* what) {
     if (!n || n->t != Node::Bool) throw std::runtime_error(std::string(what) + ": expected a boolean");
     return n->b;
@@ -254,13 +261,22 @@ static void add_wing(LineModel& m, const Node* wb, int default_sections) {
     std::string sm = variant(wb->find("section_model"), &pl, "SectionModel");
     if (sm != "Foil") throw std::runtime_error("SectionModel `" + sm + "` is not supported on the GPU path");
     FoilParams foil = parse_foil(pl);
-    std::vector<char> seg_virtual;
-    if (const Node* v = wb->find("line_segment_is_virtual"))
+    std::vector<char> seg_virtual;  // Option<Vec<bool>>: null is None
+    if (const Node* v = wb->find("line_segment_is_virtual")) {
         if (v->t == Node::Arr)
             for (auto& e : v->items) seg_virtual.push_back(boolean(e.get(), "line_segment_is_virtual"));
-    int nsec = default_sections;
+        else if (v->t != Node::Null)
+            throw std::runtime_error("line_segment_is_virtual: expected a list of booleans or null");
+    }
+    if (const Node* v = wb->find("non_zero_circulation_at_ends")) {  // [bool; 2]
+        if (v->t != Node::Arr || v->items.size() != 2)
+            throw std::runtime_error("non_zero_circulation_at_ends: expected two booleans");
+        for (auto& e : v->items) boolean(e.get(), "non_zero_circulation_at_ends");
+    }
+    int nsec = default_sections;  // Option<usize>: null is None
     if (const Node* v = wb->find("nr_sections"))
-        if (v->t == Node::Num) nsec = (int)v->n;
+        if (v->t != Node::Null) nsec = (int)count(v, "WingBuilder.nr_sections");
+    if (nsec < 1) throw std::runtime_error("nr_sections must be at least 1");
     if (const Node* v = wb->find("input_power_model")) {
         const Node* p;
         if (variant(v, &p, "InputPowerModel") != "NoPower")
@@ -308,7 +324,8 @@ static LineModel parse_line_model(const Node* o) {  // builder/mod.rs:79-113
                    "local_wing_angles", "rotation", "translation", "force_calculation_settings"},
                "LineForceModelBuilder");
     LineModel m;
-    int nsec = (int)number(o->find("nr_sections"), "nr_sections");
+    if (!o->find("nr_sections")) throw std::runtime_error("LineForceModelBuilder: missing field `nr_sections`");
+    int nsec = (int)count(o->find("nr_sections"), "LineForceModelBuilder.nr_sections");
     if (const Node* v = o->find("density")) m.density = number(v, "density");
     const Node* wbs = o->find("wing_builders");
     if (!wbs || wbs->t != Node::Arr || wbs->items.empty())
@@ -339,7 +356,9 @@ static LineModel parse_line_model(const Node* o) {  // builder/mod.rs:79-113
         }
     }
     if (const Node* v = o->find("force_calculation_settings")) {
-        allow_only(v, {"include_viscous_lift_in_the_circulation"}, "ForceCalculationSettings");
+        // ForceCalculationSettings has no deny_unknown_fields (force_calculations.rs:41-45):
+        // other keys are ignored
+        if (v->t != Node::Obj) throw std::runtime_error("ForceCalculationSettings: expected an object");
         if (const Node* b = v->find("include_viscous_lift_in_the_circulation"))
             m.viscous_lift_in_circulation = boolean(b, "include_viscous_lift_in_the_circulation");
     }
@@ -368,7 +387,7 @@ static void parse_wake(WakeConfig& w, const Node* o, const LineModel& m) {  // b
                    "DynamicWakeBuilder");
     auto f = [&](const char* k) { return o ? o->find(k) : nullptr; };
     const Node* p;
-    if (const Node* v = f("nr_panels_per_line_element")) w.M = (int)number(v, "nr_panels_per_line_element");
+    if (const Node* v = f("nr_panels_per_line_element")) w.M = (int)count(v, "nr_panels_per_line_element");
     if (w.M < 1) throw std::runtime_error("nr_panels_per_line_element must be at least 1");
     if (const Node* v = f("first_panel_relative_length")) w.first_rel = number(v, "first_panel_relative_length");
     if (const Node* v = f("last_panel_relative_length")) w.last_rel = number(v, "last_panel_relative_length");
@@ -499,7 +518,8 @@ static void parse_solver(SolverConfig& s, const Node* v, bool quasi) {
                        "strength_difference_tolerance", "velocity_corrections", "use_raw_circulation_during_iterations",
                        "start_with_linearized_solution", "correction_factor_initial_solution"},
                    "SimpleIterative");
-    if (const Node* x = p->find("max_iterations_per_time_step")) s.max_iterations = (int)number(x, "max_iterations_per_time_step");
+    if (const Node* x = p->find("max_iterations_per_time_step"))
+        s.max_iterations = (int)std::min<long>(count(x, "max_iterations_per_time_step"), 2000000000L);
     if (const Node* x = p->find("damping_factor")) s.damping = number(x, "damping_factor");
     if (const Node* x = p->find("residual_tolerance_absolute")) s.residual_tol = number(x, "residual_tolerance_absolute");
     if (const Node* x = p->find("strength_difference_tolerance")) s.strength_tol = number(x, "strength_difference_tolerance");

@@ -37,6 +37,23 @@ sb_sim* sb_sim_create_ex(const char* json, int device, int mode) {
     return sim.release();
 }
 
+int sb_setup_info(const char* json, long* out, size_t cap) {
+    if (!json || !out) return -1;
+    if (cap < 7) return -3;
+    return guarded([&] {
+        Setup s = parse_setup(json);
+        const long N = s.lfm.n(), W = s.lfm.nw(), P = (long)s.lfm.span_points.size(), M = s.wake.M;
+        const long points = s.quasi ? 0 : P * (M + 1);
+        out[0] = N;
+        out[1] = W;
+        out[2] = P;
+        out[3] = M;
+        out[4] = N + points;
+        out[5] = N * M;
+        out[6] = s.quasi ? 1 : 0;
+    });
+}
+
 sb_sim* sb_sim_create(const char* json) { return sb_sim_create_ex(json, 0, MODE_EXACT); }
 
 void sb_sim_destroy(sb_sim* sim) { delete sim; }
