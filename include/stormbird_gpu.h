@@ -145,6 +145,8 @@ int sb_sim_set_translation_and_rotation_fd(sb_sim* sim, double time_step,
                                            const double* translation3, const double* rotation3);
 /* LineForceModel::set_local_wing_angles (data_update.rs:171-177). */
 int sb_sim_set_local_wing_angles(sb_sim* sim, const double* angles, size_t n);
+/* LineForceModel::local_wing_angles (line_force_model/mod.rs:92), n = number of wings. */
+int sb_sim_get_local_wing_angles(const sb_sim* sim, double* out, size_t n);
 
 /* DynamicWake public fields points / velocity_at_points / strengths /
  * number_of_time_steps_completed (dynamic_wake/mod.rs:41-64) and
@@ -183,7 +185,7 @@ int sb_wake_induced_velocities(const double* panel_records, size_t n_panels,
                                int mode, double* out_xyz);
 
 /* Device-side elementary functions, for the host/device bit-identity test:
- * fn 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log. */
+ * fn 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log, 6 atan. */
 int sb_debug_libm_device(int fn, const double* x, const double* y, double* out, size_t n);
 /* Same functions evaluated by the library's host code. */
 int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n);
@@ -268,6 +270,79 @@ int sb_batch_get_features(const sb_batch* b, float* features, float* scale, size
 long sb_batch_last_fallbacks(const sb_batch* b);
 /* Instances whose last solve started from the network's guess. */
 long sb_batch_last_warm_taken(const sb_batch* b);
+
+/* ---- Complete sail model ---------------------------------------------------
+ * CompleteSailModel (stormbird/src/lifting_line/complete_sail_model/mod.rs:38-330):
+ * a lifting-line simulation, a WindEnvironment (wind/environment.rs:21-360)
+ * and a Controller (controller/mod.rs:22-60), built from the JSON of
+ * CompleteSailModelBuilder (complete_sail_model/builder.rs:14-20):
+ *   {"lifting_line_simulation": <SimulationBuilder>, "wind_environment": {..},
+ *    "controller": <ControllerBuilder>}.
+ * Wind conditions cross the ABI as the JSON of WindCondition
+ * (wind/wind_condition/mod.rs:18-29).  The freestream of a step is evaluated
+ * on the device for the wake points and on the host for the control points
+ * (inflow corrections and controller measurements).  Where the reference
+ * panics (a controller with time_steps_between_updates 0, set points for more
+ * sails than the model has, an empty measurement range, interpolation tables
+ * of unequal length) these calls return -4 with a message.
+ * The reference's own C interface (cstormbird) is in cstormbird_compat.h. */
+typedef struct sb_csm sb_csm;
+sb_csm* sb_csm_create_ex(const char* setup_json, int device, int mode);  /* new_from_string */
+/* Parses and checks a CompleteSailModelBuilder string without touching a GPU:
+ * out[0..7) as sb_setup_info for its simulation, out[7] controller set points,
+ * out[8] 1 with inflow corrections (cap >= 9). */
+int sb_csm_setup_info(const char* setup_json, long* out, size_t cap);
+void sb_csm_destroy(sb_csm* m);
+/* The model's simulation (owned by the model): counts, wake state, settings. */
+sb_sim* sb_csm_simulation(sb_csm* m);
+/* do_step (mod.rs:174-191): freestream from the wind condition, then one step. */
+int sb_csm_do_step(sb_csm* m, double time, double time_step, const char* wind_condition_json,
+                   double ship_velocity, sb_step_result* out);
+/* The wake starts afresh at the next step, as do_multiple_steps does (mod.rs:152). */
+int sb_csm_restart(sb_csm* m);
+/* apply_controller_based_on_wind_condition (mod.rs:193-211). */
+int sb_csm_apply_controller_wind(sb_csm* m, double time, double time_step, const char* wind_condition_json,
+                                 double ship_velocity, double loading);
+/* apply_controller_based_on_simulation_result (mod.rs:290-318): the result's
+ * angles of attack [N], force-input velocity [3N], solver input velocity [3N]
+ * and control points [3N]. */
+int sb_csm_apply_controller_result(sb_csm* m, double time, double time_step, double loading,
+                                   const double* angles_of_attack, const double* velocity,
+                                   const double* input_velocity, const double* ctrl_points);
+/* simulate_steady_state_condition (mod.rs:99-122). */
+int sb_csm_simulate_steady_state(sb_csm* m, const char* wind_condition_json, double ship_velocity,
+                                 double loading, sb_step_result* out);
+/* simulate_condition_optimal_controller_loading (mod.rs:58-97): the result of
+ * the loading (of linspace(0.1, 1, n)) with the largest delivered power. */
+int sb_csm_simulate_optimal_loading(sb_csm* m, const char* wind_condition_json, double ship_velocity,
+                                    long nr_loadings, sb_step_result* out, double* best_loading);
+/* freestream_velocity (mod.rs:213-255) at the control points and wake points
+ * [3 * sb_sim_nr_freestream_points]. */
+int sb_csm_freestream(sb_csm* m, const char* wind_condition_json, double ship_velocity, double time,
+                      double* out, size_t n);
+/* WindEnvironment queries at n locations [3n] (environment.rs:66-190), with
+ * the gust signals at `time` and linear_velocity [3] (NULL: zero):
+ * kind 0 unsteady_apparent_wind_velocity_vector_at_location, 1 steady apparent,
+ * 2 unsteady_true_wind_velocity_vector_at_location, 3 steady true vector
+ * (vectors, out [3n]); 4 steady_true_wind_velocity_at_location, 5
+ * unsteady_parallel_true_wind_velocity_at_location (speeds, out [n]). */
+int sb_wind_velocity(const char* environment_json, const char* wind_condition_json, int kind,
+                     const double* locations, size_t n, const double* linear_velocity, double time, double* out);
+/* apparent_wind_direction_from_condition_and_linear_velocity (environment.rs:258-278). */
+int sb_wind_apparent_direction(const char* environment_json, const char* wind_condition_json,
+                               const double* linear_velocity, double height, double* out);
+/* apparent_wind_velocity_vectors_at_ctrl_points_with_corrections_applied
+ * (environment.rs:192-223): wing_ranges [2 * nr_wings] as (start, end). */
+int sb_wind_ctrl_points_velocity(const char* environment_json, const char* wind_condition_json,
+                                 const double* ctrl_points, size_t n, const double* linear_velocity, double time,
+                                 const long* wing_ranges, size_t nr_wings, double* out);
+/* WindCondition values (wind_condition/mod.rs:37-61) at n heights: which 0
+ * steady_true_wind_velocity_at_height, 1 unsteady_parallel_true_wind_velocity_at_height,
+ * 2 unsteady_perpendicular_true_wind_velocity, 3 unsteady_vertical_true_wind_velocity
+ * (2 and 3 ignore the height), 4 businger_dyer_unscaled_correction (0 unless
+ * the profile is a LogarithmicModel). */
+int sb_wind_condition_values(const char* wind_condition_json, int which, const double* heights, size_t n,
+                             double time, double* out);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,7 @@
 #define ORC_ACOS sbm::acos
 #define ORC_POW sbm::pow
 #define ORC_LOG sbm::log
+#define ORC_ATAN sbm::atan
 #else
 #define ORC_SIN std::sin
 #define ORC_COS std::cos
@@ -45,6 +46,7 @@
 #define ORC_ACOS std::acos
 #define ORC_POW std::pow
 #define ORC_LOG std::log
+#define ORC_ATAN std::atan
 #endif
 
 namespace {
@@ -1687,6 +1689,43 @@ int orc_sim_set_local_wing_angles(orc_sim* s, const double* a, size_t n) {
     return 0;
 }
 
+int orc_sim_get_line_model(const orc_sim* s, double* span_lines, double* chord_global,
+                           double* chord_local_not_rotated, long* wing_ranges, double* wing_angles,
+                           double* rigid_body) {
+    if (!s) return -1;
+    const auto& m = s->lfm;
+    for (size_t i = 0; i < m.n(); ++i)
+        for (int c = 0; c < 3; ++c) {
+            if (span_lines) {
+                span_lines[6 * i + c] = m.span_lines_global[i].start[c];
+                span_lines[6 * i + 3 + c] = m.span_lines_global[i].end[c];
+            }
+            if (chord_global) chord_global[3 * i + c] = m.chord_global[i][c];
+            if (chord_local_not_rotated) chord_local_not_rotated[3 * i + c] = m.chord_local_not_rotated[i][c];
+        }
+    for (size_t w = 0; w < m.nw(); ++w) {
+        if (wing_ranges) {
+            wing_ranges[2 * w] = (long)m.wing_indices[w].start;
+            wing_ranges[2 * w + 1] = (long)m.wing_indices[w].end;
+        }
+        if (wing_angles) wing_angles[w] = m.local_wing_angles[w];
+    }
+    if (rigid_body)
+        for (int c = 0; c < 3; ++c) {
+            rigid_body[c] = m.rbm.translation[c];
+            rigid_body[3 + c] = m.rbm.rotation[c];
+            rigid_body[6 + c] = m.rbm.velocity_linear[c];
+            rigid_body[9 + c] = m.rbm.velocity_angular[c];
+        }
+    return 0;
+}
+
+int orc_sim_restart(orc_sim* s) {  // complete_sail_model/mod.rs:152
+    if (!s) return -1;
+    s->first_done = false;
+    return 0;
+}
+
 int orc_sim_get_wake_state(const orc_sim* s, double* pts, double* vel, double* str, long* steps,
                            int* first) {
     if (!s) return -1;
@@ -1803,6 +1842,7 @@ void orc_libm_eval(int fn, const double* x, const double* y, double* out, size_t
             case 2: out[i] = ORC_EXP(x[i]); break;
             case 3: out[i] = ORC_ACOS(x[i]); break;
             case 4: out[i] = ORC_POW(x[i], y[i]); break;
+            case 6: out[i] = ORC_ATAN(x[i]); break;
             default: out[i] = ORC_LOG(x[i]); break;
         }
     }

@@ -73,6 +73,17 @@ int orc_sim_set_translation_and_rotation_fd(orc_sim* sim, double time_step,
                                            const double* rotation3);
 int orc_sim_set_local_wing_angles(orc_sim* sim, const double* angles, size_t n);
 
+/* Line-force-model state the controller restatement reads: global span lines
+ * [N][2][3], chord_vectors_global [N][3], chord_vectors_local_not_rotated
+ * [N][3], wing index ranges [W][2], local wing angles [W], rigid body
+ * [translation, rotation, velocity_linear, velocity_angular][3].  Any pointer
+ * may be NULL. */
+int orc_sim_get_line_model(const orc_sim* sim, double* span_lines, double* chord_global,
+                           double* chord_local_not_rotated, long* wing_ranges, double* wing_angles,
+                           double* rigid_body);
+/* first_time_step_completed = false (complete_sail_model/mod.rs:152). */
+int orc_sim_restart(orc_sim* sim);
+
 /* Wake state access (parity tests start both sides from one state). */
 int orc_sim_get_wake_state(const orc_sim* sim, double* points, double* velocities,
                            double* strengths, long* steps_completed,
@@ -97,7 +108,7 @@ double orc_foil_cl(const char* foil_json, double alpha, int which);
 double orc_sigmoid(double x, double x0, double range);
 /* 0: host libm, 1: portable libm shared with the device code */
 int orc_libm_variant(void);
-/* fn: 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log */
+/* fn: 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log, 6 atan */
 void orc_libm_eval(int fn, const double* x, const double* y, double* out, size_t n);
 
 #ifdef __cplusplus

@@ -73,6 +73,8 @@ def lib(variant: str = "glibc"):
         L.orc_libm_eval.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
         L.orc_solve_gaussian_elimination.argtypes = [_dp, _dp, ctypes.c_size_t, _dp]
         L.orc_horseshoe_unit_velocity.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]
+        L.orc_sim_get_line_model.argtypes = [vp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long), _dp, _dp]
+        L.orc_sim_restart.argtypes = [vp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -155,6 +157,21 @@ class Simulation:
         if lib(self._v).orc_sim_set_local_wing_angles(self._h, _p(a), a.size) != 0:
             raise ValueError("wrong number of wing angles")
 
+    def line_model(self) -> dict:
+        """Global span lines, chord vectors, wing ranges, wing angles and rigid-body state."""
+        n, w = self.nr_ctrl_points, self.nr_wings
+        d = {"span_lines": np.zeros((n, 2, 3)), "chord_global": np.zeros((n, 3)),
+             "chord_local_not_rotated": np.zeros((n, 3)), "wing_ranges": np.zeros((w, 2), dtype=np.int64),
+             "wing_angles": np.zeros(w), "rigid_body": np.zeros((4, 3))}
+        lib(self._v).orc_sim_get_line_model(self._h, _p(d["span_lines"]), _p(d["chord_global"]),
+                                            _p(d["chord_local_not_rotated"]),
+                                            d["wing_ranges"].ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
+                                            _p(d["wing_angles"]), _p(d["rigid_body"]))
+        return d
+
+    def restart(self):
+        lib(self._v).orc_sim_restart(self._h)
+
     def get_wake_state(self):
         pts = np.zeros((self.nr_wake_points, 3))
         vel = np.zeros((self.nr_wake_points, 3))
@@ -209,7 +226,7 @@ def sigmoid(x, x0, r):
 
 
 def libm_eval(fn: int, x, y=None, variant: str = "glibc") -> np.ndarray:
-    """fn: 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log."""
+    """fn: 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log, 6 atan."""
     x = _f64(x)
     y = _f64(y) if y is not None else x
     out = np.zeros_like(x)

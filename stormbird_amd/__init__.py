@@ -21,7 +21,7 @@ import numpy as np
 from .results import FIELDS, BatchResults, StepBuffers, StepResultC, SimulationResult  # noqa: F401
 
 __all__ = ["Simulation", "SimulationResult", "MODE_EXACT", "MODE_FAST", "library_path", "lib",
-           "wake_induced_velocities"]
+           "wake_induced_velocities", "CompleteSailModel", "WindCondition", "WindEnvironment"]
 
 MODE_EXACT = 0
 MODE_FAST = 1
@@ -572,3 +572,6 @@ def format_f64(v: float) -> str:
     if n < 0:
         _err("sb_debug_format_f64", n)
     return buf.value.decode()
+
+
+from .complete_sail_model import CompleteSailModel, WindCondition, WindEnvironment  # noqa: E402

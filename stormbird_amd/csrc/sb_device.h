@@ -6,6 +6,7 @@
 
 #include "sb_common.h"
 #include "sb_mlp.h"
+#include "sb_wind.h"
 
 namespace sb {
 
@@ -155,6 +156,8 @@ hipError_t launch_induced_targets(const Inst& d, const double* targets, long n_t
                                   int mode, double* out, McBuffers* mc, hipStream_t st);
 hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targets, long n_panels,
                               double* partial, long* n_blocks, hipStream_t st);
+// Wind-environment freestream at the wake points (after the N control points of d.fs).
+hipError_t launch_wind_points(const Inst& d, const WindField& w, hipStream_t st);
 hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st);
 
 // ---------------------------------------------------------------------------

@@ -249,8 +249,10 @@ struct sb_sim {
                   "upload geometry");
     }
 
-    void create(const char* json, int dev, int md) {
-        setup = parse_setup(json);
+    void create(const char* json, int dev, int md) { create(parse_setup(json), dev, md); }
+
+    void create(Setup parsed, int dev, int md) {
+        setup = std::move(parsed);
         device = dev;
         mode = md;
         check(hipSetDevice(device), "hipSetDevice");
@@ -362,12 +364,14 @@ struct sb_sim {
               "upload freestream");
     }
 
+    // fs == nullptr: the freestream is already in d_fs (written by the caller's
+    // kernels on this stream).
     void step(double time, double dt, const double* fs, size_t n, bool fs_on_device, sb_step_result* out) {
         check(hipSetDevice(device), "hipSetDevice");
         if ((long)n != N + nr_points) throw std::invalid_argument("freestream length mismatch");
         setup.lfm.update_global();
         pack_geometry();
-        upload_freestream(fs, n, fs_on_device);
+        if (fs) upload_freestream(fs, n, fs_on_device);
         if (setup.quasi) update_horseshoe_core();
         Inst d = inst();
         bool first = !first_done;
@@ -562,6 +566,20 @@ struct sb_sim {
 };
 
 namespace sb_host {
+
+// Sizes of a parsed setup: strips, wings, span points, wake rows, freestream
+// points, wake panels, 1 for quasi-steady.
+inline void setup_counts(const Setup& s, long* out) {
+    const long N = s.lfm.n(), W = s.lfm.nw(), P = (long)s.lfm.span_points.size(), M = s.wake.M;
+    const long points = s.quasi ? 0 : P * (M + 1);
+    out[0] = N;
+    out[1] = W;
+    out[2] = P;
+    out[3] = M;
+    out[4] = N + points;
+    out[5] = N * M;
+    out[6] = s.quasi ? 1 : 0;
+}
 
 template <class F>
 int guarded(F&& f) {

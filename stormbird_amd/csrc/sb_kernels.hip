@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sb_device.h"
+#include "sb_wind.h"
 #include "sb_mlp.h"
 
 namespace sb {
@@ -1763,6 +1764,14 @@ __global__ void k_copy3(const double* __restrict__ src, double* __restrict__ dst
     if (q < 3 * n) dst[q] = src[q];
 }
 
+// Freestream of the wind environment at every wake point, written after the
+// control points (environment.rs:182-190 per point; complete_sail_model/mod.rs:217-255).
+__global__ void k_wind_points(Inst d, WindField w) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long)d.P * (d.M + 1)) return;
+    store3(const_cast<double*>(d.fs) + 3L * (d.N + q), apparent_wind_at(w, load3(d.pts + 3 * q)));
+}
+
 __global__ void k_libm(int fn, const double* x, const double* y, double* out, long n) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n) return;
@@ -1773,6 +1782,7 @@ __global__ void k_libm(int fn, const double* x, const double* y, double* out, lo
         case 2: r = sbm::exp(x[q]); break;
         case 3: r = sbm::acos(x[q]); break;
         case 4: r = sbm::pow(x[q], y[q]); break;
+        case 6: r = sbm::atan(x[q]); break;
         default: r = sbm::log(x[q]); break;
     }
     out[q] = r;
@@ -2428,6 +2438,13 @@ hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targe
     if (n_targets <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_count_pairs, dim3((unsigned)*n_blocks), dim3(256), 0, st, d, targets, n_targets, n_panels,
                        partial);
+    return hipGetLastError();
+}
+
+hipError_t launch_wind_points(const Inst& d, const WindField& w, hipStream_t st) {
+    long n = (long)d.P * (d.M + 1);
+    if (d.quasi || n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_wind_points, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, w);
     return hipGetLastError();
 }
 

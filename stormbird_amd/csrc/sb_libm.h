@@ -8,8 +8,9 @@
 // sqrt already do; the transcendental functions of two different libms do not.
 // These functions use only +, -, *, /, sqrt, fma and rint, in a fixed order,
 // so both compilers produce the same results.  Accuracy is <= 1 ulp for exp,
-// sin, cos and acos on the arguments the lifting-line path produces, and a
-// few ulp for pow; tests/test_libm.py measures both against the system libm.
+// sin, cos, acos and log on the arguments the lifting-line path produces, and
+// <= 2 ulp for pow and atan; tests/test_library_host.py measures them against
+// the system libm.
 //
 // Methods (all textbook, coefficients are the exact Taylor coefficients
 // rounded to double):
@@ -26,6 +27,8 @@
 //          s = (m-1)/(m+1), odd series in s, carried in double-double.
 //   pow  — x^y = exp(y log x) with log in double-double and the low part of
 //          y*log(x) applied as exp(h + l) ~ exp(h) (1 + l).
+//   atan — reduction to |u| <= tan(pi/8) (complement and pi/4 shift), odd
+//          Taylor series (wind profiles only).
 #pragma once
 
 #include <stdint.h>
@@ -265,6 +268,49 @@ SB_HD double log(double x) {
     log_dd(x, &h, &l);
     return h + l;
 }
+// atan — for |u| <= tan(pi/8) the Taylor series u + u^3 P(u^2) (odd powers to
+//   u^45); on (tan(pi/8), 1] atan(t) = pi/4 + atan((t - 1) / (t + 1)); for t > 1
+//   atan(t) = pi/2 - atan(1 / t); odd symmetry (signed zero kept).
+constexpr double PIO4_HI = 0x1.921fb54442d18p-1;
+constexpr double PIO4_LO = 0x1.1a62633145c07p-55;
+SB_HD double atan_kernel(double u) {
+    double z = u * u;
+    double p = 0x1.6c16c16c16c17p-6;  // 1/45
+    p = fma_(p, z, -0x1.7d05f417d05f4p-6);
+    p = fma_(p, z, 0x1.8f9c18f9c18fap-6);
+    p = fma_(p, z, -0x1.a41a41a41a41ap-6);
+    p = fma_(p, z, 0x1.bacf914c1bad0p-6);
+    p = fma_(p, z, -0x1.d41d41d41d41dp-6);
+    p = fma_(p, z, 0x1.f07c1f07c1f08p-6);
+    p = fma_(p, z, -0x1.0842108421084p-5);
+    p = fma_(p, z, 0x1.1a7b9611a7b96p-5);
+    p = fma_(p, z, -0x1.2f684bda12f68p-5);
+    p = fma_(p, z, 0x1.47ae147ae147bp-5);
+    p = fma_(p, z, -0x1.642c8590b2164p-5);
+    p = fma_(p, z, 0x1.8618618618618p-5);
+    p = fma_(p, z, -0x1.af286bca1af28p-5);
+    p = fma_(p, z, 0x1.e1e1e1e1e1e1ep-5);
+    p = fma_(p, z, -0x1.1111111111111p-4);
+    p = fma_(p, z, 0x1.3b13b13b13b14p-4);
+    p = fma_(p, z, -0x1.745d1745d1746p-4);
+    p = fma_(p, z, 0x1.c71c71c71c71cp-4);
+    p = fma_(p, z, -0x1.2492492492492p-3);
+    p = fma_(p, z, 0x1.999999999999ap-3);
+    p = fma_(p, z, -0x1.5555555555555p-2);  // -1/3
+    return u + (u * z) * p;
+}
+SB_HD double atan_01(double t) {  // 0 <= t <= 1
+    if (t <= 0x1.a827999fcef32p-2) return atan_kernel(t);  // tan(pi/8)
+    double u = (t - 1.0) / (t + 1.0);
+    return PIO4_HI + (atan_kernel(u) + PIO4_LO);
+}
+SB_HD double atan(double x) {
+    if (x != x) return x;
+    double t = fabs_(x);
+    double r = t <= 1.0 ? atan_01(t) : PIO2_HI - (atan_01(1.0 / t) - PIO2_LO);
+    return (bits(x) >> 63) ? -r : r;
+}
+
 // x^y for x >= 0 (all uses on the lifting-line path take |.| first)
 SB_HD double pow(double x, double y) {
     if (y == 0.0 || x == 1.0) return 1.0;

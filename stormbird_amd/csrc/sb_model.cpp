@@ -560,10 +560,8 @@ static void parse_quasi_wake(QuasiConfig& q, const Node* o) {  // wake/settings.
     }
 }
 
-Setup parse_setup(const char* text) {
-    if (!text) throw std::runtime_error("null setup string");
-    auto root = json::parse(text);
-    allow_only(root.get(), {"line_force_model", "simulation_settings"}, "SimulationBuilder");
+static Setup parse_setup_node(const Node* root) {
+    allow_only(root, {"line_force_model", "simulation_settings"}, "SimulationBuilder");
     const Node* lfm = root->find("line_force_model");
     if (!lfm) throw std::runtime_error("SimulationBuilder: missing field `line_force_model`");
     Setup s;
@@ -588,6 +586,248 @@ Setup parse_setup(const char* text) {
     parse_solver(s.solver, p ? p->find("solver") : nullptr, false);
     parse_wake(s.wake, p ? p->find("wake") : nullptr, s.lfm);
     return s;
+}
+
+Setup parse_setup(const char* text) {
+    if (!text) throw std::runtime_error("null setup string");
+    auto root = json::parse(text);
+    return parse_setup_node(root.get());
+}
+
+// ---------------------------------------------------------------------------
+// Wind, controller and complete sail model settings
+// ---------------------------------------------------------------------------
+static std::vector<double> numbers(const Node* n, const char* what) {
+    if (!n || n->t != Node::Arr) throw std::runtime_error(std::string(what) + ": expected a list of numbers");
+    std::vector<double> v;
+    for (auto& e : n->items) v.push_back(number(e.get(), what));
+    return v;
+}
+static const Node* required(const Node* o, const char* key, const char* what) {
+    const Node* v = o->find(key);
+    if (!v) throw std::runtime_error(std::string(what) + ": missing field `" + key + "`");
+    return v;
+}
+static void require_object(const Node* o, const char* what) {
+    if (!o || o->t != Node::Obj) throw std::runtime_error(std::string(what) + ": expected an object");
+}
+
+double Spectrum::value_at_time(double t) const {  // discretized_spectrum.rs:37-45
+    double out = 0.0;
+    for (size_t i = 0; i < frequencies.size(); ++i)
+        out += amplitudes[i] * sbm::sin(TAU * frequencies[i] * t + phases[i]);
+    return out;
+}
+
+static Spectrum parse_spectrum(const Node* o) {  // no deny_unknown_fields
+    require_object(o, "DiscretizedSpectrum");
+    Spectrum s;
+    s.frequencies = numbers(required(o, "frequencies", "DiscretizedSpectrum"), "frequencies");
+    s.amplitudes = numbers(required(o, "amplitudes", "DiscretizedSpectrum"), "amplitudes");
+    s.phases = numbers(required(o, "phases", "DiscretizedSpectrum"), "phases");
+    if (s.amplitudes.size() < s.frequencies.size() || s.phases.size() < s.frequencies.size())
+        throw std::runtime_error("DiscretizedSpectrum: fewer amplitudes or phases than frequencies");
+    return s;
+}
+
+static WindConditionCfg parse_wind_condition_node(const Node* o) {  // wind_condition/mod.rs:18-29
+    require_object(o, "WindCondition");  // no deny_unknown_fields
+    WindConditionCfg w;
+    w.direction_coming_from = number(required(o, "direction_coming_from", "WindCondition"), "direction_coming_from");
+    const Node* p;
+    std::string t = variant(required(o, "velocity_variation", "WindCondition"), &p, "VelocityVariation");
+    if (t == "Constant") {
+        w.profile = 0;
+        w.p0 = number(p, "Constant");
+    } else if (t == "PowerModel") {  // power_model.rs:10-23, no deny_unknown_fields
+        require_object(p, "PowerModel");
+        w.profile = 1;
+        w.p0 = number(required(p, "reference_velocity", "PowerModel"), "reference_velocity");
+        w.p1 = 10.0;
+        w.p2 = 1.0 / 9.0;
+        if (const Node* x = p->find("reference_height")) w.p1 = number(x, "reference_height");
+        if (const Node* x = p->find("power_factor")) w.p2 = number(x, "power_factor");
+    } else if (t == "LogarithmicModel") {  // logarithmic_model.rs:10-28, no deny_unknown_fields
+        require_object(p, "LogarithmicModel");
+        w.profile = 2;
+        w.p0 = number(required(p, "friction_velocity", "LogarithmicModel"), "friction_velocity");
+        w.p1 = number(required(p, "surface_roughness", "LogarithmicModel"), "surface_roughness");
+        w.p2 = 0.41;
+        if (const Node* x = p->find("obukhov_length"))
+            if (x->t != Node::Null) {
+                w.has_obukhov = true;
+                w.obukhov = number(x, "obukhov_length");
+            }
+        if (const Node* x = p->find("von_karman_constant")) w.p2 = number(x, "von_karman_constant");
+        if (const Node* x = p->find("stable_coefficient")) w.stable = number(x, "stable_coefficient");
+        if (const Node* x = p->find("unstable_coefficient")) w.unstable = number(x, "unstable_coefficient");
+    } else {
+        throw std::runtime_error("unknown VelocityVariation `" + t + "`");
+    }
+    const char* gusts[3] = {"parallel_gust", "perpendicular_gust", "vertical_gust"};
+    for (int k = 0; k < 3; ++k)
+        if (const Node* g = o->find(gusts[k]))
+            if (g->t != Node::Null) {
+                w.has_gust[k] = true;
+                w.gust[k] = parse_spectrum(g);
+            }
+    return w;
+}
+
+WindConditionCfg parse_wind_condition(const char* text) {
+    if (!text) throw std::runtime_error("null wind condition string");
+    auto root = json::parse(text);
+    return parse_wind_condition_node(root.get());
+}
+
+static std::pair<long, long> parse_range(const Node* n) {  // serde's Range<usize>: {start, end} or [start, end]
+    if (n && n->t == Node::Arr && n->items.size() == 2)
+        return {count(n->items[0].get(), "Range.start"), count(n->items[1].get(), "Range.end")};
+    allow_only(n, {"start", "end"}, "Range");
+    return {count(required(n, "start", "Range"), "Range.start"), count(required(n, "end", "Range"), "Range.end")};
+}
+
+static WindEnvironmentCfg parse_wind_environment_node(const Node* o) {  // environment.rs:21-34
+    allow_only(o, {"up_direction", "wind_rotation_axis", "zero_direction_vector", "water_plane_height",
+                   "inflow_corrections"},
+               "WindEnvironment");
+    WindEnvironmentCfg e;
+    if (const Node* v = o->find("up_direction")) e.up = vector(v, "up_direction");
+    if (const Node* v = o->find("wind_rotation_axis")) e.axis = vector(v, "wind_rotation_axis");
+    if (const Node* v = o->find("zero_direction_vector")) e.zero_dir = vector(v, "zero_direction_vector");
+    if (const Node* v = o->find("water_plane_height")) e.water_plane_height = number(v, "water_plane_height");
+    if (const Node* v = o->find("inflow_corrections"))
+        if (v->t != Node::Null) {  // inflow_corrections.rs:49-53, no deny_unknown_fields
+            require_object(v, "InflowCorrections");
+            e.has_inflow = true;
+            e.inflow_directions = numbers(required(v, "apparent_wind_directions", "InflowCorrections"),
+                                          "apparent_wind_directions");
+            const Node* cs = required(v, "corrections", "InflowCorrections");
+            if (cs->t != Node::Arr) throw std::runtime_error("InflowCorrections.corrections: expected a list");
+            for (auto& c : cs->items) {  // :13-19, no deny_unknown_fields
+                require_object(c.get(), "InflowCorrectionsSingleDirection");
+                InflowCorrectionsSingle one;
+                one.heights = numbers(required(c.get(), "height_values", "InflowCorrectionsSingleDirection"),
+                                      "height_values");
+                one.magnitude = numbers(required(c.get(), "magnitude_corrections", "InflowCorrectionsSingleDirection"),
+                                        "magnitude_corrections");
+                one.angle = numbers(required(c.get(), "angle_corrections", "InflowCorrectionsSingleDirection"),
+                                    "angle_corrections");
+                const Node* wi = required(c.get(), "wing_indices", "InflowCorrectionsSingleDirection");
+                if (wi->t != Node::Arr) throw std::runtime_error("wing_indices: expected a list of ranges");
+                for (auto& r : wi->items) one.wing_ranges.push_back(parse_range(r.get()));
+                e.inflow.push_back(std::move(one));
+            }
+        }
+    return e;
+}
+
+WindEnvironmentCfg parse_wind_environment(const char* text) {
+    if (!text) throw std::runtime_error("null wind environment string");
+    auto root = json::parse(text);
+    return parse_wind_environment_node(root.get());
+}
+
+static MeasurementCfg parse_measurement(const Node* o) {  // measurements.rs:20-30, no deny_unknown_fields
+    MeasurementCfg m;
+    if (!o) return m;
+    require_object(o, "MeasurementSettings");
+    if (const Node* v = o->find("measurement_type")) {
+        const Node* p;
+        std::string t = variant(v, &p, "MeasurementType");
+        if (t == "Mean") m.type = 0;
+        else if (t == "Max") m.type = 1;
+        else if (t == "Min") m.type = 2;
+        else throw std::runtime_error("unknown MeasurementType `" + t + "`");
+    }
+    if (const Node* v = o->find("start_index")) m.start_index = count(v, "start_index");
+    if (const Node* v = o->find("end_offset")) m.end_offset = count(v, "end_offset");
+    return m;
+}
+
+static SetPointsCfg parse_set_points(const Node* o) {  // set_points.rs:52-68
+    allow_only(o, {"apparent_wind_directions_data", "angle_of_attack_data", "section_model_internal_state_data",
+                   "internal_state_type", "use_effective_angle_of_attack", "max_local_wing_angle_change_rate",
+                   "max_internal_section_state_change_rate"},
+               "ControllerSetPoints");
+    SetPointsCfg c;
+    c.directions = numbers(required(o, "apparent_wind_directions_data", "ControllerSetPoints"),
+                           "apparent_wind_directions_data");
+    if (const Node* v = o->find("angle_of_attack_data"))
+        if (v->t != Node::Null) {
+            c.has_aoa = true;
+            c.aoa = numbers(v, "angle_of_attack_data");
+        }
+    if (const Node* v = o->find("section_model_internal_state_data"))
+        if (v->t != Node::Null) {
+            c.has_state = true;
+            c.state = numbers(v, "section_model_internal_state_data");
+        }
+    if (const Node* v = o->find("internal_state_type")) {
+        const Node* p;
+        std::string t = variant(v, &p, "InternalStateType");
+        if (t == "SpinRatio") {  // SpinRatioConversion: no deny_unknown_fields
+            require_object(p, "SpinRatioConversion");
+            c.spin_ratio = true;
+            c.diameter = number(required(p, "diameter", "SpinRatioConversion"), "diameter");
+            c.max_rps = number(required(p, "max_rps", "SpinRatioConversion"), "max_rps");
+        } else if (t != "Generic") {
+            throw std::runtime_error("unknown InternalStateType `" + t + "`");
+        }
+    }
+    if (const Node* v = o->find("use_effective_angle_of_attack")) c.use_effective = boolean(v, "use_effective_angle_of_attack");
+    if (const Node* v = o->find("max_local_wing_angle_change_rate"))
+        if (v->t != Node::Null) {
+            c.has_max_angle_rate = true;
+            c.max_angle_rate = number(v, "max_local_wing_angle_change_rate");
+        }
+    if (const Node* v = o->find("max_internal_section_state_change_rate"))
+        if (v->t != Node::Null) {
+            c.has_max_state_rate = true;
+            c.max_state_rate = number(v, "max_internal_section_state_change_rate");
+        }
+    return c;
+}
+
+static ControllerCfg parse_controller(const Node* o) {  // controller/builder.rs:14-31
+    allow_only(o, {"set_points", "flow_measurement_settings", "time_steps_between_updates", "start_time",
+                   "moving_average_window_size", "use_input_velocity_for_apparent_wind_direction"},
+               "ControllerBuilder");
+    ControllerCfg c;
+    const Node* sp = required(o, "set_points", "ControllerBuilder");
+    if (sp->t != Node::Arr) throw std::runtime_error("ControllerBuilder.set_points: expected a list");
+    for (auto& e : sp->items) c.set_points.push_back(parse_set_points(e.get()));
+    if (const Node* v = o->find("flow_measurement_settings")) {  // measurements.rs:32-41
+        allow_only(v, {"angle_of_attack", "wind_direction", "wind_velocity"}, "FlowMeasurementSettings");
+        c.aoa = parse_measurement(v->find("angle_of_attack"));
+        c.direction = parse_measurement(v->find("wind_direction"));
+        c.velocity = parse_measurement(v->find("wind_velocity"));
+    }
+    if (const Node* v = o->find("time_steps_between_updates")) c.steps_between_updates = count(v, "time_steps_between_updates");
+    if (const Node* v = o->find("start_time")) c.start_time = number(v, "start_time");
+    if (const Node* v = o->find("moving_average_window_size"))  // parsed; the built controller does not use it
+        if (v->t != Node::Null) count(v, "moving_average_window_size");
+    if (const Node* v = o->find("use_input_velocity_for_apparent_wind_direction"))
+        c.use_input_velocity = boolean(v, "use_input_velocity_for_apparent_wind_direction");
+    return c;
+}
+
+CompleteSailModelCfg parse_complete_sail_model(const char* text) {  // complete_sail_model/builder.rs:14-20
+    if (!text) throw std::runtime_error("null setup string");
+    auto root = json::parse(text);
+    allow_only(root.get(), {"lifting_line_simulation", "wind_environment", "controller"}, "CompleteSailModelBuilder");
+    CompleteSailModelCfg c;
+    c.sim = parse_setup_node(required(root.get(), "lifting_line_simulation", "CompleteSailModelBuilder"));
+    c.env = parse_wind_environment_node(required(root.get(), "wind_environment", "CompleteSailModelBuilder"));
+    c.controller = parse_controller(required(root.get(), "controller", "CompleteSailModelBuilder"));
+    return c;
+}
+
+double interpolate_scalar(double x, const double* xs, const double* ys, size_t n) {
+    return interpolate(x, std::vector<double>(xs, xs + n), std::vector<double>(ys, ys + n));
+}
+V3 interpolate_vector(double x, const double* xs, const V3* ys, size_t n) {
+    return interpolate(x, std::vector<double>(xs, xs + n), std::vector<V3>(ys, ys + n));
 }
 
 }  // namespace sb

@@ -104,4 +104,72 @@ struct Setup {
 // std::runtime_error for malformed input and for settings outside this path.
 Setup parse_setup(const char* json);
 
+// ---- wind and the complete sail model (stormbird/src/wind, controller,
+// lifting_line/complete_sail_model) --------------------------------------------
+struct Spectrum {  // DiscretizedSpectrum (wind_condition/discretized_spectrum.rs:12-17)
+    std::vector<double> frequencies, amplitudes, phases;
+    double value_at_time(double t) const;  // :37-45
+};
+
+struct WindConditionCfg {  // wind_condition/mod.rs:18-29
+    double direction_coming_from = 0.0;
+    int profile = 0;  // VelocityProfile (sb_wind.h)
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+    bool has_obukhov = false;
+    double obukhov = 0.0, stable = 6.0, unstable = 19.3;
+    bool has_gust[3] = {false, false, false};  // parallel, perpendicular, vertical
+    Spectrum gust[3];
+};
+
+struct InflowCorrectionsSingle {  // inflow_corrections.rs:13-19
+    std::vector<double> heights, magnitude, angle;
+    std::vector<std::pair<long, long>> wing_ranges;
+};
+
+struct WindEnvironmentCfg {  // environment.rs:21-34
+    V3 up{0.0, 0.0, 1.0}, axis{0.0, 0.0, 1.0}, zero_dir{1.0, 0.0, 0.0};
+    double water_plane_height = 0.0;
+    bool has_inflow = false;
+    std::vector<double> inflow_directions;
+    std::vector<InflowCorrectionsSingle> inflow;
+};
+
+struct MeasurementCfg {  // controller/measurements.rs:20-30
+    int type = 0;  // 0 Mean, 1 Max, 2 Min
+    long start_index = 0, end_offset = 0;
+};
+
+struct SetPointsCfg {  // controller/set_points.rs:52-68
+    std::vector<double> directions;
+    bool has_aoa = false, has_state = false;
+    std::vector<double> aoa, state;
+    bool spin_ratio = false;  // InternalStateType::SpinRatio
+    double diameter = 0.0, max_rps = 0.0;
+    bool use_effective = false;
+    bool has_max_angle_rate = false, has_max_state_rate = false;
+    double max_angle_rate = 0.0, max_state_rate = 0.0;
+};
+
+struct ControllerCfg {  // controller/builder.rs:14-31
+    std::vector<SetPointsCfg> set_points;
+    MeasurementCfg aoa, direction, velocity;
+    long steps_between_updates = 1;
+    double start_time = 0.0;
+    bool use_input_velocity = false;
+};
+
+struct CompleteSailModelCfg {  // complete_sail_model/builder.rs:14-20
+    Setup sim;
+    WindEnvironmentCfg env;
+    ControllerCfg controller;
+};
+
+CompleteSailModelCfg parse_complete_sail_model(const char* json);
+WindConditionCfg parse_wind_condition(const char* json);
+WindEnvironmentCfg parse_wind_environment(const char* json);
+
+// stormath/src/interpolation.rs:17-100 (clamped piecewise-linear, binary search)
+double interpolate_scalar(double x, const double* xs, const double* ys, size_t n);
+V3 interpolate_vector(double x, const double* xs, const V3* ys, size_t n);
+
 }  // namespace sb

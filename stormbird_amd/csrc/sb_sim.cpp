@@ -40,18 +40,7 @@ sb_sim* sb_sim_create_ex(const char* json, int device, int mode) {
 int sb_setup_info(const char* json, long* out, size_t cap) {
     if (!json || !out) return -1;
     if (cap < 7) return -3;
-    return guarded([&] {
-        Setup s = parse_setup(json);
-        const long N = s.lfm.n(), W = s.lfm.nw(), P = (long)s.lfm.span_points.size(), M = s.wake.M;
-        const long points = s.quasi ? 0 : P * (M + 1);
-        out[0] = N;
-        out[1] = W;
-        out[2] = P;
-        out[3] = M;
-        out[4] = N + points;
-        out[5] = N * M;
-        out[6] = s.quasi ? 1 : 0;
-    });
+    return guarded([&] { setup_counts(parse_setup(json), out); });
 }
 
 sb_sim* sb_sim_create(const char* json) { return sb_sim_create_ex(json, 0, MODE_EXACT); }
@@ -150,6 +139,13 @@ int sb_sim_set_local_wing_angles(sb_sim* s, const double* a, size_t n) {
     if ((long)n != s->W) return -3;
     for (size_t i = 0; i < n; ++i) s->setup.lfm.wing_angles[i] = a[i];
     s->setup.lfm.update_global();
+    return 0;
+}
+
+int sb_sim_get_local_wing_angles(const sb_sim* s, double* out, size_t n) {
+    if (!s || !out) return -1;
+    if ((long)n != s->W) return -3;
+    for (size_t i = 0; i < n; ++i) out[i] = s->setup.lfm.wing_angles[i];
     return 0;
 }
 
@@ -349,6 +345,7 @@ int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, si
             case 2: out[i] = sbm::exp(x[i]); break;
             case 3: out[i] = sbm::acos(x[i]); break;
             case 4: out[i] = sbm::pow(x[i], y[i]); break;
+            case 6: out[i] = sbm::atan(x[i]); break;
             default: out[i] = sbm::log(x[i]); break;
         }
     }

@@ -106,6 +106,15 @@ class SectionalForcesInput:
         self.angles_of_attack = angles_of_attack
 
 
+class SingleSailResult:
+    """results/simplfied.rs:9-15: total force and moment of one sail and its input power."""
+
+    def __init__(self, force, moment, input_power):
+        self.force = force
+        self.moment = moment
+        self.input_power = input_power
+
+
 class SimulationResult:
     """One step's result (same field names as Stormbird's SimulationResult).  The nested
     views (force_input, sectional_forces, per-wing integrated forces) are built on first use."""
@@ -147,7 +156,18 @@ class SimulationResult:
 
     @property
     def input_power(self):
+        """Per-wing input power: zero, as for every sail whose input power model is
+        NoPower (line_force_model/input_power.rs:76-98), the only model this path takes."""
         return [0.0] * self._if.shape[1]
+
+    def input_power_sum(self) -> float:
+        """results/simulation.rs:81-89."""
+        return 0.0
+
+    def as_simplified(self):
+        """Per-wing total force and moment and input power (results/simulation.rs:150-166)."""
+        return [SingleSailResult(self._if[5, w].copy(), self._im[5, w].copy(), 0.0)
+                for w in range(self._if.shape[1])]
 
     def integrated_forces_sum(self) -> np.ndarray:
         """Sum of the per-wing total forces (results/simulation.rs:61-69)."""

@@ -17,7 +17,8 @@ from test_gpu_parity import compare_results
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "setups")
-FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "*.json")))
+# SimulationBuilder strings; csm_*.json hold CompleteSailModelBuilder strings
+FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.json")) if not os.path.basename(p).startswith("csm_"))
 
 
 def heave(t):

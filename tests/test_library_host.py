@@ -19,14 +19,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def header_functions():
-    text = open(os.path.join(ROOT, "include", "stormbird_gpu.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(sb_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in ("stormbird_gpu.h", "cstormbird_compat.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b((?:sb|complete_sail_model)_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_function():
     names = header_functions()
     assert len(names) >= 20
+    assert "complete_sail_model_simulate_steady_state_condition" in names
     lib = ctypes.CDLL(sb.library_path)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
@@ -47,6 +51,7 @@ CASES = {
     "acos_near_one": (3, lambda r, n: (1 - 10 ** r.uniform(-14, -0.3, n), None), 1),
     "pow": (4, lambda r, n: (r.uniform(0, 1, n), r.uniform(0.2, 4, n)), 2),
     "log": (5, lambda r, n: (np.exp(r.uniform(-60, 60, n)), None), 1),
+    "atan": (6, lambda r, n: (np.sinh(r.uniform(-8, 8, n)), None), 2),
 }
 
 
@@ -71,6 +76,9 @@ def test_libm_special_values():
     assert abs(sb.libm_host(3, np.array([-1.0]))[0] - np.pi) < 1e-15
     assert sb.libm_host(2, np.array([0.0]))[0] == 1.0
     assert np.isnan(sb.libm_host(3, np.array([1.5]))[0])
+    a = sb.libm_host(6, np.array([0.0, -0.0, np.inf, -np.inf, 1.0]))
+    assert a[0] == 0.0 and np.signbit(a[1]) and a[2] == np.pi / 2 and a[3] == -np.pi / 2
+    assert a[4] == np.pi / 4
 
 
 @pytest.mark.parametrize("bad,msg", [

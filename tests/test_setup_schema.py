@@ -24,7 +24,8 @@ import stormbird_amd as sb
 from stormbird_amd import setups as S
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "setups")
-FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "*.json")))
+# SimulationBuilder strings; csm_*.json hold CompleteSailModelBuilder strings
+FIXTURES = sorted(p for p in glob.glob(os.path.join(GOLDEN, "*.json")) if not os.path.basename(p).startswith("csm_"))
 
 
 def test_fixtures_exist():
@@ -40,6 +41,22 @@ def test_reference_setup_package_output_loads(oracle, path):
     assert info["nr_freestream_points"] == o.nr_freestream_points()
     assert info["nr_panels"] == o.nr_panels
     assert info["quasi_steady"] == (1 if "QuasiSteady" in json.loads(text)["simulation_settings"] else 0)
+
+
+CSM_FIXTURES = sorted(glob.glob(os.path.join(GOLDEN, "csm_*.json")))
+
+
+@pytest.mark.parametrize("path", CSM_FIXTURES, ids=[os.path.basename(p) for p in CSM_FIXTURES])
+def test_reference_complete_sail_model_output_loads(oracle, path):
+    from stormbird_amd.complete_sail_model import csm_setup_info
+
+    text = open(path).read()
+    d = json.loads(text)
+    info = csm_setup_info(text)
+    o = oracle.Simulation(json.dumps(d["lifting_line_simulation"]), "pm")
+    assert info["nr_ctrl_points"] == o.nr_ctrl_points
+    assert info["nr_freestream_points"] == o.nr_freestream_points()
+    assert info["nr_set_points"] == len(d["controller"]["set_points"])
 
 
 def base():
