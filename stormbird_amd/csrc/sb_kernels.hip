@@ -1902,25 +1902,46 @@ __device__ inline int next_job(unsigned* head, int* slot) {
 
 // Fixed velocities of many instances (frozen_wake.rs:189-261).  Job = one
 // instance's 64 control points (lane = target) over its panels of rows >= 1,
-// run by one 16-wave workgroup (a whole CU): the waves evaluate different
-// panels of each 32-panel block into LDS, then waves 0-2 fold component
-// x/y/z of the block's terms in ascending panel order, so each target sees the
-// same sequence of additions as the one-target fold (a skipped panel adds +0,
-// which leaves a sum that started at +0 unchanged).  Sixteen waves per job
-// keep the longest job (the largest instance) short; jobs are taken longest
-// first from a queue by persistent workgroups.
+// run by one 16-wave workgroup (a whole CU).  Thirteen compute waves evaluate
+// the panels of a 39-panel block (three consecutive panels each) into one
+// half of a double-buffered LDS term array while three fold waves add the
+// previous block's terms, component x/y/z each, in ascending panel order: one
+// barrier per block, and every target sees the same sequence of additions as
+// the one-target fold (a skipped panel adds +0, which leaves a sum that
+// started at +0 unchanged).  Panel records are read with scalar loads (the
+// address is wave-uniform), one panel ahead, so the compute waves use no LDS
+// for them.  Jobs are taken longest first from a queue by persistent
+// workgroups.
 constexpr int COOP_WAVES = 16;
-constexpr int COOP_PB = 2;                        // panels per wave per block
-constexpr int COOP_BLOCK = COOP_WAVES * COOP_PB;  // 32
-constexpr int COOP_TILE = 128;                    // panel records staged per tile (10 KiB)
+constexpr int FX_COMPUTE = 13;                   // compute waves
+constexpr int FX_PER = 3;                        // panels per compute wave per block
+constexpr int FX_BLOCK = FX_COMPUTE * FX_PER;    // 39 panels per block
+typedef const __attribute__((address_space(4))) PanelRec* ScalarRecs;
+
+// Record k read through the constant address space: a wave-uniform k gives
+// scalar loads into SGPRs.
+__device__ inline PanelRec scalar_rec(ScalarRecs recs, long k) {
+    PanelRec o;
+    o.cx = recs[k].cx;
+    o.cy = recs[k].cy;
+    o.cz = recs[k].cz;
+    o.nx = recs[k].nx;
+    o.ny = recs[k].ny;
+    o.nz = recs[k].nz;
+    o.far2 = recs[k].far2;
+    o.area_term = recs[k].area_term;
+    o.strength = recs[k].strength;
+    o.cbase = recs[k].cbase;
+    return o;
+}
 
 template <int MODE, int SYM>
 __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __restrict__ jobs, int n_jobs,
                                                  unsigned* head) {
-    __shared__ double T[COOP_BLOCK][3][WAVE];
-    __shared__ PanelRec tile[COOP_TILE];
+    __shared__ double T[2][FX_BLOCK][3][WAVE];  // 119,808 B
     __shared__ int slot;
-    const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
+    const int lane = threadIdx.x % WAVE;
     for (;;) {
         int j = next_job(head, &slot);
         if (j >= n_jobs) break;
@@ -1930,38 +1951,50 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
         int tc = t < d.N ? t : d.N - 1;
         V3 x = load3(d.geo + d.gl.ctrl + 3 * tc);
         int wt = d.neglect_self ? d.wing_of[tc] : -1;
-        long p_begin = d.N, p_end = (long)d.N * d.nz_rows;
+        const long p_begin = d.N, p_end = (long)d.N * d.nz_rows;
+        const long nblk = p_end > p_begin ? (p_end - p_begin + FX_BLOCK - 1) / FX_BLOCK : 0;
         double acc = 0.0;
-        for (long tb = p_begin; tb < p_end; tb += COOP_TILE) {
-            int tcnt = (int)(p_end - tb < COOP_TILE ? p_end - tb : COOP_TILE);
-            __syncthreads();
-            const double* src = reinterpret_cast<const double*>(d.rec + tb);
-            double* dst = reinterpret_cast<double*>(tile);
-            for (int q = threadIdx.x; q < tcnt * RECD; q += blockDim.x) dst[q] = src[q];
-            __syncthreads();
-            for (int base = 0; base < tcnt; base += COOP_BLOCK) {
-                for (int q = 0; q < COOP_PB; ++q) {
-                    int kl = base + wave * COOP_PB + q;
-                    V3 term{0.0, 0.0, 0.0};
-                    if (kl < tcnt) {
-                        const PanelRec& r = tile[kl];
-                        int kk = (int)(tb + kl);
-                        bool skip = r.strength == 0.0 || (d.neglect_self && d.wing_of[kk % d.N] == wt);
-                        if (!skip) term = pair_unit<MODE, SYM>(d, r, kk, x) * r.strength;
+        if (wave < FX_COMPUTE) {
+            ScalarRecs recs = (ScalarRecs)d.rec;
+            const long mine = p_begin + (long)wave * FX_PER;  // first panel of this wave in block 0
+            PanelRec nxt;
+            if (nblk > 0) nxt = scalar_rec(recs, mine < p_end ? mine : p_end - 1);
+            for (long b = 0; b <= nblk; ++b) {
+                if (b < nblk) {
+                    const int buf = (int)(b & 1);
+#pragma unroll
+                    for (int q = 0; q < FX_PER; ++q) {
+                        const long k = mine + b * FX_BLOCK + q;
+                        const PanelRec r = nxt;
+                        // next panel of this wave: q + 1 here, or the first of the next block
+                        long kn = q + 1 < FX_PER ? k + 1 : mine + (b + 1) * FX_BLOCK;
+                        nxt = scalar_rec(recs, kn < p_end ? kn : p_end - 1);
+                        V3 term{0.0, 0.0, 0.0};
+                        if (k < p_end) {
+                            bool skip = r.strength == 0.0 || (d.neglect_self && d.wing_of[(int)(k % d.N)] == wt);
+                            if (!skip) term = pair_unit<MODE, SYM>(d, r, (int)k, x) * r.strength;
+                        }
+                        const int kl = wave * FX_PER + q;
+                        T[buf][kl][0][lane] = term.x;
+                        T[buf][kl][1][lane] = term.y;
+                        T[buf][kl][2][lane] = term.z;
                     }
-                    T[wave * COOP_PB + q][0][lane] = term.x;
-                    T[wave * COOP_PB + q][1][lane] = term.y;
-                    T[wave * COOP_PB + q][2][lane] = term.z;
-                }
-                __syncthreads();
-                if (wave < 3) {
-                    int cnt = tcnt - base < COOP_BLOCK ? tcnt - base : COOP_BLOCK;
-                    for (int q = 0; q < cnt; ++q) acc = acc + T[q][wave][lane];
                 }
                 __syncthreads();
             }
+        } else {
+            const int c = wave - FX_COMPUTE;
+            for (long b = 0; b <= nblk; ++b) {
+                if (b > 0) {
+                    const int buf = (int)((b - 1) & 1);
+                    const long base = p_begin + (b - 1) * FX_BLOCK;
+                    const int cnt = (int)(p_end - base < FX_BLOCK ? p_end - base : FX_BLOCK);
+                    for (int q = 0; q < cnt; ++q) acc = acc + T[buf][q][c][lane];
+                }
+                __syncthreads();
+            }
+            if (t < d.N) d.fixed[3 * t + c] = acc;
         }
-        if (wave < 3 && t < d.N) d.fixed[3 * t + wave] = acc;
     }
 }
 
