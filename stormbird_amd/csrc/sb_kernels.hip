@@ -225,6 +225,58 @@ __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
     }
 }
 
+// The point-doublet branch of panel_unit without control flow, so that several
+// panels' evaluations interleave: returns true when the lane takes that branch
+// with operands in the range of the fast quotient sequence, and then `out`
+// equals panel_unit's result bit for bit (same operations, same order).  Other
+// lanes (near field, or operands outside the range) must call panel_unit.
+template <int MODE>
+__device__ inline bool far_fast(const PanelRec& r, V3 x, V3& out) {
+    double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
+    if (MODE == MODE_EXACT) {
+        double d2 = tx * tx + ty * ty + tz * tz;
+        double nh = tx * r.nx + ty * r.ny + tz * r.nz;
+        double h3 = 3.0 * nh;
+        V3 num{(tx * h3 - r.nx * d2) * r.area_term, (ty * h3 - r.ny * d2) * r.area_term,
+               (tz * h3 - r.nz * d2) * r.area_term};
+        double mx = fmax(fmax(fabs(num.x), fabs(num.y)), fabs(num.z));
+        double mn = fmin(fmin(fabs(num.x), fabs(num.y)), fabs(num.z));
+        bool ok = d2 > r.far2 && d2 >= 0x1p-120 && d2 <= 0x1p120 && mx <= 0x1p400 && mn >= 0x1p-600;
+        double y = __builtin_amdgcn_rsq(d2);
+        double h = d2 * y;
+        double g = y * 0.5;
+        double e = fma(-g, h, 0.5);
+        h = fma(h, e, h);
+        double rr = fma(-h, h, d2);
+        g = fma(g, e, g);
+        h = fma(rr, g, h);
+        rr = fma(-h, h, d2);
+        h = fma(rr, g, h);
+        double den = d2 * d2 * h;
+        double v = __builtin_amdgcn_rcp(den);
+        double t = fma(-den, v, 1.0);
+        v = fma(v, t, v);
+        t = fma(-den, v, 1.0);
+        v = fma(v, t, v);
+        double qx = num.x * v, qy = num.y * v, qz = num.z * v;
+        double ex = fma(-den, qx, num.x), ey = fma(-den, qy, num.y), ez = fma(-den, qz, num.z);
+        out = V3{fma(ex, v, qx), fma(ey, v, qy), fma(ez, v, qz)};
+        return ok;
+    } else {
+        double d2 = fma(tx, tx, fma(ty, ty, tz * tz));
+        double y = __builtin_amdgcn_rsq(d2);
+        double e = fma(-d2 * y, y, 1.0);
+        y = fma(0.5 * y, e, y);
+        double y2 = y * y;
+        double inv5 = (y2 * y2) * y;
+        double nh = fma(tx, r.nx, fma(ty, r.ny, tz * r.nz));
+        double h3 = 3.0 * nh;
+        double s = r.area_term * inv5;
+        out = V3{fma(tx, h3, -r.nx * d2) * s, fma(ty, h3, -r.ny * d2) * s, fma(tz, h3, -r.nz * d2) * s};
+        return d2 > r.far2;
+    }
+}
+
 // induced_velocity_calc.rs:182-198 with symmetry_condition.rs:27-43
 // The mirror flips one coordinate (multiplying by -1 is exact) and the
 // correction adds s*u_m component-wise, which equals the reference's
@@ -1956,29 +2008,47 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
         double acc = 0.0;
         if (wave < FX_COMPUTE) {
             ScalarRecs recs = (ScalarRecs)d.rec;
-            const long mine = p_begin + (long)wave * FX_PER;  // first panel of this wave in block 0
-            PanelRec nxt;
-            if (nblk > 0) nxt = scalar_rec(recs, mine < p_end ? mine : p_end - 1);
-            for (long b = 0; b <= nblk; ++b) {
-                if (b < nblk) {
-                    const int buf = (int)(b & 1);
+            const int pb = (int)p_begin, pe = (int)p_end;
+            const int mine = pb + wave * FX_PER;  // first panel of this wave in block 0
+            PanelRec r[FX_PER];
+            if (nblk > 0) {
+#pragma unroll
+                for (int q = 0; q < FX_PER; ++q) r[q] = scalar_rec(recs, mine + q < pe ? mine + q : pe - 1);
+            }
+            for (int b = 0; b <= (int)nblk; ++b) {
+                if (b < (int)nblk) {
+                    const int buf = b & 1;
+                    const int k0 = mine + b * FX_BLOCK;
+                    V3 term[FX_PER];
+                    if constexpr (SYM == 0 && MODE != MODE_FP32_KAHAN) {
+                        // far-field terms of the three panels, interleaved; then the
+                        // lanes that need the general evaluation
+                        bool ok[FX_PER];
+#pragma unroll
+                        for (int q = 0; q < FX_PER; ++q) ok[q] = far_fast<MODE>(r[q], x, term[q]);
+#pragma unroll
+                        for (int q = 0; q < FX_PER; ++q)
+                            if (!ok[q]) term[q] = panel_unit<MODE>(d, r[q], k0 + q < pe ? k0 + q : pe - 1, x);
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < FX_PER; ++q)
+                            term[q] = pair_unit<MODE, SYM>(d, r[q], k0 + q < pe ? k0 + q : pe - 1, x);
+                    }
 #pragma unroll
                     for (int q = 0; q < FX_PER; ++q) {
-                        const long k = mine + b * FX_BLOCK + q;
-                        const PanelRec r = nxt;
-                        // next panel of this wave: q + 1 here, or the first of the next block
-                        long kn = q + 1 < FX_PER ? k + 1 : mine + (b + 1) * FX_BLOCK;
-                        nxt = scalar_rec(recs, kn < p_end ? kn : p_end - 1);
-                        V3 term{0.0, 0.0, 0.0};
-                        if (k < p_end) {
-                            bool skip = r.strength == 0.0 || (d.neglect_self && d.wing_of[(int)(k % d.N)] == wt);
-                            if (!skip) term = pair_unit<MODE, SYM>(d, r, (int)k, x) * r.strength;
-                        }
+                        const int k = k0 + q;
+                        bool skip = k >= pe || r[q].strength == 0.0 ||
+                                    (d.neglect_self && d.wing_of[k % d.N] == wt);
+                        V3 u = skip ? V3{0.0, 0.0, 0.0} : term[q] * r[q].strength;
                         const int kl = wave * FX_PER + q;
-                        T[buf][kl][0][lane] = term.x;
-                        T[buf][kl][1][lane] = term.y;
-                        T[buf][kl][2][lane] = term.z;
+                        T[buf][kl][0][lane] = u.x;
+                        T[buf][kl][1][lane] = u.y;
+                        T[buf][kl][2][lane] = u.z;
                     }
+                    // records of the next block, loaded across the barrier
+                    const int kn = k0 + FX_BLOCK;
+#pragma unroll
+                    for (int q = 0; q < FX_PER; ++q) r[q] = scalar_rec(recs, kn + q < pe ? kn + q : pe - 1);
                 }
                 __syncthreads();
             }
