@@ -1954,20 +1954,21 @@ __device__ inline int next_job(unsigned* head, int* slot) {
 
 // Fixed velocities of many instances (frozen_wake.rs:189-261).  Job = one
 // instance's 64 control points (lane = target) over its panels of rows >= 1,
-// run by one 16-wave workgroup (a whole CU).  Thirteen compute waves evaluate
-// the panels of a 39-panel block (three consecutive panels each) into one
-// half of a double-buffered LDS term array while three fold waves add the
-// previous block's terms, component x/y/z each, in ascending panel order: one
-// barrier per block, and every target sees the same sequence of additions as
-// the one-target fold (a skipped panel adds +0, which leaves a sum that
+// run by one 16-wave workgroup (a whole CU).  Fifteen compute waves evaluate
+// the panels of a 45-panel block (three consecutive panels each) into one
+// half of a double-buffered LDS term array while the last wave adds the
+// previous block's terms, component by component, in ascending panel order:
+// one barrier per block, and every target sees the same sequence of additions
+// as the one-target fold (a skipped panel adds +0, which leaves a sum that
 // started at +0 unchanged).  Panel records are read with scalar loads (the
-// address is wave-uniform), one panel ahead, so the compute waves use no LDS
-// for them.  Jobs are taken longest first from a queue by persistent
-// workgroups.
+// address is wave-uniform), the next block's across the barrier, so the
+// compute waves use no LDS for them.  Jobs are taken longest first from a
+// queue by persistent workgroups.
 constexpr int COOP_WAVES = 16;
-constexpr int FX_COMPUTE = 13;                   // compute waves
+constexpr int FX_COMPUTE = 15;                   // compute waves; wave 15 folds
 constexpr int FX_PER = 3;                        // panels per compute wave per block
-constexpr int FX_BLOCK = FX_COMPUTE * FX_PER;    // 39 panels per block
+constexpr int FX_BLOCK = FX_COMPUTE * FX_PER;    // 45 panels per block
+static_assert(FX_BLOCK % 9 == 0, "the fold reads full blocks nine panels at a time");
 typedef const __attribute__((address_space(4))) PanelRec* ScalarRecs;
 
 // Record k read through the constant address space: a wave-uniform k gives
@@ -1990,7 +1991,7 @@ __device__ inline PanelRec scalar_rec(ScalarRecs recs, long k) {
 template <int MODE, int SYM>
 __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __restrict__ jobs, int n_jobs,
                                                  unsigned* head) {
-    __shared__ double T[2][FX_BLOCK][3][WAVE];  // 119,808 B
+    __shared__ double T[2][FX_BLOCK][3][WAVE];  // 138,240 B
     __shared__ int slot;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int lane = threadIdx.x % WAVE;
@@ -2053,17 +2054,41 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
                 __syncthreads();
             }
         } else {
-            const int c = wave - FX_COMPUTE;
+            double ay = 0.0, az = 0.0;
             for (long b = 0; b <= nblk; ++b) {
                 if (b > 0) {
                     const int buf = (int)((b - 1) & 1);
                     const long base = p_begin + (b - 1) * FX_BLOCK;
                     const int cnt = (int)(p_end - base < FX_BLOCK ? p_end - base : FX_BLOCK);
-                    for (int q = 0; q < cnt; ++q) acc = acc + T[buf][q][c][lane];
+                    if (cnt == FX_BLOCK) {
+                        // a full block: loads issued nine panels ahead of the adds
+#pragma unroll
+                        for (int q0 = 0; q0 < FX_BLOCK; q0 += 9) {
+                            double vx[9], vy[9], vz[9];
+#pragma unroll
+                            for (int i = 0; i < 9; ++i) {
+                                vx[i] = T[buf][q0 + i][0][lane];
+                                vy[i] = T[buf][q0 + i][1][lane];
+                                vz[i] = T[buf][q0 + i][2][lane];
+                            }
+#pragma unroll
+                            for (int i = 0; i < 9; ++i) {
+                                acc = acc + vx[i];
+                                ay = ay + vy[i];
+                                az = az + vz[i];
+                            }
+                        }
+                    } else {
+                        for (int q = 0; q < cnt; ++q) {
+                            acc = acc + T[buf][q][0][lane];
+                            ay = ay + T[buf][q][1][lane];
+                            az = az + T[buf][q][2][lane];
+                        }
+                    }
                 }
                 __syncthreads();
             }
-            if (t < d.N) d.fixed[3 * t + c] = acc;
+            if (t < d.N) store3(d.fixed + 3 * t, V3{acc, ay, az});
         }
     }
 }
