@@ -499,6 +499,77 @@ def quasi_steady(steps=50):
                           "device time is the sum of the step's kernel intervals"}
 
 
+def complete_sail_model_section(n_directions=36, n_loadings=5, steps=3):
+    """CompleteSailModel (complete_sail_model/mod.rs): a polar of optimal-loading steady
+    states for the reference's speed-test sails (quasi-steady, 3 x 40 strips), GPU beside the
+    CPU restatement on one thread; and dynamic steps of the four-sail free wake (2,000 rows)
+    driven by a power-law wind with gusts, the wake-point freestream evaluated on the device,
+    beside the plain step with a device-resident freestream."""
+    import torch
+
+    import stormbird_amd as sb
+    from stormbird_amd import setups as S
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_csm
+
+    gust = {"frequencies": [0.05, 0.12, 0.31], "amplitudes": [0.6, 0.3, 0.15], "phases": [0.2, 1.9, 4.0]}
+    conditions = [{"direction_coming_from": math.radians(-180.0 + 360.0 * i / n_directions),
+                   "velocity_variation": {"PowerModel": {"reference_velocity": 9.0}}}
+                  for i in range(n_directions)]
+    js = S.complete_sail_model(S.wing_sails_quasi_steady(), 3)
+    g = sb.CompleteSailModel(js)
+    g.simulate_condition_optimal_controller_loading(wind_condition=sb.WindCondition(conditions[0]), ship_velocity=6.0,
+                                                    nr_loadings_to_test=n_loadings)
+    t0 = time.perf_counter()
+    for c in conditions:
+        g.simulate_condition_optimal_controller_loading(wind_condition=sb.WindCondition(c), ship_velocity=6.0,
+                                                        nr_loadings_to_test=n_loadings)
+    t_gpu = (time.perf_counter() - t0) / len(conditions)
+    o = oracle_csm.CompleteSailModel(js, "glibc")
+    sample = conditions[::6]
+    t0 = time.perf_counter()
+    for c in sample:
+        o.optimal_loading(c, 6.0, n_loadings)
+    t_cpu = (time.perf_counter() - t0) / len(sample)
+    polar = {"workload": (f"3 wing sails x 40 strips (quasi-steady, Linearized), {n_directions} wind directions, "
+                          f"power-law wind 9 m/s at 10 m, ship 6 m/s, optimal loading of {n_loadings}"),
+             "gpu_ms_per_condition": 1e3 * t_gpu, "cpu_ms_per_condition": 1e3 * t_cpu,
+             "cpu": f"restatement (Python wind and controller, C++ lifting line), 1 thread, {len(sample)} conditions",
+             "gpu_vs_cpu": t_cpu / t_gpu}
+    del g, o
+
+    js4, case = S.four_sail_free_wake()
+    dt = case["time_step"]
+    wc = {"direction_coming_from": math.radians(case["awa_deg"]),
+          "velocity_variation": {"PowerModel": {"reference_velocity": 12.0}},
+          "parallel_gust": gust, "perpendicular_gust": gust}
+    m = sb.CompleteSailModel(S.complete_sail_model(js4, 4))
+    sim = m.simulation
+    pts, vel, gam, prev, fs = synthesize_developed_state(sim, case)
+    sim.set_wake_state(pts, vel, gam, 2000)
+    sim.previous_circulation_strength = prev
+    w = sb.WindCondition(wc)
+    m.do_step(time=2000 * dt, time_step=dt, wind_condition=w, ship_velocity=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        m.do_step(time=(2001 + k) * dt, time_step=dt, wind_condition=w, ship_velocity=0.0)
+    torch.cuda.synchronize()
+    t_csm = (time.perf_counter() - t0) / steps
+    n_points = sim.nr_freestream_points()
+    fs_dev = torch.from_numpy(np.ascontiguousarray(fs)).to("cuda")
+    e, _ = time_gpu(sim, fs_dev.data_ptr(), n_points, dt, steps, 1, 2010)
+    t_plain = e / steps
+    dynamic = {"workload": ("four 30x10 m wing sails, 40 strips each, free wake of 2000 rows (328164 wake "
+                            "points), power-law wind 12 m/s at 10 m with gusts along and across, developed wake"),
+               "csm_ms_per_step": 1e3 * t_csm, "plain_device_freestream_ms_per_step": 1e3 * t_plain,
+               "wind_evaluation_overhead_ms": 1e3 * (t_csm - t_plain),
+               "freestream": "control points on the host (inflow corrections, controller), wake points on the device"}
+    del m, sim
+    return {"optimal_loading_polar": polar, "dynamic_steps": dynamic}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -515,7 +586,8 @@ def main():
     ap.add_argument("--route-ships", type=int, default=64)
     ap.add_argument("--train-conditions", type=int, default=16, help="inflow conditions per instance for training")
     ap.add_argument("--no-quasi", action="store_true")
-    ap.add_argument("--only", choices=("batched", "warm", "route", "quasi"), default=None,
+    ap.add_argument("--no-csm", action="store_true")
+    ap.add_argument("--only", choices=("batched", "warm", "route", "quasi", "csm"), default=None,
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -539,6 +611,8 @@ def main():
             out, _ = warm_start_polar(n_inst=args.batch_instances, train_conditions=args.train_conditions)
         elif args.only == "quasi":
             out = quasi_steady()
+        elif args.only == "csm":
+            out = complete_sail_model_section()
         else:
             out = route_simulation(n_ships=args.route_ships, steps=args.steps)
         print(json.dumps({"section": args.only, **out}))
@@ -653,6 +727,8 @@ def main():
     mlp = None
     if not args.no_quasi:
         line["quasi_steady"] = quasi_steady()
+    if not args.no_csm:
+        line["complete_sail_model"] = complete_sail_model_section()
     if not args.no_batch:
         del sim
         line["batched"] = batched_polar_sweep(n_inst=args.batch_instances)

@@ -195,6 +195,25 @@ def four_sail_free_wake(nr_panels=2000, nr_strips=40, seed=1, ratio=1.0, alpha_d
     return json.dumps(setup), case
 
 
+def complete_sail_model(setup_json: str, n_sails: int, environment=None, max_angle_deg=15.0) -> str:
+    """CompleteSailModelBuilder JSON (complete_sail_model/builder.rs:14-20) around a
+    SimulationBuilder string: the default wind environment (or `environment`) and, per sail,
+    the default single-element wing-sail set points of the reference's setup package
+    (stormbird_setup/controller.py: angle of attack +-max_angle_deg beyond 15 deg of apparent
+    wind, zero within 10 deg) with its measurement defaults (start_index 1, end_offset 1)."""
+    dirs = [math.radians(a) for a in (-180.0, -15.0, -10.0, 10.0, 15.0, 180.0)]
+    a = math.radians(max_angle_deg)
+    set_point = {"apparent_wind_directions_data": dirs, "angle_of_attack_data": [-a, -a, 0.0, 0.0, a, a]}
+    measure = {"measurement_type": "Mean", "start_index": 1, "end_offset": 1}
+    return json.dumps({
+        "lifting_line_simulation": json.loads(setup_json),
+        "wind_environment": environment if environment is not None else {},
+        "controller": {"set_points": [set_point] * int(n_sails),
+                       "flow_measurement_settings": {"angle_of_attack": measure, "wind_direction": measure,
+                                                     "wind_velocity": measure}},
+    })
+
+
 def polar_sweep_instance(rng: np.random.Generator, max_panels=3000, min_panels=200):
     """One instance of the batched polar sweep.
 
