@@ -91,6 +91,31 @@ def test_steps_with_controller_match_oracle(oc):
     assert any(abs(a) > 0.01 for a in g.local_wing_angles())
 
 
+def test_controller_on_a_moving_ship_matches_oracle(oc):
+    """Translation and roll/pitch/yaw of the ship: rotated chord vectors in the apparent wind
+    direction and the rotational velocity removed from the measured velocity
+    (results/simulation.rs:118-149, environment.rs:297-327)."""
+    g, o = pair(oc, DYNAMIC)
+    wc = conditions()[1]
+    dt = 0.5
+    for k in range(5):
+        t = k * dt
+        tr = [0.3 * k, 0.1 * k, 0.0]
+        rot = [0.02 * k, -0.01 * k, 0.05 * k]
+        g.simulation.set_translation_and_rotation_with_finite_difference_for_the_velocity(dt, tr, rot)
+        o.sim.set_translation_and_rotation_with_finite_difference_for_the_velocity(dt, tr, rot)
+        g.apply_controller_based_on_wind_condition(time=t, time_step=dt, wind_condition=WindCondition(wc),
+                                                   ship_velocity=4.0, controller_loading=0.9)
+        o.apply_controller_wind(t, dt, wc, 4.0, 0.9)
+        same_angles(g, o)
+        rg = g.do_step(time=t, time_step=dt, wind_condition=WindCondition(wc), ship_velocity=4.0)
+        ro = o.do_step(t, dt, wc, 4.0)
+        compare_results(rg, ro, where=f"moving step {k}")
+        g.apply_controller_based_on_simulation_result(time=t, time_step=dt, loading=0.9, simulation_result=rg)
+        o.apply_controller_result(t, dt, 0.9, ro)
+        same_angles(g, o)
+
+
 @pytest.mark.parametrize("name", [QUASI, DYNAMIC])
 def test_steady_state_and_optimal_loading_match_oracle(oc, name):
     g, o = pair(oc, name)
