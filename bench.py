@@ -167,7 +167,12 @@ def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     k0 += plain_steps
-    # batched: first call captures the step graph
+    # batched: one step on the developed state, the autotuner on it, then the first call
+    # with the chosen variants captures the step graph
+    batch.do_step_device(times=k0 * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(), n_points=n_pts,
+                         want_result=False)
+    k0 += 1
+    _, tuning, tune_ms = batch.autotune(None)
     batch.do_step_device(times=k0 * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(), n_points=n_pts,
                          want_result=False)
     k0 += 1
@@ -199,6 +204,8 @@ def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
         "steps": steps,
         "plain_steps": plain_steps,
         "create_s": t_create,
+        "autotuner": {"fixed_per": tuning[0], "targets_tile": tuning[1],
+                      "fixed_ms_per_panels_per_wave_1_2_3": [float(x) for x in tune_ms[:3]]},
         "warm_start": None,
     }
 
@@ -365,7 +372,10 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
         n, P = s.nr_ctrl_points, s.nr_span_points
         M = s.nr_panels // n
         pairs += float(P * (M + 1)) * float(n * M) + float(n * n * (M - 1)) + float(n * n)
-    route.step(want_result=False)  # captures the step graph
+    route.step(want_result=False)
+    # the autotuner on a sample of the queues (one resident wave of jobs per variant)
+    _, tuning, tune_ms = route.batch.autotune(None, reps=1, max_jobs=1024)
+    route.step(want_result=False)  # captures the step graph with the chosen variants
     torch.cuda.synchronize()
     dev_ms, iters = [], []
     t0 = time.perf_counter()
@@ -390,6 +400,9 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
         "iterations_mean": float(np.mean(iters)),
         "setup_s": t_setup,
         "freestream": "resident in device memory, refilled per ship per step (inside the timed step)",
+        "autotuner": {"fixed_per": tuning[0], "targets_tile": tuning[1],
+                      "sample_ms": {"fixed_1_2_3": [float(x) for x in tune_ms[:3]],
+                                    "targets_64_128_256": [float(x) for x in tune_ms[3:]]}},
     }
     if mlp is not None:
         route.batch.set_warm_start(mlp, 3)

@@ -230,6 +230,27 @@ double sb_batch_last_step_ms(const sb_batch* b);
 /* sb_sim_reset on every instance. */
 int sb_batch_reset(sb_batch* b);
 
+/* Kernel variants of the batched pair sums (no counterpart in the reference):
+ * fixed_per = panels per compute wave per block of the fixed-velocity kernel
+ * (1, 2, 3), targets_tile = panel records per LDS tile of the self-induction
+ * kernel (64, 128, 256).  Results do not depend on them (every target keeps its
+ * ascending panel order). */
+int sb_batch_set_tuning(sb_batch* b, int fixed_per, int targets_tile);
+int sb_batch_get_tuning(const sb_batch* b, int* fixed_per, int* targets_tile);
+/* Autotuner: times every variant on the batch's last-step state (a step must
+ * have been taken; the state is left unchanged) and applies the fastest.
+ * flags bit 0: use a choice stored in cache_path for the same device and shape
+ * class (instance count, pair counts and largest strip count, in powers of
+ * two) without timing; bit 1: store the result in cache_path (a JSON file).
+ * Each variant runs once untimed, then `reps` timed launches (the fastest
+ * counts); max_jobs > 0 times only the first max_jobs jobs of each kernel's
+ * queue (the fixed-velocity queue is longest first).  timings [6] (may be
+ * NULL): ms of the fixed-velocity kernel for 1, 2, 3 panels per wave, then of
+ * the self-induction kernel for tiles of 64, 128, 256 (-1: no work).  Returns
+ * 1 when the choice came from the cache, 0 when timed. */
+int sb_batch_autotune(sb_batch* b, const char* cache_path, int flags, int reps, long max_jobs, double* timings,
+                      size_t cap);
+
 /* ---- Learned warm start ---------------------------------------------------
  * A per-strip network (64 -> 256 -> 256 -> 256 -> 256 -> 1, GELU; bf16 matrix
  * -core GEMMs with fp32 accumulation, fp32 master weights, Adam) predicts

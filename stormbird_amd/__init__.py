@@ -118,6 +118,10 @@ def lib():
         L.sb_batch_last_fallbacks.argtypes = [vp]
         L.sb_batch_last_warm_taken.restype = ctypes.c_long
         L.sb_batch_last_warm_taken.argtypes = [vp]
+        L.sb_batch_set_tuning.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.sb_batch_get_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.sb_batch_autotune.argtypes = [vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_long, _dp,
+                                        ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -478,6 +482,33 @@ class Batch:
         rc = lib().sb_batch_reset(self._h)
         if rc != 0:
             _err("sb_batch_reset", rc)
+
+    def set_tuning(self, fixed_per: int, targets_tile: int):
+        """Kernel variants of the batched pair sums: panels per compute wave per block of the
+        fixed-velocity kernel (1, 2, 3) and panel records per LDS tile of the self-induction
+        kernel (64, 128, 256).  Results do not depend on them."""
+        rc = lib().sb_batch_set_tuning(self._h, int(fixed_per), int(targets_tile))
+        if rc != 0:
+            _err("sb_batch_set_tuning", rc)
+
+    def get_tuning(self):
+        p, t = ctypes.c_int(), ctypes.c_int()
+        lib().sb_batch_get_tuning(self._h, ctypes.byref(p), ctypes.byref(t))
+        return p.value, t.value
+
+    def autotune(self, cache_path=None, use_cache=True, write_cache=True, reps=3, max_jobs=0):
+        """Times every variant on the batch's last-step state (take a step first; the state is
+        left unchanged) and applies the fastest.  With a cache file, a stored choice for the
+        same device and shape class is applied without timing; `max_jobs` > 0 times only the first
+        jobs of each kernel's queue (a sample, for very long steps).  Returns
+        (cached, (fixed_per, targets_tile), timings_ms [6])."""
+        t = np.zeros(6)
+        flags = (1 if use_cache else 0) | (2 if write_cache else 0)
+        path = cache_path.encode() if cache_path else None
+        rc = lib().sb_batch_autotune(self._h, path, flags, int(reps), int(max_jobs), _p(t), 6)
+        if rc < 0:
+            _err("sb_batch_autotune", rc)
+        return bool(rc == 1), self.get_tuning(), t
 
     def set_warm_start(self, mlp, mode: int = 1):
         """mode 0 off, 1 first step, 2 every step, 3 every step judged by the solver's first

@@ -19,11 +19,16 @@
 // prepare, horseshoe factors, one-workgroup Linearized solve (and/or the
 // fixed-point solve), forces.
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
 #include <exception>
+#include <fstream>
 #include <numeric>
+#include <sstream>
 #include <thread>
 
 #include "sb_host.h"
+#include "sb_json.h"
 
 namespace {
 
@@ -102,6 +107,9 @@ struct sb_batch {
     int lin_variant = 0;
     size_t lin_shm = 0;
     double *d_singular = nullptr, *h_singular = nullptr;  // [n]
+    // work of the last dynamic step, for the tuning key: pairs of the fixed-velocity and
+    // self-induction sums as the kernels evaluate them
+    double fixed_work = 0.0, target_work = 0.0;
 
     ~sb_batch() {
         (void)hipSetDevice(device);
@@ -223,9 +231,7 @@ struct sb_batch {
         bd.sym = sym;
         bd.fixed_jobs = d_jobs;
         bd.target_jobs = d_jobs + cap_fixed_jobs;
-        bd.fixed_grid = batch_resident_groups(0, mode, sym);
-        bd.target_grid = batch_resident_groups(1, mode, sym);
-        if (bd.fixed_grid <= 0 || bd.target_grid <= 0) throw HipError("occupancy query failed", -1);
+        set_tuning(bd.fixed_per, bd.targets_tile);
         const char* g = getenv("SB_BATCH_GRAPH");
         use_graph = !(g && strcmp(g, "0") == 0);
         // per-strip arrays of the warm start
@@ -319,6 +325,13 @@ struct sb_batch {
         }
         bd.n_fixed_jobs = (int)fj.size();
         bd.n_target_jobs = (int)nt;
+        fixed_work = 0.0;
+        for (const auto& f : fj) fixed_work += 64.0 * (double)f.first;
+        target_work = 0.0;
+        for (long k = 0; k < nt; ++k) {
+            const Inst& d = h_inst[3 * n + h_jobs[cap_fixed_jobs + k].x];
+            target_work += 256.0 * (double)d.N * (double)d.nz_rows;
+        }
 
         // freestream into the slab (outside the graph: the caller's pointer may change)
         check(hipMemcpyAsync(d_fs, fs, sizeof(double) * fs_total,
@@ -433,6 +446,138 @@ struct sb_batch {
     }
 
     // The launches of one step (captured into the graph after the first step).
+    // Kernel variants (results do not depend on them: every target's sum keeps its
+    // ascending panel order); the captured step graph is rebuilt at the next step.
+    void set_tuning(int fixed_per, int targets_tile) {
+        if (fixed_per < 1 || fixed_per > 3) throw std::invalid_argument("fixed_per must be 1, 2 or 3");
+        if (targets_tile != 64 && targets_tile != 128 && targets_tile != 256)
+            throw std::invalid_argument("targets_tile must be 64, 128 or 256");
+        check(hipSetDevice(device), "hipSetDevice");
+        int fg = batch_resident_groups(0, mode, bd.sym, fixed_per);
+        int tg = batch_resident_groups(1, mode, bd.sym, targets_tile);
+        if (fg <= 0 || tg <= 0) throw HipError("occupancy query failed", -1);
+        bd.fixed_per = fixed_per;
+        bd.targets_tile = targets_tile;
+        bd.fixed_grid = fg;
+        bd.target_grid = tg;
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        graph_sig[0] = graph_sig[1] = graph_sig[2] = -1;
+    }
+
+    // Key of the tuning cache: the device, the arithmetic, and the shape class of the
+    // last step's work (powers of two of the instance count and of both pair sums, the
+    // largest strip count in units of 64).
+    std::string tuning_key() const {
+        hipDeviceProp_t prop;
+        check(hipGetDeviceProperties(&prop, device), "device properties");
+        int max_n = 0;
+        for (auto& s : sims) max_n = std::max(max_n, s->N);
+        auto lg = [](double x) { return x >= 1.0 ? (int)std::floor(std::log2(x)) : -1; };
+        std::ostringstream k;
+        k << prop.gcnArchName << "|" << prop.name << "|" << (mode == MODE_FAST ? "fast" : "exact") << "|sym"
+          << bd.sym << "|n" << lg((double)n) << "|f" << lg(fixed_work) << "|t" << lg(target_work) << "|N"
+          << (max_n + 63) / 64;
+        return k.str();
+    }
+
+    // Times every variant of the batched pair kernels on the batch's own last-step state
+    // (outputs saved and restored, so the state is unchanged) and applies the fastest.
+    // timings: kb_fixed for 1, 2, 3 panels per wave, then kb_targets for 64, 128, 256
+    // records per tile (ms, -1 where a kernel had no work).
+    void autotune(double* timings, int reps, long max_jobs) {
+        if (quasi) throw std::invalid_argument("the autotuner covers dynamic batches");
+        if (!first_done) throw std::invalid_argument("take a step before tuning: the jobs come from the last step");
+        check(hipSetDevice(device), "hipSetDevice");
+        long nf = 0, nv = 0;
+        for (auto& s : sims) {
+            nf += 3L * s->N;
+            nv += 3L * s->nr_points;
+        }
+        const bool targets = bd.n_target_jobs > 0;
+        double* save = dalloc<double>(nf + (targets ? nv : 0));
+        auto copy_all = [&](bool to_save) {
+            long o = 0;
+            for (auto& s : sims) {
+                size_t bytes = sizeof(double) * 3 * s->N;
+                check(hipMemcpyAsync(to_save ? save + o : s->d_fixed, to_save ? s->d_fixed : save + o, bytes,
+                                     hipMemcpyDeviceToDevice, stream),
+                      "tuning copy");
+                o += 3L * s->N;
+            }
+            if (!targets) return;
+            for (auto& s : sims) {
+                size_t bytes = sizeof(double) * 3 * s->nr_points;
+                check(hipMemcpyAsync(to_save ? save + o : s->d_vel, to_save ? s->d_vel : save + o, bytes,
+                                     hipMemcpyDeviceToDevice, stream),
+                      "tuning copy");
+                o += 3L * s->nr_points;
+            }
+        };
+        const int prev_per = bd.fixed_per, prev_tile = bd.targets_tile;
+        int best_per = prev_per, best_tile = prev_tile;
+        try {
+            copy_all(true);
+            auto time_launch = [&](auto&& launch) {
+                double best = 1e300;
+                for (int r = 0; r <= reps; ++r) {  // the first launch loads the code object
+                    check(hipEventRecord(ev[0], stream), "event");
+                    launch();
+                    check(hipEventRecord(ev[1], stream), "event");
+                    check(hipStreamSynchronize(stream), "tuning");
+                    float ms = 0.f;
+                    check(hipEventElapsedTime(&ms, ev[0], ev[1]), "event time");
+                    if (r > 0) best = std::min(best, (double)ms);
+                }
+                return best;
+            };
+            const int pers[3] = {1, 2, 3}, tiles[3] = {64, 128, 256};
+            double tbest = 1e300;
+            for (int v = 0; v < 3; ++v) {
+                double t = -1.0;
+                if (bd.n_fixed_jobs > 0) {
+                    set_tuning(pers[v], prev_tile);
+                    BatchDev b = bd;
+                    b.inst = d_inst + 2L * n;
+                    if (max_jobs > 0 && b.n_fixed_jobs > max_jobs) b.n_fixed_jobs = (int)max_jobs;
+                    t = time_launch([&] { check(launch_batch_fixed(b, mode, stream), "fixed velocities"); });
+                    if (t < tbest) {
+                        tbest = t;
+                        best_per = pers[v];
+                    }
+                }
+                if (timings) timings[v] = t;
+            }
+            tbest = 1e300;
+            for (int v = 0; v < 3; ++v) {
+                double t = -1.0;
+                if (targets) {
+                    set_tuning(best_per, tiles[v]);
+                    BatchDev b = bd;
+                    b.inst = d_inst + 3L * n;
+                    if (max_jobs > 0 && b.n_target_jobs > max_jobs) b.n_target_jobs = (int)max_jobs;
+                    t = time_launch([&] { check(launch_batch_self_induction(b, mode, stream), "self induction"); });
+                    if (t < tbest) {
+                        tbest = t;
+                        best_tile = tiles[v];
+                    }
+                }
+                if (timings) timings[3 + v] = t;
+            }
+            copy_all(false);
+            check(hipStreamSynchronize(stream), "tuning restore");
+        } catch (...) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipFree(save);
+            set_tuning(prev_per, prev_tile);
+            throw;
+        }
+        check(hipFree(save), "hipFree");
+        set_tuning(best_per, best_tile);
+    }
+
     void enqueue(bool first, bool apply, bool feats) {
         check(hipMemcpyAsync(d_geo, h_geo, sizeof(double) * geo_total, hipMemcpyHostToDevice, stream), "upload geometry");
         check(hipMemcpyAsync(d_inst, h_inst, sizeof(Inst) * kStages * n, hipMemcpyHostToDevice, stream), "upload");
@@ -464,7 +609,132 @@ struct sb_batch {
     }
 };
 
+namespace {
+
+// Tuning cache file: {"entries": [{"key": ..., "fixed_per": p, "targets_tile": t,
+// "ms": [six timings]}, ...]}.  Unreadable or malformed files count as empty.
+struct TuneEntry {
+    std::string key;
+    int fixed_per = 3, targets_tile = 256;
+    double ms[6] = {-1, -1, -1, -1, -1, -1};
+};
+
+std::vector<TuneEntry> read_cache(const char* path) {
+    std::vector<TuneEntry> out;
+    std::ifstream f(path);
+    if (!f) return out;
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::string text = ss.str();
+    try {
+        auto root = json::parse(text.c_str());
+        const json::Node* es = root->find("entries");
+        if (!es || es->t != json::Node::Arr) return out;
+        for (auto& e : es->items) {
+            const json::Node* k = e->find("key");
+            const json::Node* p = e->find("fixed_per");
+            const json::Node* t = e->find("targets_tile");
+            if (!k || k->t != json::Node::Str || !p || p->t != json::Node::Num || !t || t->t != json::Node::Num)
+                continue;
+            TuneEntry te;
+            te.key = k->s;
+            te.fixed_per = (int)p->n;
+            te.targets_tile = (int)t->n;
+            if (const json::Node* ms = e->find("ms"))
+                if (ms->t == json::Node::Arr)
+                    for (size_t i = 0; i < ms->items.size() && i < 6; ++i)
+                        if (ms->items[i]->t == json::Node::Num) te.ms[i] = ms->items[i]->n;
+            out.push_back(te);
+        }
+    } catch (const std::exception&) {
+        out.clear();
+    }
+    return out;
+}
+
+std::string escape_json(const std::string& s) {
+    std::string o;
+    for (char c : s) {
+        if (c == '"' || c == '\\') o += '\\';
+        if ((unsigned char)c < 0x20) continue;
+        o += c;
+    }
+    return o;
+}
+
+void write_cache(const char* path, const std::vector<TuneEntry>& es) {
+    std::ostringstream o;
+    o << "{\"entries\": [";
+    for (size_t i = 0; i < es.size(); ++i) {
+        const TuneEntry& e = es[i];
+        o << (i ? ",\n  " : "\n  ") << "{\"key\": \"" << escape_json(e.key) << "\", \"fixed_per\": " << e.fixed_per
+          << ", \"targets_tile\": " << e.targets_tile << ", \"ms\": [";
+        for (int k = 0; k < 6; ++k) {
+            char buf[32];
+            snprintf(buf, sizeof buf, "%.4f", e.ms[k]);
+            o << (k ? ", " : "") << buf;
+        }
+        o << "]}";
+    }
+    o << "\n]}\n";
+    std::string tmp = std::string(path) + ".tmp";
+    {
+        std::ofstream f(tmp);
+        if (!f) throw std::runtime_error(std::string("cannot write the tuning cache ") + tmp);
+        f << o.str();
+    }
+    if (std::rename(tmp.c_str(), path) != 0) throw std::runtime_error(std::string("cannot replace ") + path);
+}
+
+}  // namespace
+
 extern "C" {
+
+int sb_batch_set_tuning(sb_batch* b, int fixed_per, int targets_tile) {
+    if (!b) return -1;
+    return guarded([&] { b->set_tuning(fixed_per, targets_tile); });
+}
+
+int sb_batch_get_tuning(const sb_batch* b, int* fixed_per, int* targets_tile) {
+    if (!b || !fixed_per || !targets_tile) return -1;
+    *fixed_per = b->bd.fixed_per;
+    *targets_tile = b->bd.targets_tile;
+    return 0;
+}
+
+int sb_batch_autotune(sb_batch* b, const char* cache_path, int flags, int reps, long max_jobs, double* timings,
+                      size_t cap) {
+    if (!b) return -1;
+    if (timings && cap < 6) return -3;
+    if (reps < 1 || max_jobs < 0) return -4;
+    int cached = 0;
+    int rc = guarded([&] {
+        std::string key = b->tuning_key();
+        std::vector<TuneEntry> es = cache_path ? read_cache(cache_path) : std::vector<TuneEntry>{};
+        if ((flags & 1) && cache_path) {
+            for (const TuneEntry& e : es)
+                if (e.key == key) {
+                    b->set_tuning(e.fixed_per, e.targets_tile);
+                    if (timings) std::copy(e.ms, e.ms + 6, timings);
+                    cached = 1;
+                    return;
+                }
+        }
+        TuneEntry te;
+        te.key = key;
+        b->autotune(te.ms, reps, max_jobs);
+        te.fixed_per = b->bd.fixed_per;
+        te.targets_tile = b->bd.targets_tile;
+        if (timings) std::copy(te.ms, te.ms + 6, timings);
+        if ((flags & 2) && cache_path) {
+            es.erase(std::remove_if(es.begin(), es.end(), [&](const TuneEntry& e) { return e.key == key; }),
+                     es.end());
+            es.push_back(te);
+            write_cache(cache_path, es);
+        }
+    });
+    return rc != 0 ? rc : cached;
+}
 
 sb_batch* sb_batch_create(const char* const* setup_jsons, long n, int device, int mode) {
     if (!setup_jsons) {

@@ -181,6 +181,8 @@ struct BatchDev {
     int n_target_jobs = 0;
     unsigned* counters = nullptr;       // device [2] job-queue heads, zero between launches
     int fixed_grid = 0, target_grid = 0;  // persistent workgroups
+    int fixed_per = 3;            // kb_fixed: panels per compute wave per block (1, 2, 3)
+    int targets_tile = 256;       // kb_targets: panel records per LDS tile (64, 128, 256)
     int sym = 0;                  // 1 when any instance uses a symmetry plane
     // learned warm start (sb_mlp): per-strip features [n_strips][64], scale, prediction
     const long* strip_off = nullptr;  // device [n + 1]: first strip of each instance
@@ -220,6 +222,8 @@ hipError_t launch_batch_linearized(const BatchDev& b, int variant, size_t shm, i
 // Singular-pivot flags of every instance into flags[n] (device), cleared in the instances.
 hipError_t launch_batch_take_singular(const BatchDev& b, double* flags, hipStream_t st);
 // Resident workgroups of the persistent pair-sum kernels (0 fixed, 1 self-induction).
-int batch_resident_groups(int which, int mode, int sym);
+// Resident workgroups of kb_fixed (which 0; variant = panels per wave) or
+// kb_targets (which 1; variant = tile length).
+int batch_resident_groups(int which, int mode, int sym, int variant);
 
 }  // namespace sb

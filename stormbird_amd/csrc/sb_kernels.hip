@@ -323,11 +323,11 @@ __device__ inline void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
-template <int SYM>
+template <int SYM, int TL = TILE>
 __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1) {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
-    for (long base = p0; base < p1; base += TILE) {
-        int cnt = (int)((p1 - base) < TILE ? (p1 - base) : TILE);
+    for (long base = p0; base < p1; base += TL) {
+        int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
         __syncthreads();
         const double* src = reinterpret_cast<const double*>(d.rec + base);
         double* dst = reinterpret_cast<double*>(tile);
@@ -346,11 +346,11 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
     return acc + V3{(double)sx - (double)cx, (double)sy - (double)cy, (double)sz - (double)cz};
 }
 
-template <int MODE, int SYM>
+template <int MODE, int SYM, int TL = TILE>
 __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1) {
-    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM>(d, tile, x, acc, p0, p1);
-    for (long base = p0; base < p1; base += TILE) {
-        int cnt = (int)((p1 - base) < TILE ? (p1 - base) : TILE);
+    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL>(d, tile, x, acc, p0, p1);
+    for (long base = p0; base < p1; base += TL) {
+        int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
         __syncthreads();
         const double* src = reinterpret_cast<const double*>(d.rec + base);
         double* dst = reinterpret_cast<double*>(tile);
@@ -1965,10 +1965,7 @@ __device__ inline int next_job(unsigned* head, int* slot) {
 // compute waves use no LDS for them.  Jobs are taken longest first from a
 // queue by persistent workgroups.
 constexpr int COOP_WAVES = 16;
-constexpr int FX_COMPUTE = 15;                   // compute waves; wave 15 folds
-constexpr int FX_PER = 3;                        // panels per compute wave per block
-constexpr int FX_BLOCK = FX_COMPUTE * FX_PER;    // 45 panels per block
-static_assert(FX_BLOCK % 9 == 0, "the fold reads full blocks nine panels at a time");
+constexpr int FX_COMPUTE = 15;  // compute waves; wave 15 folds
 typedef const __attribute__((address_space(4))) PanelRec* ScalarRecs;
 
 // Record k read through the constant address space: a wave-uniform k gives
@@ -1988,10 +1985,15 @@ __device__ inline PanelRec scalar_rec(ScalarRecs recs, long k) {
     return o;
 }
 
-template <int MODE, int SYM>
+// PER panels per compute wave per block (1, 2 or 3; the autotuner picks it):
+// blocks of 15 PER panels, a double-buffered term array of 2 x 15 PER x 1.5 KiB.
+template <int MODE, int SYM, int PER>
 __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __restrict__ jobs, int n_jobs,
                                                  unsigned* head) {
-    __shared__ double T[2][FX_BLOCK][3][WAVE];  // 138,240 B
+    constexpr int FX_BLOCK = FX_COMPUTE * PER;
+    constexpr int FOLD_CHUNK = FX_BLOCK / 5;  // the fold's loads run one chunk ahead of its adds
+    static_assert(FX_BLOCK % FOLD_CHUNK == 0, "the fold reads full blocks in chunks");
+    __shared__ double T[2][FX_BLOCK][3][WAVE];  // 46,080 B per panel per wave
     __shared__ int slot;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / WAVE));
     const int lane = threadIdx.x % WAVE;
@@ -2010,38 +2012,38 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
         if (wave < FX_COMPUTE) {
             ScalarRecs recs = (ScalarRecs)d.rec;
             const int pb = (int)p_begin, pe = (int)p_end;
-            const int mine = pb + wave * FX_PER;  // first panel of this wave in block 0
-            PanelRec r[FX_PER];
+            const int mine = pb + wave * PER;  // first panel of this wave in block 0
+            PanelRec r[PER];
             if (nblk > 0) {
 #pragma unroll
-                for (int q = 0; q < FX_PER; ++q) r[q] = scalar_rec(recs, mine + q < pe ? mine + q : pe - 1);
+                for (int q = 0; q < PER; ++q) r[q] = scalar_rec(recs, mine + q < pe ? mine + q : pe - 1);
             }
             for (int b = 0; b <= (int)nblk; ++b) {
                 if (b < (int)nblk) {
                     const int buf = b & 1;
                     const int k0 = mine + b * FX_BLOCK;
-                    V3 term[FX_PER];
+                    V3 term[PER];
                     if constexpr (SYM == 0 && MODE != MODE_FP32_KAHAN) {
                         // far-field terms of the three panels, interleaved; then the
                         // lanes that need the general evaluation
-                        bool ok[FX_PER];
+                        bool ok[PER];
 #pragma unroll
-                        for (int q = 0; q < FX_PER; ++q) ok[q] = far_fast<MODE>(r[q], x, term[q]);
+                        for (int q = 0; q < PER; ++q) ok[q] = far_fast<MODE>(r[q], x, term[q]);
 #pragma unroll
-                        for (int q = 0; q < FX_PER; ++q)
+                        for (int q = 0; q < PER; ++q)
                             if (!ok[q]) term[q] = panel_unit<MODE>(d, r[q], k0 + q < pe ? k0 + q : pe - 1, x);
                     } else {
 #pragma unroll
-                        for (int q = 0; q < FX_PER; ++q)
+                        for (int q = 0; q < PER; ++q)
                             term[q] = pair_unit<MODE, SYM>(d, r[q], k0 + q < pe ? k0 + q : pe - 1, x);
                     }
 #pragma unroll
-                    for (int q = 0; q < FX_PER; ++q) {
+                    for (int q = 0; q < PER; ++q) {
                         const int k = k0 + q;
                         bool skip = k >= pe || r[q].strength == 0.0 ||
                                     (d.neglect_self && d.wing_of[k % d.N] == wt);
                         V3 u = skip ? V3{0.0, 0.0, 0.0} : term[q] * r[q].strength;
-                        const int kl = wave * FX_PER + q;
+                        const int kl = wave * PER + q;
                         T[buf][kl][0][lane] = u.x;
                         T[buf][kl][1][lane] = u.y;
                         T[buf][kl][2][lane] = u.z;
@@ -2049,7 +2051,7 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
                     // records of the next block, loaded across the barrier
                     const int kn = k0 + FX_BLOCK;
 #pragma unroll
-                    for (int q = 0; q < FX_PER; ++q) r[q] = scalar_rec(recs, kn + q < pe ? kn + q : pe - 1);
+                    for (int q = 0; q < PER; ++q) r[q] = scalar_rec(recs, kn + q < pe ? kn + q : pe - 1);
                 }
                 __syncthreads();
             }
@@ -2061,18 +2063,18 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
                     const long base = p_begin + (b - 1) * FX_BLOCK;
                     const int cnt = (int)(p_end - base < FX_BLOCK ? p_end - base : FX_BLOCK);
                     if (cnt == FX_BLOCK) {
-                        // a full block: loads issued nine panels ahead of the adds
+                        // a full block: loads issued a chunk ahead of the adds
 #pragma unroll
-                        for (int q0 = 0; q0 < FX_BLOCK; q0 += 9) {
-                            double vx[9], vy[9], vz[9];
+                        for (int q0 = 0; q0 < FX_BLOCK; q0 += FOLD_CHUNK) {
+                            double vx[FOLD_CHUNK], vy[FOLD_CHUNK], vz[FOLD_CHUNK];
 #pragma unroll
-                            for (int i = 0; i < 9; ++i) {
+                            for (int i = 0; i < FOLD_CHUNK; ++i) {
                                 vx[i] = T[buf][q0 + i][0][lane];
                                 vy[i] = T[buf][q0 + i][1][lane];
                                 vz[i] = T[buf][q0 + i][2][lane];
                             }
 #pragma unroll
-                            for (int i = 0; i < 9; ++i) {
+                            for (int i = 0; i < FOLD_CHUNK; ++i) {
                                 acc = acc + vx[i];
                                 ay = ay + vy[i];
                                 az = az + vz[i];
@@ -2096,10 +2098,11 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
 // Free-wake self-induction of many instances (update_data.rs:125-141): job =
 // one instance's 256 wake points over all its panels, one target per lane,
 // panel records staged through LDS as in k_targets.
-template <int MODE, int SYM>
+// TL panel records per LDS tile (64, 128 or 256; the autotuner picks it).
+template <int MODE, int SYM, int TL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_targets(
     const Inst* I, const int2* __restrict__ jobs, int n_jobs, unsigned* head) {
-    __shared__ PanelRec tile[TILE];
+    __shared__ PanelRec tile[TL];
     __shared__ int slot;
     for (;;) {
         int j = next_job(head, &slot);
@@ -2111,7 +2114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         long t = (long)jb.y * 256 + threadIdx.x;
         long tc = t < end ? t : end - 1;
         V3 x = load3(d.pts + 3 * tc);
-        V3 acc = sweep<MODE, SYM>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, (long)d.N * d.nz_rows);
+        V3 acc = sweep<MODE, SYM, TL>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, (long)d.N * d.nz_rows);
         if (t < end) store3(d.vel + 3 * t, load3(d.vel + 3 * t) + acc);
     }
 }
@@ -2631,16 +2634,23 @@ hipError_t launch_batch_panels(const BatchDev& b, int shift, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int MODE, int SYM>
+template <int MODE, int SYM, int PER>
 static void launch_kb_fixed(const BatchDev& b, hipStream_t st) {
-    hipLaunchKernelGGL((kb_fixed<MODE, SYM>), dim3(b.fixed_grid), dim3(COOP_WAVES * WAVE), 0, st, b.inst, b.fixed_jobs,
-                       b.n_fixed_jobs, b.counters);
+    hipLaunchKernelGGL((kb_fixed<MODE, SYM, PER>), dim3(b.fixed_grid), dim3(COOP_WAVES * WAVE), 0, st, b.inst,
+                       b.fixed_jobs, b.n_fixed_jobs, b.counters);
+}
+
+template <int MODE, int SYM>
+static void launch_kb_fixed_per(const BatchDev& b, hipStream_t st) {
+    if (b.fixed_per == 1) launch_kb_fixed<MODE, SYM, 1>(b, st);
+    else if (b.fixed_per == 2) launch_kb_fixed<MODE, SYM, 2>(b, st);
+    else launch_kb_fixed<MODE, SYM, 3>(b, st);
 }
 
 hipError_t launch_batch_fixed(const BatchDev& b, int mode, hipStream_t st) {
     if (b.n_fixed_jobs <= 0) return hipSuccess;
-    if (mode == MODE_FAST) b.sym ? launch_kb_fixed<MODE_FAST, 1>(b, st) : launch_kb_fixed<MODE_FAST, 0>(b, st);
-    else b.sym ? launch_kb_fixed<MODE_EXACT, 1>(b, st) : launch_kb_fixed<MODE_EXACT, 0>(b, st);
+    if (mode == MODE_FAST) b.sym ? launch_kb_fixed_per<MODE_FAST, 1>(b, st) : launch_kb_fixed_per<MODE_FAST, 0>(b, st);
+    else b.sym ? launch_kb_fixed_per<MODE_EXACT, 1>(b, st) : launch_kb_fixed_per<MODE_EXACT, 0>(b, st);
     hipLaunchKernelGGL(kb_zero_counters, dim3(1), dim3(64), 0, st, b.counters);
     return hipGetLastError();
 }
@@ -2705,16 +2715,24 @@ hipError_t launch_batch_velocity_copy(const BatchDev& b, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int MODE, int SYM>
+template <int MODE, int SYM, int TL>
 static void launch_kb_targets(const BatchDev& b, hipStream_t st) {
-    hipLaunchKernelGGL((kb_targets<MODE, SYM>), dim3(b.target_grid), dim3(256), 0, st, b.inst, b.target_jobs,
+    hipLaunchKernelGGL((kb_targets<MODE, SYM, TL>), dim3(b.target_grid), dim3(256), 0, st, b.inst, b.target_jobs,
                        b.n_target_jobs, b.counters + 1);
+}
+
+template <int MODE, int SYM>
+static void launch_kb_targets_tile(const BatchDev& b, hipStream_t st) {
+    if (b.targets_tile == 64) launch_kb_targets<MODE, SYM, 64>(b, st);
+    else if (b.targets_tile == 128) launch_kb_targets<MODE, SYM, 128>(b, st);
+    else launch_kb_targets<MODE, SYM, 256>(b, st);
 }
 
 hipError_t launch_batch_self_induction(const BatchDev& b, int mode, hipStream_t st) {
     if (b.n_target_jobs <= 0) return hipSuccess;
-    if (mode == MODE_FAST) b.sym ? launch_kb_targets<MODE_FAST, 1>(b, st) : launch_kb_targets<MODE_FAST, 0>(b, st);
-    else b.sym ? launch_kb_targets<MODE_EXACT, 1>(b, st) : launch_kb_targets<MODE_EXACT, 0>(b, st);
+    if (mode == MODE_FAST)
+        b.sym ? launch_kb_targets_tile<MODE_FAST, 1>(b, st) : launch_kb_targets_tile<MODE_FAST, 0>(b, st);
+    else b.sym ? launch_kb_targets_tile<MODE_EXACT, 1>(b, st) : launch_kb_targets_tile<MODE_EXACT, 0>(b, st);
     hipLaunchKernelGGL(kb_zero_counters, dim3(1), dim3(64), 0, st, b.counters);
     return hipGetLastError();
 }
@@ -2733,16 +2751,28 @@ static int resident_groups(K kernel, int threads) {
     return per_cu * cus;
 }
 
-int batch_resident_groups(int which, int mode, int sym) {
+template <int MODE, int SYM>
+static int fixed_groups(int per) {
+    const int th = COOP_WAVES * WAVE;
+    if (per == 1) return resident_groups(kb_fixed<MODE, SYM, 1>, th);
+    if (per == 2) return resident_groups(kb_fixed<MODE, SYM, 2>, th);
+    return resident_groups(kb_fixed<MODE, SYM, 3>, th);
+}
+
+template <int MODE, int SYM>
+static int target_groups(int tile) {
+    if (tile == 64) return resident_groups(kb_targets<MODE, SYM, 64>, 256);
+    if (tile == 128) return resident_groups(kb_targets<MODE, SYM, 128>, 256);
+    return resident_groups(kb_targets<MODE, SYM, 256>, 256);
+}
+
+int batch_resident_groups(int which, int mode, int sym, int variant) {
     if (which == 0) {
-        const int th = COOP_WAVES * WAVE;
-        if (mode == MODE_FAST)
-            return sym ? resident_groups(kb_fixed<MODE_FAST, 1>, th) : resident_groups(kb_fixed<MODE_FAST, 0>, th);
-        return sym ? resident_groups(kb_fixed<MODE_EXACT, 1>, th) : resident_groups(kb_fixed<MODE_EXACT, 0>, th);
+        if (mode == MODE_FAST) return sym ? fixed_groups<MODE_FAST, 1>(variant) : fixed_groups<MODE_FAST, 0>(variant);
+        return sym ? fixed_groups<MODE_EXACT, 1>(variant) : fixed_groups<MODE_EXACT, 0>(variant);
     }
-    if (mode == MODE_FAST)
-        return sym ? resident_groups(kb_targets<MODE_FAST, 1>, 256) : resident_groups(kb_targets<MODE_FAST, 0>, 256);
-    return sym ? resident_groups(kb_targets<MODE_EXACT, 1>, 256) : resident_groups(kb_targets<MODE_EXACT, 0>, 256);
+    if (mode == MODE_FAST) return sym ? target_groups<MODE_FAST, 1>(variant) : target_groups<MODE_FAST, 0>(variant);
+    return sym ? target_groups<MODE_EXACT, 1>(variant) : target_groups<MODE_EXACT, 0>(variant);
 }
 
 }  // namespace sb

@@ -158,3 +158,56 @@ def test_batch_rejects_mixed_and_warm_quasi():
     m = sb.Mlp(seed=1)
     with pytest.raises(RuntimeError, match="dynamic batches"):
         b.set_warm_start(m, 1)
+
+
+@pytest.mark.parametrize("symmetric", [False, True])
+def test_autotuner_and_every_variant_keep_results(tmp_path, symmetric):
+    """The autotuner times the kernel variants on the batch's own state without changing
+    it; every variant (panels per wave of the fixed-velocity kernel, LDS tile length of the
+    self-induction kernel) gives the same bits; the choice is cached per device and shape
+    class in a JSON file and reused by a batch of the same shape."""
+    import json
+
+    setups, cases = mixed_setups()
+    if not symmetric:
+        setups, cases = setups[:-1], cases[:-1]
+    ref, tuned = sb.Batch(setups), sb.Batch(setups)
+    fs_all = np.concatenate([S.uniform_freestream(s.nr_freestream_points(), c["velocity"])
+                             for s, c in zip(ref.instances, cases)])
+    dts = np.array([c["time_step"] for c in cases])
+
+    def step_both(k):
+        print(f"tuning step {k}", flush=True)
+        ra = ref.do_step(times=k * dts, time_steps=dts, freestream_velocity=fs_all)
+        rb = tuned.do_step(times=k * dts, time_steps=dts, freestream_velocity=fs_all)
+        for i in range(len(setups)):
+            compare_results(rb[i], ra[i], exact=True, where=f"step {k} instance {i}")
+
+    for k in range(5):  # the free wake's self-induction runs from the fourth step on
+        step_both(k)
+    cache = str(tmp_path / "tuning.json")
+    cached, choice, t = tuned.autotune(cache)
+    assert not cached and choice[0] in (1, 2, 3) and choice[1] in (64, 128, 256)
+    assert np.all(t > 0.0), t  # both kernels had work
+    for i in range(len(setups)):  # the state is unchanged
+        wa, wb = ref.instances[i].get_wake_state(), tuned.instances[i].get_wake_state()
+        for key in ("points", "velocities", "strengths"):
+            assert same(wa[key], wb[key]), f"instance {i}: wake {key} after tuning"
+    k = 5
+    for variant in (choice, (1, 64), (2, 128), (3, 256)):
+        tuned.set_tuning(*variant)
+        assert tuned.get_tuning() == variant
+        step_both(k)
+        k += 1
+    entries = json.load(open(cache))["entries"]
+    assert len(entries) == 1 and entries[0]["fixed_per"] == choice[0] and entries[0]["targets_tile"] == choice[1]
+    other = sb.Batch(setups)
+    for j in range(5):
+        other.do_step(times=j * dts, time_steps=dts, freestream_velocity=fs_all, want_result=False)
+    hit, choice2, t2 = other.autotune(cache)
+    assert hit and choice2 == choice and np.allclose(t2, t, atol=1e-4)
+    with pytest.raises(RuntimeError):
+        other.set_tuning(4, 256)
+    fresh = sb.Batch(setups)
+    with pytest.raises(RuntimeError, match="take a step"):
+        fresh.autotune(None)
