@@ -96,7 +96,24 @@ def cpu_baseline(js, state, n_panels_total, pairs_per_step, budget_s=12.0):
     orc.induced_velocities(sample)
     t = time.perf_counter() - t0
     pairs_s = sample.shape[0] * n_panels_total / t
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = oracle_py.set_threads(0)
+    # one thread on a small sample of the same sum
+    one = pts[:: max(1, pts.shape[0] // 64)][:64]
+    oracle_py.set_threads(1)
+    t0 = time.perf_counter()
+    orc.induced_velocities(one)
+    t1 = time.perf_counter() - t0
+    oracle_py.set_threads(threads)
+    pairs_1 = one.shape[0] * n_panels_total / t1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": pairs_s / pairs_per_step,
         "unit": "time-steps/s",
@@ -107,6 +124,10 @@ def cpu_baseline(js, state, n_panels_total, pairs_per_step, budget_s=12.0):
                    f"{pairs_per_step:.3e} pairs of one step; C++ f64 restatement of the reference, "
                    f"OpenMP over targets"),
         "pairs_per_s": pairs_s,
+        "one_thread": {"pairs_per_s": pairs_1, "time_steps_per_s": pairs_1 / pairs_per_step,
+                       "sample": f"{one.shape[0]} wake points x {n_panels_total} panels ({t1:.2f} s)"},
+        "cpu_model": model,
+        "host_threads_visible": os.cpu_count(),
     }
 
 
@@ -559,7 +580,8 @@ def complete_sail_model_section(n_directions=36, n_loadings=5, steps=3):
           "parallel_gust": gust, "perpendicular_gust": gust}
     m = sb.CompleteSailModel(S.complete_sail_model(js4, 4))
     sim = m.simulation
-    pts, vel, gam, prev, fs = synthesize_developed_state(sim, case)
+    pts, vel, gam, prev, _ = S.developed_four_sail_state(case)
+    fs = S.uniform_freestream(sim.nr_freestream_points(), case["velocity"])
     sim.set_wake_state(pts, vel, gam, 2000)
     sim.previous_circulation_strength = prev
     w = sb.WindCondition(wc)
@@ -575,12 +597,65 @@ def complete_sail_model_section(n_directions=36, n_loadings=5, steps=3):
     e, _ = time_gpu(sim, fs_dev.data_ptr(), n_points, dt, steps, 1, 2010)
     t_plain = e / steps
     dynamic = {"workload": ("four 30x10 m wing sails, 40 strips each, free wake of 2000 rows (328164 wake "
-                            "points), power-law wind 12 m/s at 10 m with gusts along and across, developed wake"),
+                            "points), power-law wind 12 m/s at 10 m with gusts along and across, from the rolled-up "
+                            "wake of a 2,000-step run"),
                "csm_ms_per_step": 1e3 * t_csm, "plain_device_freestream_ms_per_step": 1e3 * t_plain,
                "wind_evaluation_overhead_ms": 1e3 * (t_csm - t_plain),
                "freestream": "control points on the host (inflow corrections, controller), wake points on the device"}
     del m, sim
     return {"optimal_loading_polar": polar, "dynamic_steps": dynamic}
+
+
+def full_run(steps=2000, nr_strips=40, dump=None):
+    """The four-sail free wake run from the start for `steps` steps (the wake fills one row
+    per step, so the self-induction work grows with the square of the step number); progress
+    on stderr every 100 steps."""
+    import torch
+
+    import stormbird_amd as sb
+    from stormbird_amd import setups as S
+
+    js, case = S.four_sail_free_wake(nr_panels=steps, nr_strips=nr_strips)
+    dt = case["time_step"]
+    sim = sb.Simulation(js, mode=sb.MODE_EXACT)
+    fs = S.uniform_freestream(sim.nr_freestream_points(), case["velocity"])
+    fs_dev = torch.from_numpy(np.ascontiguousarray(fs)).to("cuda")
+    n_pts = sim.nr_freestream_points()
+    marks = {}
+    t0 = time.perf_counter()
+    res = None
+    for k in range(steps):
+        want = (k + 1) % 100 == 0 or k == steps - 1
+        res = sim.do_step_device(time=k * dt, time_step=dt, freestream_ptr=fs_dev.data_ptr(), n_points=n_pts,
+                                 want_result=want)
+        if want:
+            torch.cuda.synchronize()
+            marks[k + 1] = time.perf_counter() - t0
+            print(f"full run: step {k + 1}, {marks[k + 1]:.1f} s, iterations {res.iterations}", file=sys.stderr,
+                  flush=True)
+    total = time.perf_counter() - t0
+    gam = res.force_input.circulation_strength
+    st = sim.get_wake_state()
+    near, pairs = sim.count_self_induction_pairs()
+    kernel_ms = {k: sim.last_kernel_ms(k) for k in ("wake_update", "fixed", "factors", "solve", "self_induction",
+                                                     "forces")}
+    if dump:
+        np.savez(dump, points=st["points"], velocities=st["velocities"], strengths=st["strengths"],
+                 previous_circulation=sim.previous_circulation_strength, steps_completed=st["steps_completed"])
+    last = steps - 100 if steps > 100 else 0
+    return {
+        "workload": (f"four 30x10 m wing sails, {nr_strips} strips each, free wake run from the start for "
+                     f"{steps} steps (every wake point self-induced), EXACT"),
+        "steps": steps, "total_s": total, "mean_time_steps_per_s": steps / total,
+        "last_100_time_steps_per_s": (steps - last) / (total - marks.get(last, 0.0)) if last else None,
+        "seconds_at": {str(k): v for k, v in marks.items() if k % 500 == 0 or k == steps},
+        "final": {"iterations": res.iterations, "residual": res.residual,
+                  "circulation_max_abs": float(np.max(np.abs(gam))),
+                  "all_finite": bool(np.all(np.isfinite(gam)) and np.all(np.isfinite(st["points"]))),
+                  "wake_extent_m": float(np.max(np.linalg.norm(st["points"] - st["points"][0], axis=1))),
+                  "integrated_force": [float(x) for x in res.integrated_forces_sum()]},
+        "last_step": {"self_induction_pairs": {"near": near, "far": pairs - near}, "kernel_ms": kernel_ms},
+    }
 
 
 def main():
@@ -600,7 +675,8 @@ def main():
     ap.add_argument("--train-conditions", type=int, default=16, help="inflow conditions per instance for training")
     ap.add_argument("--no-quasi", action="store_true")
     ap.add_argument("--no-csm", action="store_true")
-    ap.add_argument("--only", choices=("batched", "warm", "route", "quasi", "csm"), default=None,
+    ap.add_argument("--dump-state", default=None, help="with --only full: write the final wake state (.npz)")
+    ap.add_argument("--only", choices=("batched", "warm", "route", "quasi", "csm", "full"), default=None,
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -626,6 +702,8 @@ def main():
             out = quasi_steady()
         elif args.only == "csm":
             out = complete_sail_model_section()
+        elif args.only == "full":
+            out = full_run(steps=args.panels, nr_strips=args.strips, dump=args.dump_state)
         else:
             out = route_simulation(n_ships=args.route_ships, steps=args.steps)
         print(json.dumps({"section": args.only, **out}))
@@ -634,7 +712,13 @@ def main():
     js, case = S.four_sail_free_wake(nr_panels=args.panels, nr_strips=args.strips)
     dt = case["time_step"]
     sim = sb.Simulation(js, mode=sb.MODE_EXACT)
-    pts, vel, gam, prev, fs = synthesize_developed_state(sim, case)
+    if args.panels == 2000 and args.strips == 40:
+        pts, vel, gam, prev, _ = S.developed_four_sail_state(case)
+        fs = S.uniform_freestream(sim.nr_freestream_points(), case["velocity"])
+        state = "rolled-up wake of a 2,000-step run from the start (stored state)"
+    else:
+        pts, vel, gam, prev, fs = synthesize_developed_state(sim, case)
+        state = "synthesized developed wake (convected sheet with seeded noise)"
     sim.set_wake_state(pts, vel, gam, 2000)
     sim.previous_circulation_strength = prev
     n_points = sim.nr_freestream_points()
@@ -713,7 +797,7 @@ def main():
         "config": {
             "workload": (f"four 30x10 m wing sails 25 m apart, {args.strips} strips each, free wake of "
                          f"{args.panels} panel rows ({sim.nr_panels} panels, {sim.nr_wake_points} wake points), "
-                         "all wake points self-induced, developed-wake steady state"),
+                         f"all wake points self-induced, steady state from the {state}"),
             "mode": "exact (bit-identical to the f64 restatement)",
             "n_ctrl_points": sim.nr_ctrl_points,
             "n_panels": sim.nr_panels,

@@ -26,6 +26,10 @@
 
 #include "json_lite.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 // Elementary functions: the host libm by default (what the Rust build calls on
 // Linux); with ORC_PORTABLE_LIBM the portable implementation that the device
 // code also uses, so that whole free-wake trajectories can be compared bit for
@@ -1825,6 +1829,16 @@ double orc_foil_cl(const char* foil_json, double alpha, int which) {
     }
 }
 double orc_sigmoid(double x, double x0, double r) { return sigmoid_zero_to_one(x, x0, r); }
+
+int orc_set_threads(int n) {  // OpenMP threads of the target loop (0: the runtime default)
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 int orc_libm_variant(void) {
 #ifdef ORC_PORTABLE_LIBM

@@ -106,6 +106,8 @@ void orc_panel_unit_velocity(const double* corners12, double far_ratio,
                              double* out);
 double orc_foil_cl(const char* foil_json, double alpha, int which);
 double orc_sigmoid(double x, double x0, double range);
+/* OpenMP threads of the per-target loops (n > 0 sets it); returns the current count. */
+int orc_set_threads(int n);
 /* 0: host libm, 1: portable libm shared with the device code */
 int orc_libm_variant(void);
 /* fn: 0 sin, 1 cos, 2 exp, 3 acos, 4 pow(x, y), 5 log, 6 atan */

@@ -75,6 +75,8 @@ def lib(variant: str = "glibc"):
         L.orc_horseshoe_unit_velocity.argtypes = [_dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp]
         L.orc_sim_get_line_model.argtypes = [vp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long), _dp, _dp]
         L.orc_sim_restart.argtypes = [vp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = ctypes.c_int
         _libs[variant] = L
     return _libs[variant]
 
@@ -251,3 +253,8 @@ def horseshoe_unit_velocity(start, end, wake_start, wake_end, x, core=0.0):
     lib().orc_horseshoe_unit_velocity(_p(_f64(start)), _p(_f64(end)), _p(_f64(wake_start)), _p(_f64(wake_end)),
                                       _p(_f64(x)), float(core), _p(out))
     return out
+
+
+def set_threads(n: int, variant: str = "glibc") -> int:
+    """OpenMP threads of the oracle's per-target loops (n > 0 sets it); returns the count."""
+    return lib(variant).orc_set_threads(int(n))

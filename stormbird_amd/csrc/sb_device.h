@@ -70,6 +70,9 @@ struct Inst {
     const double* core;  // viscous core length [N*M]
     const double* corners;  // optional explicit corners [N*M][12] (standalone panel sets)
     PanelRec* rec;       // far-field panel records [N*M]
+    double* segs;        // optional [N*M][16]: per ring edge e (corner e to e+1) its unit
+                         // direction and length, the target-independent part of the
+                         // viscous core term; null: computed per pair
     const int* col;      // [N] span-point column of strip i's first corner (i + wing index)
     const int* wing_of;  // [N]
     const int* span_src; // [P][2] ctrl-point indices mapped to span point j

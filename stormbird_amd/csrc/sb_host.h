@@ -79,6 +79,7 @@ struct sb_sim {
     double *d_fixed = nullptr, *d_F = nullptr, *d_prev = nullptr, *d_hist = nullptr, *d_scal = nullptr;
     double *d_out = nullptr, *d_tmp = nullptr;
     double *d_lin = nullptr, *d_lin_gamma = nullptr;  // Linearized solve scratch
+    double* d_segs = nullptr;  // ring-edge geometry per panel, when the free wake is self-induced
     double hs_core = 0.0;                             // quasi-steady horseshoe core, per step
     double h_singular = 0.0;
     size_t tmp_cap = 0;
@@ -100,7 +101,7 @@ struct sb_sim {
         void* bufs[] = {d_pts[0], d_pts[1], d_vel, d_str[0], d_str[1], d_core, d_geo, d_fs, d_felt,
                         d_felt_span, d_fixed, d_F, d_prev, d_hist, d_scal, d_out, d_tmp, d_rec,
                         d_col, d_wing, d_span_src, d_foil, mc.partial, mc.flags, tmp_mc.partial, tmp_mc.flags,
-                        fb.terms, fb.acc, d_lin, d_lin_gamma};
+                        fb.terms, fb.acc, d_lin, d_lin_gamma, d_segs};
         for (void* p : bufs)
             if (p) (void)hipFree(p);
         for (auto& e : ev)
@@ -154,6 +155,7 @@ struct sb_sim {
         d.core = d_core;
         d.corners = nullptr;
         d.rec = d_rec;
+        d.segs = d_segs;
         d.col = d_col;
         d.wing_of = d_wing;
         d.span_src = d_span_src;
@@ -274,6 +276,8 @@ struct sb_sim {
         d_str[1] = dalloc<double>(nr_panels);
         d_core = dalloc<double>(nr_panels);
         d_rec = dalloc<PanelRec>(nr_panels);
+        // ring-edge geometry: where near pairs are many (a self-induced free wake)
+        if (!setup.quasi && setup.wake.end_index > 0) d_segs = dalloc<double>(16 * nr_panels);
         d_geo = dalloc<double>(gl.total);
         d_fs = dalloc<double>(3 * (N + nr_points));
         d_felt = dalloc<double>(3 * N);

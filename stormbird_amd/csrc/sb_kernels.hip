@@ -79,12 +79,62 @@ __device__ inline void panel_corners(const Inst& d, const PanelRec& r, int k, V3
     c[3] = load3(d.pts + 3 * b);
 }
 
+// segment_unit for the ring edge from corner i to corner j with the quantities
+// that do not depend on the edge's direction shared: r = x - corner, |r|^2 and
+// |r| of both corners, and the edge's unit direction and length (computed per
+// panel by panels_body with the operations viscous_core_term uses).  Every
+// value equals the one segment_unit computes, so the result is the same bits.
+__device__ inline V3 ring_edge(V3 r1, V3 r2, double r1sq, double r2sq, double r1l, double r2l, const double* seg,
+                               double rc) {
+    double r1r2 = r1l * r2l;
+    double den = r1r2 * (r1r2 + dot(r1, r2));
+    if (fabs(den) > CLOSENESS_ERROR) {
+        double core = 1.0;
+        if (rc != 0.0) {  // vortex_line.rs:59-84 + 90-106 with rel_pt = r1, |rel_pt|^2 = r1sq
+            V3 dir{seg[0], seg[1], seg[2]};
+            double line_len = seg[3];
+            double par = dot(r1, dir);
+            double d2;
+            if (par < 0.0) d2 = r1sq;
+            else if (par > line_len) d2 = r2sq;
+            else d2 = r1sq - par * par;
+            double rc2 = rc * rc;
+            double cden = sqrt(rc2 * rc2 + d2 * d2);
+            core = cden > 0.0 ? d2 / cden : 0.0;
+        }
+        double k = (r1l + r2l) / den;
+        return (cross(r1, r2) * core) * (k * FOUR_PI_INVERSE);
+    }
+    return V3{0.0, 0.0, 0.0};
+}
+
 // panel.rs:99-118 (vortex ring) — used for near pairs only
 __device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x) {
-    V3 c[4];
-    panel_corners(d, r, k, c);
     double rc = d.core[k];
     V3 u{0.0, 0.0, 0.0};
+    if (d.segs && !d.corners) {
+        // corners loaded one at a time (rows s and s+1, dynamic_wake/mod.rs:82-103);
+        // each corner's r, |r|^2 and |r| serve the two edges that meet there
+        const double* g = d.segs + 16L * k;
+        const long a = (long)r.cbase;
+        const long idx[4] = {a, a + 1, a + d.P + 1, a + d.P};
+        const V3 r0 = x - load3(d.pts + 3 * idx[0]);
+        const double sq0 = len2(r0), ln0 = sqrt(sq0);
+        V3 ra = r0;
+        double sqa = sq0, lna = ln0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const V3 rb = x - load3(d.pts + 3 * idx[e + 1]);
+            const double sqb = len2(rb), lnb = sqrt(sqb);
+            u = u + ring_edge(ra, rb, sqa, sqb, lna, lnb, g + 4 * e, rc);
+            ra = rb;
+            sqa = sqb;
+            lna = lnb;
+        }
+        return u + ring_edge(ra, r0, sqa, sq0, lna, ln0, g + 12, rc);
+    }
+    V3 c[4];
+    panel_corners(d, r, k, c);
     u = u + segment_unit(c[0], c[1], x, rc);
     u = u + segment_unit(c[1], c[2], x, rc);
     u = u + segment_unit(c[2], c[3], x, rc);
@@ -843,6 +893,19 @@ __device__ inline void panels_body(const Inst& d, int shift, long k) {
     }
     r.strength = g;
     d.rec[k] = r;
+    if (d.segs) {  // viscous_core_term's rel_line, its length and unit direction, per edge
+        double* q = d.segs + 16L * k;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            V3 rel_line = c[(e + 1) & 3] - c[e];
+            double line_len = sqrt(len2(rel_line));
+            V3 dir = rel_line / line_len;
+            q[4 * e] = dir.x;
+            q[4 * e + 1] = dir.y;
+            q[4 * e + 2] = dir.z;
+            q[4 * e + 3] = line_len;
+        }
+    }
 }
 __global__ void k_panels(Inst d, int shift) { panels_body(d, shift, (long)blockIdx.x * blockDim.x + threadIdx.x); }
 

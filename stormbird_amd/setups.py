@@ -16,6 +16,7 @@ Workloads:
 from __future__ import annotations
 
 import json
+import os
 import math
 
 import numpy as np
@@ -212,6 +213,32 @@ def complete_sail_model(setup_json: str, n_sails: int, environment=None, max_ang
                        "flow_measurement_settings": {"angle_of_attack": measure, "wind_direction": measure,
                                                      "wind_velocity": measure}},
     })
+
+
+DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEVELOPED_FOUR_SAIL = os.path.join(DATA, "four_sail_developed_wake_2000rows.npz")
+
+
+def compact_state(src: str, dst: str = DEVELOPED_FOUR_SAIL) -> None:
+    """Stores a dumped wake state (bench.py --only full --dump-state src) compactly: wake
+    points and strengths in float32 (0.25 mm at the wake's 5 km extent), the previous
+    circulation in float64; the velocities are not kept."""
+    d = np.load(src)
+    np.savez_compressed(dst, points=d["points"].astype(np.float32), strengths=d["strengths"].astype(np.float32),
+                        previous_circulation=d["previous_circulation"], steps_completed=d["steps_completed"])
+
+
+def developed_four_sail_state(case):
+    """The wake of four_sail_free_wake() (defaults: 40 strips, 2,000 rows) after 2,000 steps run
+    from the start on the GPU in EXACT arithmetic: rolled-up tip and root vortices, 17 % of
+    the self-induction pairs in the near field.  Returns (points, velocities, strengths,
+    previous circulation, steps completed); the velocities are the freestream (the first step
+    convects with them, later steps with the induced field)."""
+    d = np.load(DEVELOPED_FOUR_SAIL)
+    pts = d["points"].astype(np.float64)
+    vel = np.tile(np.asarray(case["velocity"], dtype=np.float64), (pts.shape[0], 1))
+    return (pts, vel, d["strengths"].astype(np.float64), np.array(d["previous_circulation"], dtype=np.float64),
+            int(d["steps_completed"]))
 
 
 def polar_sweep_instance(rng: np.random.Generator, max_panels=3000, min_panels=200):
