@@ -177,3 +177,20 @@ def test_c_headers_compile_and_link(tmp_path):
     first, second = r.stdout.strip().splitlines()
     assert first.startswith("-4 ") and "lifting_line_simulation" in first  # missing field named
     assert second == "0 7.500"
+
+
+def test_stored_developed_state_loads():
+    """The rolled-up four-sail wake the bench starts from: shapes of the default setup
+    (40 strips x 4 sails, 2,000 rows), finite values, a wake convected downstream."""
+    from stormbird_amd import setups as S
+
+    js, case = S.four_sail_free_wake()
+    info = sb.setup_info(js)
+    pts, vel, gam, prev, steps = S.developed_four_sail_state(case)
+    assert pts.shape == (info["nr_freestream_points"] - info["nr_ctrl_points"], 3)
+    assert gam.shape == (info["nr_panels"],) and prev.shape == (info["nr_ctrl_points"],)
+    assert vel.shape == pts.shape and steps == 2000
+    assert np.all(np.isfinite(pts)) and np.all(np.isfinite(gam)) and np.all(np.isfinite(prev))
+    u = np.asarray(case["velocity"])
+    downstream = (pts - pts[:164].mean(axis=0)) @ (u / np.linalg.norm(u))
+    assert downstream.max() > 4000.0  # 2,000 steps x 0.208 s x 12 m/s = 5 km of wake
