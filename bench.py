@@ -734,42 +734,36 @@ def main():
     flops = (total - near) * FLOP_FAR + near * FLOP_NEAR
     achieved = flops / (k_ms * 1e-3) / 1e12
 
-    fast = None
+    fast = fp32 = None
     if not args.no_fast:
-        simf = sb.Simulation(js, mode=sb.MODE_FAST)
-        simf.set_wake_state(pts, vel, gam, 2000)
-        simf.previous_circulation_strength = prev
-        ef, kf = time_gpu(simf, fs_dev.data_ptr(), n_points, dt, args.steps, args.warmup, 2000)
-        fast = {"time_steps_per_s": args.steps / ef, "ms_per_step": 1000.0 * ef / args.steps,
-                "self_induction_ms": float(np.mean(kf)),
-                "self_induction_tflops": flops / (float(np.mean(kf)) * 1e-3) / 1e12}
-        del simf
-
-    fp32 = None
-    if not args.no_fast:
-        # FP32-Kahan: one step from the developed state against the EXACT step from the same
-        # state (EXACT equals the f64 restatement bit for bit), then its step rate.
+        # FAST and FP32-Kahan: one step from the developed state against the EXACT step from
+        # the same state (EXACT equals the f64 restatement bit for bit), then each mode's rate.
         refs = {}
-        for md in (sb.MODE_EXACT, sb.MODE_FP32_KAHAN):
+        for md in (sb.MODE_EXACT, sb.MODE_FAST, sb.MODE_FP32_KAHAN):
             s_ = sb.Simulation(js, mode=md)
             s_.set_wake_state(pts, vel, gam, 2000)
             s_.previous_circulation_strength = prev
             r_ = s_.do_step(time=2000 * dt, time_step=dt, freestream_velocity=fs)
             refs[md] = (s_, r_.as_arrays(), s_.get_wake_state()["velocities"])
-        s32, a32, v32 = refs[sb.MODE_FP32_KAHAN]
-        _, a64, v64 = refs[sb.MODE_EXACT]
+        _, a64, v64 = refs.pop(sb.MODE_EXACT)
 
         def rel(a, b):
             return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
-        e32, k32 = time_gpu(s32, fs_dev.data_ptr(), n_points, dt, args.steps, args.warmup, 2001)
-        fp32 = {"time_steps_per_s": args.steps / e32, "ms_per_step": 1000.0 * e32 / args.steps,
-                "self_induction_ms": float(np.mean(k32)),
-                "max_rel_dev_vs_f64": {"circulation": rel(a32["circulation_strength"], a64["circulation_strength"]),
-                                       "integrated_forces": rel(a32["integrated_forces"], a64["integrated_forces"]),
-                                       "wake_velocities": rel(v32, v64)},
-                "deviation_measured": "one step from the developed state, against the EXACT (f64) step"}
-        del refs, s32
+        def mode_line(md):
+            s_, a_, v_ = refs.pop(md)
+            e_, k_ = time_gpu(s_, fs_dev.data_ptr(), n_points, dt, args.steps, args.warmup, 2001)
+            km = float(np.mean(k_))
+            return {"time_steps_per_s": args.steps / e_, "ms_per_step": 1000.0 * e_ / args.steps,
+                    "self_induction_ms": km, "self_induction_tflops": flops / (km * 1e-3) / 1e12,
+                    "max_rel_dev_vs_f64": {"circulation": rel(a_["circulation_strength"], a64["circulation_strength"]),
+                                           "integrated_forces": rel(a_["integrated_forces"], a64["integrated_forces"]),
+                                           "wake_velocities": rel(v_, v64)},
+                    "deviation_measured": "one step from the developed state, against the EXACT (f64) step"}
+
+        fast = mode_line(sb.MODE_FAST)
+        fp32 = mode_line(sb.MODE_FP32_KAHAN)
+        del refs
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "self_induction_traffic.json")

@@ -114,7 +114,7 @@ struct McBuffers {
     long long* flags = nullptr;   // [cap_jobs], initialised to -1
     long cap_jobs = 0;
     long long epoch = 0;          // incremented per launch; flags compare against it
-    int slots[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // resident workgroups per (mode, symmetry)
+    int slots[3][2][2] = {};  // resident workgroups per (mode, symmetry, near-field staging)
 };
 
 // Scratch of the two-pass few-target sum (k_terms + k_fold_terms): one chunk

@@ -108,33 +108,197 @@ __device__ inline V3 ring_edge(V3 r1, V3 r2, double r1sq, double r2sq, double r1
     return V3{0.0, 0.0, 0.0};
 }
 
-// panel.rs:99-118 (vortex ring) — used for near pairs only
-__device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x) {
-    double rc = d.core[k];
+// 1/sqrt(a) from the hardware estimate and one Newton step (~1 ulp) for a in
+// [2^-900, 2^1000]; outside that range (never for wake geometry) the IEEE
+// operations, so that a = 0 gives 0 and no estimate meets a subnormal.
+__device__ inline double rsqrt_fast(double a) {
+    if (a < 0x1p-900 || a > 0x1p1000) return a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+    double y = __builtin_amdgcn_rsq(a);
+    return fma(0.5 * y, fma(-a * y, y, 1.0), y);
+}
+
+// IEEE f64 square root and division without their operand scaling.  gfx950
+// expands sqrt(a) into a scaling of a < 2^-767, v_rsq_f64, two Goldschmidt /
+// Newton corrections, the unscaling and a zero/inf select; and n / d into
+// v_div_scale of both operands, a reciprocal refined by two Newton steps, a
+// quotient corrected by one fma residual step and v_div_fixup.  For a in
+// [2^-767, 2^1000], and for |d| in [2^-300, 2^300] with |n| in [2^-600, 2^400],
+// the scalings, the select and the fix-up return their input unchanged, so
+// these sequences (the rest of the expansions, step for step) return the same
+// bits as sqrt(a) and n / d.  Callers check the ranges.
+__device__ inline double sqrt_unscaled(double a) {
+    double y = __builtin_amdgcn_rsq(a);
+    double h = a * y;
+    double g = y * 0.5;
+    double e = fma(-g, h, 0.5);
+    h = fma(h, e, h);
+    double rr = fma(-h, h, a);
+    g = fma(g, e, g);
+    h = fma(rr, g, h);
+    rr = fma(-h, h, a);
+    return fma(rr, g, h);
+}
+
+__device__ inline double div_unscaled(double n, double d) {
+    double v = __builtin_amdgcn_rcp(d);
+    double t = fma(-d, v, 1.0);
+    v = fma(v, t, v);
+    t = fma(-d, v, 1.0);
+    v = fma(v, t, v);
+    double q = n * v;
+    double e = fma(-d, q, n);
+    return fma(e, v, q);
+}
+
+// ring_edge with sqrt_unscaled / div_unscaled.  Equal to ring_edge bit for bit
+// when |r1|^2, |r2|^2 lie in [2^-700, 2^140], rc is 0 or |rc| in [2^-150, 2^69]
+// (then |den| <= 2^281, the core denominator lies in [2^-300, 2^141] and the
+// numerators below 2^141) and, with a core, d2 >= 2^-600: the last condition
+// is and-ed into `ok`, the others are the caller's.
+__device__ inline V3 ring_edge_unscaled(V3 r1, V3 r2, double r1sq, double r2sq, double r1l, double r2l,
+                                        const double* seg, double rc, bool& ok) {
+    double r1r2 = r1l * r2l;
+    double den = r1r2 * (r1r2 + dot(r1, r2));
+    if (fabs(den) > CLOSENESS_ERROR) {
+        double core = 1.0;
+        if (rc != 0.0) {
+            V3 dir{seg[0], seg[1], seg[2]};
+            double line_len = seg[3];
+            double par = dot(r1, dir);
+            double d2;
+            if (par < 0.0) d2 = r1sq;
+            else if (par > line_len) d2 = r2sq;
+            else d2 = r1sq - par * par;
+            double rc2 = rc * rc;
+            ok = ok & (d2 >= 0x1p-600);
+            core = div_unscaled(d2, sqrt_unscaled(rc2 * rc2 + d2 * d2));
+        }
+        double k = div_unscaled(r1l + r2l, den);
+        return (cross(r1, r2) * core) * (k * FOUR_PI_INVERSE);
+    }
+    return V3{0.0, 0.0, 0.0};
+}
+
+// |r|^2 in the range ring_edge_unscaled needs
+__device__ inline bool ring_sq_ok(double sq) { return (sq >= 0x1p-700) & (sq <= 0x1p140); }
+
+// ring_edge in FAST arithmetic: fused dot products, square roots and divisions
+// from the hardware reciprocal (square root) with one Newton step (~1 ulp).
+__device__ inline V3 ring_edge_fast(V3 r1, V3 r2, double r1sq, double r2sq, double r1l, double r2l,
+                                    const double* seg, double rc) {
+    double r1r2 = r1l * r2l;
+    double den = r1r2 * (r1r2 + fma(r1.x, r2.x, fma(r1.y, r2.y, r1.z * r2.z)));
+    if (fabs(den) > CLOSENESS_ERROR) {
+        double core = 1.0;
+        if (rc != 0.0) {
+            double par = fma(r1.x, seg[0], fma(r1.y, seg[1], r1.z * seg[2]));
+            double d2 = par < 0.0 ? r1sq : (par > seg[3] ? r2sq : fma(-par, par, r1sq));
+            double rc2 = rc * rc;
+            double a = fma(rc2, rc2, d2 * d2);
+            core = d2 * rsqrt_fast(a);
+        }
+        // den > CLOSENESS_ERROR here; the estimate is valid below 2^1000
+        double v;
+        if (fabs(den) > 0x1p1000) v = 1.0 / den;
+        else {
+            v = __builtin_amdgcn_rcp(den);
+            v = fma(v, fma(-den, v, 1.0), v);
+        }
+        double kk = (r1l + r2l) * v * FOUR_PI_INVERSE * core;
+        return V3{fma(r1.y, r2.z, -r1.z * r2.y) * kk, fma(r1.z, r2.x, -r1.x * r2.z) * kk,
+                  fma(r1.x, r2.y, -r1.y * r2.x) * kk};
+    }
+    return V3{0.0, 0.0, 0.0};
+}
+
+// |r| in FAST arithmetic (0 at r = 0)
+__device__ inline double length_fast(double sq) { return sq * rsqrt_fast(sq); }
+
+// One vortex ring from its corners c0..c3 (ring order) and its ring-edge
+// geometry g (16 doubles: per edge e, from corner e to e+1, the unit direction
+// and the length), for target x.  The corner pointers may point to global
+// memory or to the shared tile.  Each corner's r, |r|^2 and |r| serve the two
+// edges that meet there.  FAST: fused operations and refined hardware
+// estimates.  EXACT: sqrt_unscaled / div_unscaled, and the IEEE operators for
+// lanes outside their ranges (a target at a corner or on an edge), so the
+// result is ring_edge's, the reference's operations, bit for bit.
+template <int MODE>
+__device__ inline V3 rings_at(const double* c0, const double* c1, const double* c2, const double* c3,
+                              const double* g, double rc, V3 x) {
+    const double* cp[4] = {c0, c1, c2, c3};
     V3 u{0.0, 0.0, 0.0};
-    if (d.segs && !d.corners) {
-        // corners loaded one at a time (rows s and s+1, dynamic_wake/mod.rs:82-103);
-        // each corner's r, |r|^2 and |r| serve the two edges that meet there
-        const double* g = d.segs + 16L * k;
-        const long a = (long)r.cbase;
-        const long idx[4] = {a, a + 1, a + d.P + 1, a + d.P};
-        const V3 r0 = x - load3(d.pts + 3 * idx[0]);
-        const double sq0 = len2(r0), ln0 = sqrt(sq0);
+    if (MODE == MODE_FAST) {
+        const V3 r0 = x - load3(cp[0]);
+        const double sq0 = fma(r0.x, r0.x, fma(r0.y, r0.y, r0.z * r0.z)), ln0 = length_fast(sq0);
         V3 ra = r0;
         double sqa = sq0, lna = ln0;
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
-            const V3 rb = x - load3(d.pts + 3 * idx[e + 1]);
-            const double sqb = len2(rb), lnb = sqrt(sqb);
-            u = u + ring_edge(ra, rb, sqa, sqb, lna, lnb, g + 4 * e, rc);
+            const V3 rb = x - load3(cp[e + 1]);
+            const double sqb = fma(rb.x, rb.x, fma(rb.y, rb.y, rb.z * rb.z)), lnb = length_fast(sqb);
+            u = u + ring_edge_fast(ra, rb, sqa, sqb, lna, lnb, g + 4 * e, rc);
             ra = rb;
             sqa = sqb;
             lna = lnb;
         }
-        return u + ring_edge(ra, r0, sqa, sq0, lna, ln0, g + 12, rc);
+        return u + ring_edge_fast(ra, r0, sqa, sq0, lna, ln0, g + 12, rc);
+    }
+    {
+        bool ok = (rc == 0.0) | ((fabs(rc) >= 0x1p-150) & (fabs(rc) <= 0x1p69));
+        const V3 r0 = x - load3(cp[0]);
+        const double sq0 = len2(r0), ln0 = sqrt_unscaled(sq0);
+        ok = ok & ring_sq_ok(sq0);
+        V3 ra = r0;
+        double sqa = sq0, lna = ln0;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const V3 rb = x - load3(cp[e + 1]);
+            const double sqb = len2(rb), lnb = sqrt_unscaled(sqb);
+            ok = ok & ring_sq_ok(sqb);
+            u = u + ring_edge_unscaled(ra, rb, sqa, sqb, lna, lnb, g + 4 * e, rc, ok);
+            ra = rb;
+            sqa = sqb;
+            lna = lnb;
+        }
+        u = u + ring_edge_unscaled(ra, r0, sqa, sq0, lna, ln0, g + 12, rc, ok);
+        if (ok) return u;
+    }
+    u = V3{0.0, 0.0, 0.0};
+    const V3 r0 = x - load3(cp[0]);
+    const double sq0 = len2(r0), ln0 = sqrt(sq0);
+    V3 ra = r0;
+    double sqa = sq0, lna = ln0;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const V3 rb = x - load3(cp[e + 1]);
+        const double sqb = len2(rb), lnb = sqrt(sqb);
+        u = u + ring_edge(ra, rb, sqa, sqb, lna, lnb, g + 4 * e, rc);
+        ra = rb;
+        sqa = sqb;
+        lna = lnb;
+    }
+    return u + ring_edge(ra, r0, sqa, sq0, lna, ln0, g + 12, rc);
+}
+
+// Near-field data of one panel as staged in a shared tile: the four ring
+// corners (12 doubles, ring order) and the ring-edge geometry (16 doubles).
+constexpr int NEARD = 28;
+
+// panel.rs:99-118 (vortex ring) — used for near pairs only.  With NB the
+// panel's corners and edge geometry come from the shared tile `nb`.
+template <int MODE = MODE_EXACT, bool NB = false>
+__device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
+    double rc = d.core[k];
+    if (NB) return rings_at<MODE>(nb, nb + 3, nb + 6, nb + 9, nb + 12, rc, x);
+    if (d.segs && !d.corners) {
+        // corners: rows s and s+1 of the wake point array (dynamic_wake/mod.rs:82-103)
+        const long a = (long)r.cbase;
+        return rings_at<MODE>(d.pts + 3 * a, d.pts + 3 * (a + 1), d.pts + 3 * (a + d.P + 1), d.pts + 3 * (a + d.P),
+                              d.segs + 16L * k, rc, x);
     }
     V3 c[4];
     panel_corners(d, r, k, c);
+    V3 u{0.0, 0.0, 0.0};
     u = u + segment_unit(c[0], c[1], x, rc);
     u = u + segment_unit(c[1], c[2], x, rc);
     u = u + segment_unit(c[2], c[3], x, rc);
@@ -242,8 +406,8 @@ __device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 
     return F3{(t.x * h3 - (float)r.nx * q) * sc, (t.y * h3 - (float)r.ny * q) * sc, (t.z * h3 - (float)r.nz * q) * sc};
 }
 
-template <int MODE>
-__device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
+template <int MODE, bool NB = false>
+__device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
     if (MODE == MODE_FP32_KAHAN) {
         F3 u = panel_unit_f32(d, r, k, x);
         return V3{(double)u.x, (double)u.y, (double)u.z};
@@ -252,7 +416,7 @@ __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
     double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
     if (MODE == MODE_EXACT) {
         double d2 = tx * tx + ty * ty + tz * tz;
-        if (d2 <= r.far2) return panel_rings(d, r, k, x);
+        if (d2 <= r.far2) return panel_rings<MODE_EXACT, NB>(d, r, k, x, nb);
         // panel.rs:82-96 point doublet
         double nh = tx * r.nx + ty * r.ny + tz * r.nz;
         double h3 = 3.0 * nh;
@@ -261,7 +425,7 @@ __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
         return doublet_quotients(d2, num);
     } else {
         double d2 = fma(tx, tx, fma(ty, ty, tz * tz));
-        if (d2 <= r.far2) return panel_rings(d, r, k, x);
+        if (d2 <= r.far2) return panel_rings<MODE_FAST, NB>(d, r, k, x, nb);
         double y = __builtin_amdgcn_rsq(d2);
         // one Newton step on 1/sqrt(d2): y <- y (3 - d2 y^2) / 2
         double e = fma(-d2 * y, y, 1.0);
@@ -333,15 +497,15 @@ __device__ inline bool far_fast(const PanelRec& r, V3 x, V3& out) {
 // u -/+ u_m bit for bit (a - b is a + (-b) in IEEE arithmetic).  Both points
 // go through one evaluation site: with two separately inlined near-field
 // evaluations the gfx950 compiler returned the first result for both.
-template <int MODE>
-__device__ inline V3 unit_velocity_sym(const Inst& d, const PanelRec& r, int k, V3 x) {
+template <int MODE, bool NB = false>
+__device__ inline V3 unit_velocity_sym(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
     int nq = d.symmetry == 0 ? 1 : 2;
     V3 s{d.symmetry == 1 ? -1.0 : 1.0, d.symmetry == 2 ? -1.0 : 1.0, d.symmetry == 3 ? -1.0 : 1.0};
     V3 out{0.0, 0.0, 0.0};
 #pragma nounroll
     for (int q = 0; q < nq; ++q) {
         V3 p = q == 0 ? x : V3{x.x * s.x, x.y * s.y, x.z * s.z};
-        V3 v = panel_unit<MODE>(d, r, k, p);
+        V3 v = panel_unit<MODE, NB>(d, r, k, p, nb);
         out = q == 0 ? v : V3{out.x + s.x * v.x, out.y + s.y * v.y, out.z + s.z * v.z};
     }
     return out;
@@ -353,10 +517,10 @@ __device__ inline V3 unit_velocity_sym(const Inst& d, const PanelRec& r, int k, 
 constexpr int TILE = 256;  // panel records per LDS tile (20 KiB)
 constexpr int RECD = (int)(sizeof(PanelRec) / sizeof(double));
 
-template <int MODE, int SYM>
-__device__ inline V3 pair_unit(const Inst& d, const PanelRec& r, int k, V3 x) {
-    if (SYM) return unit_velocity_sym<MODE>(d, r, k, x);
-    return panel_unit<MODE>(d, r, k, x);
+template <int MODE, int SYM, bool NB = false>
+__device__ inline V3 pair_unit(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
+    if (SYM) return unit_velocity_sym<MODE, NB>(d, r, k, x, nb);
+    return panel_unit<MODE, NB>(d, r, k, x, nb);
 }
 
 // Folds panels [p0, p1) into `acc` for this lane's target x, in ascending
@@ -396,25 +560,46 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
     return acc + V3{(double)sx - (double)cx, (double)sy - (double)cy, (double)sz - (double)cz};
 }
 
-template <int MODE, int SYM, int TL = TILE>
-__device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1) {
+// With NB (free wake with ring-edge geometry, no explicit corner array) the
+// tile also stages each panel's near-field data (NEARD doubles in `nbuf`), so
+// that the lanes taking the vortex-ring branch read it from the shared tile.
+template <int MODE, int SYM, int TL = TILE, bool NB = false>
+__device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, double* nbuf = nullptr) {
     if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL>(d, tile, x, acc, p0, p1);
+    const long n_pts = (long)d.P * (d.M + 1);
     for (long base = p0; base < p1; base += TL) {
         int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
         __syncthreads();
         const double* src = reinterpret_cast<const double*>(d.rec + base);
         double* dst = reinterpret_cast<double*>(tile);
         for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
+        if (NB) {
+            // corners in ring order: rows s and s+1 (dynamic_wake/mod.rs:82-103)
+            for (int q = threadIdx.x; q < cnt * 4; q += blockDim.x) {
+                const int p = q >> 2, c = q & 3;
+                const long a = (long)d.rec[base + p].cbase;
+                const long i = c == 0 ? a : c == 1 ? a + 1 : c == 2 ? a + d.P + 1 : a + d.P;
+                double* o = nbuf + NEARD * p + 3 * c;
+                const bool in = a >= 0 && i < n_pts;
+                o[0] = in ? d.pts[3 * i] : 0.0;
+                o[1] = in ? d.pts[3 * i + 1] : 0.0;
+                o[2] = in ? d.pts[3 * i + 2] : 0.0;
+            }
+            for (int q = threadIdx.x; q < cnt * 16; q += blockDim.x)
+                nbuf[NEARD * (q >> 4) + 12 + (q & 15)] = d.segs[16L * base + q];
+        }
         __syncthreads();
         // Two records in flight, in alternating registers: b is read while a
         // is evaluated and the next a while b is evaluated.
         PanelRec a = tile[0];
         for (int p = 0; p < cnt; p += 2) {
             PanelRec b = tile[p + 1 < cnt ? p + 1 : p];
-            if (a.strength != 0.0) acc = acc + pair_unit<MODE, SYM>(d, a, (int)(base + p), x) * a.strength;
+            if (a.strength != 0.0)
+                acc = acc + pair_unit<MODE, SYM, NB>(d, a, (int)(base + p), x, NB ? nbuf + NEARD * p : nullptr) * a.strength;
             if (p + 1 >= cnt) break;
             a = tile[p + 2 < cnt ? p + 2 : p + 1];
-            if (b.strength != 0.0) acc = acc + pair_unit<MODE, SYM>(d, b, (int)(base + p + 1), x) * b.strength;
+            if (b.strength != 0.0)
+                acc = acc + pair_unit<MODE, SYM, NB>(d, b, (int)(base + p + 1), x, NB ? nbuf + NEARD * (p + 1) : nullptr) * b.strength;
         }
     }
     return acc;
@@ -455,10 +640,16 @@ struct McPlan {
     int spin_limit;
 };
 
-template <int MODE, int SYM>
+// NB: near-field data staged with the records (sweep), in tiles of NB_TILE
+// panels so that four workgroups still fit a CU's shared memory.
+constexpr int NB_TILE = 128;
+
+template <int MODE, int SYM, bool NB = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_targets_mc(Inst d, const double* __restrict__ targets, long n_targets,
                                                     double* __restrict__ out, int add, McPlan pl) {
-    __shared__ PanelRec tile[TILE];
+    constexpr int TL = NB ? NB_TILE : TILE;
+    __shared__ PanelRec tile[TL];
+    __shared__ double nbuf[NB ? NB_TILE * NEARD : 1];
     __shared__ int ready;
     const long L = pl.L;
     const long g0 = (long)blockIdx.x * pl.C;
@@ -498,7 +689,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 start = 0;  // recompute the first panels here: same order, same bits
             }
         }
-        acc = sweep<MODE, SYM>(d, tile, x, acc, start, p1);
+        acc = sweep<MODE, SYM, TL, NB>(d, tile, x, acc, start, p1, nbuf);
         if (head) {
             store3(part, acc);
             __threadfence();
@@ -2539,11 +2730,11 @@ hipError_t launch_velocity_copy(const Inst& d, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int MODE, int SYM>
+template <int MODE, int SYM, bool NB>
 static int mc_slots_for() {
     int per_cu = 0, cus = 0, dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_targets_mc<MODE, SYM>, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_targets_mc<MODE, SYM, NB>, 256, 0) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
     return per_cu * cus;
 }
@@ -2552,10 +2743,14 @@ template <int MODE, int SYM>
 static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_targets, long n_panels, double* out,
                                    int add, McBuffers* mc, hipStream_t st) {
     long n_jobs = (n_targets + 255) / 256;
+    // near-field data staged in shared memory when the panels have ring-edge geometry
+    constexpr bool can_stage = MODE != MODE_FP32_KAHAN;
+    const bool nb = can_stage && d.segs && !d.corners;
     int slots = 0;
     if (mc) {
-        if (mc->slots[MODE][SYM] == 0) mc->slots[MODE][SYM] = mc_slots_for<MODE, SYM>();
-        slots = mc->slots[MODE][SYM];
+        if (mc->slots[MODE][SYM][nb] == 0)
+            mc->slots[MODE][SYM][nb] = nb ? mc_slots_for<MODE, SYM, can_stage>() : mc_slots_for<MODE, SYM, false>();
+        slots = mc->slots[MODE][SYM][nb];
     }
     if (mc && slots > 0 && n_jobs > slots && mc->cap_jobs >= n_jobs && n_panels > 0) {
         McPlan pl;
@@ -2567,8 +2762,12 @@ static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_
         pl.epoch = ++mc->epoch;
         pl.spin_limit = 200000;
         long grid = (n_jobs * n_panels + pl.C - 1) / pl.C;
-        hipLaunchKernelGGL((k_targets_mc<MODE, SYM>), dim3((unsigned)grid), dim3(256), 0, st, d, targets, n_targets,
-                           out, add, pl);
+        if (nb)
+            hipLaunchKernelGGL((k_targets_mc<MODE, SYM, can_stage>), dim3((unsigned)grid), dim3(256), 0, st, d, targets,
+                               n_targets, out, add, pl);
+        else
+            hipLaunchKernelGGL((k_targets_mc<MODE, SYM, false>), dim3((unsigned)grid), dim3(256), 0, st, d, targets,
+                               n_targets, out, add, pl);
     } else {
         hipLaunchKernelGGL((k_targets<MODE, SYM>), dim3((unsigned)n_jobs), dim3(256), 0, st, d, targets, n_targets,
                            n_panels, out, add);
