@@ -229,6 +229,9 @@ struct sb_batch {
         bd.solve_shm = solve_shared_bytes(max_n, max_w);
         bd.counters = d_counters;
         bd.sym = sym;
+        bd.stage_near = 1;
+        for (auto& s : sims)
+            if (std::min<long>(s->setup.wake.end_index, (long)s->P * (s->M + 1)) > 0 && !s->d_segs) bd.stage_near = 0;
         bd.fixed_jobs = d_jobs;
         bd.target_jobs = d_jobs + cap_fixed_jobs;
         set_tuning(bd.fixed_per, bd.targets_tile);
@@ -453,8 +456,8 @@ struct sb_batch {
         if (targets_tile != 64 && targets_tile != 128 && targets_tile != 256)
             throw std::invalid_argument("targets_tile must be 64, 128 or 256");
         check(hipSetDevice(device), "hipSetDevice");
-        int fg = batch_resident_groups(0, mode, bd.sym, fixed_per);
-        int tg = batch_resident_groups(1, mode, bd.sym, targets_tile);
+        int fg = batch_resident_groups(0, mode, bd.sym, fixed_per, 0);
+        int tg = batch_resident_groups(1, mode, bd.sym, targets_tile, bd.stage_near);
         if (fg <= 0 || tg <= 0) throw HipError("occupancy query failed", -1);
         bd.fixed_per = fixed_per;
         bd.targets_tile = targets_tile;

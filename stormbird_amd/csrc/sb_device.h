@@ -34,6 +34,10 @@ struct GeoLayout {
 };
 
 // Instance descriptor passed by value to every kernel.
+// Doubles per panel of the near-field record (Inst::segs): 12 corner coordinates
+// and 4 x (unit direction, length) of the ring edges.
+constexpr int NEARD = 28;
+
 struct Inst {
     int N, W, P, M;     // strips, wings, span points, wake rows (panels per strip)
     int nz_rows;        // leading panel rows that may hold nonzero strength
@@ -70,9 +74,9 @@ struct Inst {
     const double* core;  // viscous core length [N*M]
     const double* corners;  // optional explicit corners [N*M][12] (standalone panel sets)
     PanelRec* rec;       // far-field panel records [N*M]
-    double* segs;        // optional [N*M][16]: per ring edge e (corner e to e+1) its unit
-                         // direction and length, the target-independent part of the
-                         // viscous core term; null: computed per pair
+    double* segs;        // optional [N*M][NEARD]: the ring's four corners (ring order), then per
+                         // ring edge e (corner e to e+1) its unit direction and length, the
+                         // target-independent part of the viscous core term; null: computed per pair
     const int* col;      // [N] span-point column of strip i's first corner (i + wing index)
     const int* wing_of;  // [N]
     const int* span_src; // [P][2] ctrl-point indices mapped to span point j
@@ -186,6 +190,8 @@ struct BatchDev {
     int fixed_grid = 0, target_grid = 0;  // persistent workgroups
     int fixed_per = 3;            // kb_fixed: panels per compute wave per block (1, 2, 3)
     int targets_tile = 256;       // kb_targets: panel records per LDS tile (64, 128, 256)
+    int stage_near = 0;           // kb_targets: 1 when every self-induced instance has ring-edge
+                                  // geometry (near-field data staged with the records)
     int sym = 0;                  // 1 when any instance uses a symmetry plane
     // learned warm start (sb_mlp): per-strip features [n_strips][64], scale, prediction
     const long* strip_off = nullptr;  // device [n + 1]: first strip of each instance
@@ -227,6 +233,6 @@ hipError_t launch_batch_take_singular(const BatchDev& b, double* flags, hipStrea
 // Resident workgroups of the persistent pair-sum kernels (0 fixed, 1 self-induction).
 // Resident workgroups of kb_fixed (which 0; variant = panels per wave) or
 // kb_targets (which 1; variant = tile length).
-int batch_resident_groups(int which, int mode, int sym, int variant);
+int batch_resident_groups(int which, int mode, int sym, int variant, int stage_near);
 
 }  // namespace sb

@@ -277,7 +277,7 @@ struct sb_sim {
         d_core = dalloc<double>(nr_panels);
         d_rec = dalloc<PanelRec>(nr_panels);
         // ring-edge geometry: where near pairs are many (a self-induced free wake)
-        if (!setup.quasi && setup.wake.end_index > 0) d_segs = dalloc<double>(16 * nr_panels);
+        if (!setup.quasi && setup.wake.end_index > 0) d_segs = dalloc<double>(NEARD * nr_panels);
         d_geo = dalloc<double>(gl.total);
         d_fs = dalloc<double>(3 * (N + nr_points));
         d_felt = dalloc<double>(3 * N);

@@ -280,21 +280,15 @@ __device__ inline V3 rings_at(const double* c0, const double* c1, const double* 
     return u + ring_edge(ra, r0, sqa, sq0, lna, ln0, g + 12, rc);
 }
 
-// Near-field data of one panel as staged in a shared tile: the four ring
-// corners (12 doubles, ring order) and the ring-edge geometry (16 doubles).
-constexpr int NEARD = 28;
-
-// panel.rs:99-118 (vortex ring) — used for near pairs only.  With NB the
-// panel's corners and edge geometry come from the shared tile `nb`.
+// panel.rs:99-118 (vortex ring) — used for near pairs only.  The panel's
+// near-field record (corners and edge geometry, NEARD doubles) comes from
+// Inst::segs, or with NB from its copy `nb` in the shared tile.
 template <int MODE = MODE_EXACT, bool NB = false>
 __device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
     double rc = d.core[k];
-    if (NB) return rings_at<MODE>(nb, nb + 3, nb + 6, nb + 9, nb + 12, rc, x);
-    if (d.segs && !d.corners) {
-        // corners: rows s and s+1 of the wake point array (dynamic_wake/mod.rs:82-103)
-        const long a = (long)r.cbase;
-        return rings_at<MODE>(d.pts + 3 * a, d.pts + 3 * (a + 1), d.pts + 3 * (a + d.P + 1), d.pts + 3 * (a + d.P),
-                              d.segs + 16L * k, rc, x);
+    if (NB || (d.segs && !d.corners)) {
+        const double* q = NB ? nb : d.segs + (long)NEARD * k;
+        return rings_at<MODE>(q, q + 3, q + 6, q + 9, q + 12, rc, x);
     }
     V3 c[4];
     panel_corners(d, r, k, c);
@@ -566,7 +560,6 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
 template <int MODE, int SYM, int TL = TILE, bool NB = false>
 __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, double* nbuf = nullptr) {
     if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL>(d, tile, x, acc, p0, p1);
-    const long n_pts = (long)d.P * (d.M + 1);
     for (long base = p0; base < p1; base += TL) {
         int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
         __syncthreads();
@@ -574,19 +567,8 @@ __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0,
         double* dst = reinterpret_cast<double*>(tile);
         for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
         if (NB) {
-            // corners in ring order: rows s and s+1 (dynamic_wake/mod.rs:82-103)
-            for (int q = threadIdx.x; q < cnt * 4; q += blockDim.x) {
-                const int p = q >> 2, c = q & 3;
-                const long a = (long)d.rec[base + p].cbase;
-                const long i = c == 0 ? a : c == 1 ? a + 1 : c == 2 ? a + d.P + 1 : a + d.P;
-                double* o = nbuf + NEARD * p + 3 * c;
-                const bool in = a >= 0 && i < n_pts;
-                o[0] = in ? d.pts[3 * i] : 0.0;
-                o[1] = in ? d.pts[3 * i + 1] : 0.0;
-                o[2] = in ? d.pts[3 * i + 2] : 0.0;
-            }
-            for (int q = threadIdx.x; q < cnt * 16; q += blockDim.x)
-                nbuf[NEARD * (q >> 4) + 12 + (q & 15)] = d.segs[16L * base + q];
+            const double* nsrc = d.segs + (long)NEARD * base;
+            for (int q = threadIdx.x; q < cnt * NEARD; q += blockDim.x) nbuf[q] = nsrc[q];
         }
         __syncthreads();
         // Two records in flight, in alternating registers: b is read while a
@@ -1084,17 +1066,23 @@ __device__ inline void panels_body(const Inst& d, int shift, long k) {
     }
     r.strength = g;
     d.rec[k] = r;
-    if (d.segs) {  // viscous_core_term's rel_line, its length and unit direction, per edge
-        double* q = d.segs + 16L * k;
+    if (d.segs) {  // the corners, then viscous_core_term's rel_line length and unit direction per edge
+        double* q = d.segs + (long)NEARD * k;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            q[3 * e] = c[e].x;
+            q[3 * e + 1] = c[e].y;
+            q[3 * e + 2] = c[e].z;
+        }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             V3 rel_line = c[(e + 1) & 3] - c[e];
             double line_len = sqrt(len2(rel_line));
             V3 dir = rel_line / line_len;
-            q[4 * e] = dir.x;
-            q[4 * e + 1] = dir.y;
-            q[4 * e + 2] = dir.z;
-            q[4 * e + 3] = line_len;
+            q[12 + 4 * e] = dir.x;
+            q[12 + 4 * e + 1] = dir.y;
+            q[12 + 4 * e + 2] = dir.z;
+            q[12 + 4 * e + 3] = line_len;
         }
     }
 }
@@ -2353,10 +2341,11 @@ __global__ __launch_bounds__(1024) void kb_fixed(const Inst* I, const int2* __re
 // one instance's 256 wake points over all its panels, one target per lane,
 // panel records staged through LDS as in k_targets.
 // TL panel records per LDS tile (64, 128 or 256; the autotuner picks it).
-template <int MODE, int SYM, int TL>
+template <int MODE, int SYM, int TL, bool NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_targets(
     const Inst* I, const int2* __restrict__ jobs, int n_jobs, unsigned* head) {
     __shared__ PanelRec tile[TL];
+    __shared__ double nbuf[NB ? TL * NEARD : 1];
     __shared__ int slot;
     for (;;) {
         int j = next_job(head, &slot);
@@ -2368,7 +2357,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         long t = (long)jb.y * 256 + threadIdx.x;
         long tc = t < end ? t : end - 1;
         V3 x = load3(d.pts + 3 * tc);
-        V3 acc = sweep<MODE, SYM, TL>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, (long)d.N * d.nz_rows);
+        V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, (long)d.N * d.nz_rows, nbuf);
         if (t < end) store3(d.vel + 3 * t, load3(d.vel + 3 * t) + acc);
     }
 }
@@ -2979,8 +2968,12 @@ hipError_t launch_batch_velocity_copy(const BatchDev& b, hipStream_t st) {
 
 template <int MODE, int SYM, int TL>
 static void launch_kb_targets(const BatchDev& b, hipStream_t st) {
-    hipLaunchKernelGGL((kb_targets<MODE, SYM, TL>), dim3(b.target_grid), dim3(256), 0, st, b.inst, b.target_jobs,
-                       b.n_target_jobs, b.counters + 1);
+    if (b.stage_near)
+        hipLaunchKernelGGL((kb_targets<MODE, SYM, TL, true>), dim3(b.target_grid), dim3(256), 0, st, b.inst,
+                           b.target_jobs, b.n_target_jobs, b.counters + 1);
+    else
+        hipLaunchKernelGGL((kb_targets<MODE, SYM, TL, false>), dim3(b.target_grid), dim3(256), 0, st, b.inst,
+                           b.target_jobs, b.n_target_jobs, b.counters + 1);
 }
 
 template <int MODE, int SYM>
@@ -3021,20 +3014,28 @@ static int fixed_groups(int per) {
     return resident_groups(kb_fixed<MODE, SYM, 3>, th);
 }
 
-template <int MODE, int SYM>
+template <int MODE, int SYM, bool NB>
 static int target_groups(int tile) {
-    if (tile == 64) return resident_groups(kb_targets<MODE, SYM, 64>, 256);
-    if (tile == 128) return resident_groups(kb_targets<MODE, SYM, 128>, 256);
-    return resident_groups(kb_targets<MODE, SYM, 256>, 256);
+    if (tile == 64) return resident_groups(kb_targets<MODE, SYM, 64, NB>, 256);
+    if (tile == 128) return resident_groups(kb_targets<MODE, SYM, 128, NB>, 256);
+    return resident_groups(kb_targets<MODE, SYM, 256, NB>, 256);
 }
 
-int batch_resident_groups(int which, int mode, int sym, int variant) {
+template <int MODE, int SYM>
+static int target_groups_staged(int tile, int stage_near) {
+    return stage_near ? target_groups<MODE, SYM, true>(tile) : target_groups<MODE, SYM, false>(tile);
+}
+
+int batch_resident_groups(int which, int mode, int sym, int variant, int stage_near) {
     if (which == 0) {
         if (mode == MODE_FAST) return sym ? fixed_groups<MODE_FAST, 1>(variant) : fixed_groups<MODE_FAST, 0>(variant);
         return sym ? fixed_groups<MODE_EXACT, 1>(variant) : fixed_groups<MODE_EXACT, 0>(variant);
     }
-    if (mode == MODE_FAST) return sym ? target_groups<MODE_FAST, 1>(variant) : target_groups<MODE_FAST, 0>(variant);
-    return sym ? target_groups<MODE_EXACT, 1>(variant) : target_groups<MODE_EXACT, 0>(variant);
+    if (mode == MODE_FAST)
+        return sym ? target_groups_staged<MODE_FAST, 1>(variant, stage_near)
+                   : target_groups_staged<MODE_FAST, 0>(variant, stage_near);
+    return sym ? target_groups_staged<MODE_EXACT, 1>(variant, stage_near)
+               : target_groups_staged<MODE_EXACT, 0>(variant, stage_near);
 }
 
 }  // namespace sb
