@@ -380,12 +380,20 @@ __device__ inline F3 segment_unit_f32(V3 l0, V3 l1, V3 x, float rc) {
     return F3{0.0f, 0.0f, 0.0f};
 }
 
-__device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 x) {
+template <bool NB = false>
+__device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
     double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
     double d2 = tx * tx + ty * ty + tz * tz;
     if (d2 <= r.far2) {
         V3 c[4];
-        panel_corners(d, r, k, c);
+        if (NB) {
+            c[0] = load3(nb);
+            c[1] = load3(nb + 3);
+            c[2] = load3(nb + 6);
+            c[3] = load3(nb + 9);
+        } else {
+            panel_corners(d, r, k, c);
+        }
         float rc = (float)d.core[k];
         F3 a = segment_unit_f32(c[0], c[1], x, rc), b = segment_unit_f32(c[1], c[2], x, rc);
         F3 e = segment_unit_f32(c[2], c[3], x, rc), f = segment_unit_f32(c[3], c[0], x, rc);
@@ -403,7 +411,7 @@ __device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 
 template <int MODE, bool NB = false>
 __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
     if (MODE == MODE_FP32_KAHAN) {
-        F3 u = panel_unit_f32(d, r, k, x);
+        F3 u = panel_unit_f32<NB>(d, r, k, x, nb);
         return V3{(double)u.x, (double)u.y, (double)u.z};
     }
     // panel.rs:53-76: near/far switch on |x - c|^2 <= far^2
@@ -531,8 +539,8 @@ __device__ inline void kahan_add(float& s, float& c, float v) {
     s = t;
 }
 
-template <int SYM, int TL = TILE>
-__device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1) {
+template <int SYM, int TL = TILE, bool NB = false>
+__device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, double* nbuf) {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
     for (long base = p0; base < p1; base += TL) {
         int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
@@ -540,11 +548,15 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
         const double* src = reinterpret_cast<const double*>(d.rec + base);
         double* dst = reinterpret_cast<double*>(tile);
         for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
+        if (NB) {
+            const double* nsrc = d.segs + (long)NEARD * base;
+            for (int q = threadIdx.x; q < cnt * NEARD; q += blockDim.x) nbuf[q] = nsrc[q];
+        }
         __syncthreads();
         for (int p = 0; p < cnt; ++p) {
             const PanelRec& r = tile[p];
             if (r.strength == 0.0) continue;
-            V3 u = pair_unit<MODE_FP32_KAHAN, SYM>(d, r, (int)(base + p), x);
+            V3 u = pair_unit<MODE_FP32_KAHAN, SYM, NB>(d, r, (int)(base + p), x, NB ? nbuf + NEARD * p : nullptr);
             float g = (float)r.strength;
             kahan_add(sx, cx, (float)u.x * g);
             kahan_add(sy, cy, (float)u.y * g);
@@ -559,7 +571,7 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
 // that the lanes taking the vortex-ring branch read it from the shared tile.
 template <int MODE, int SYM, int TL = TILE, bool NB = false>
 __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, double* nbuf = nullptr) {
-    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL>(d, tile, x, acc, p0, p1);
+    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL, NB>(d, tile, x, acc, p0, p1, nbuf);
     for (long base = p0; base < p1; base += TL) {
         int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
         __syncthreads();
@@ -587,17 +599,24 @@ __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0,
     return acc;
 }
 
+// NB: near-field data staged with the records (sweep), in tiles of NB_TILE
+// panels so that four workgroups still fit a CU's shared memory.
+constexpr int NB_TILE = 128;
+
 // One target per lane, the whole panel range per block.  out[t] += acc when
 // `add` is set (update_velocity_at_points, update_data.rs:125-141), else
 // out[t] = acc.
-template <int MODE, int SYM>
+template <int MODE, int SYM, bool NB = false>
 __global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restrict__ targets, long n_targets,
                                                  long n_panels, double* __restrict__ out, int add) {
-    __shared__ PanelRec tile[TILE];
+    // staged tiles of 64 panels (19.5 KB): eight workgroups per CU, as without staging
+    constexpr int TL = NB ? 64 : TILE;
+    __shared__ PanelRec tile[TL];
+    __shared__ double nbuf[NB ? TL * NEARD : 1];
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long tc = t < n_targets ? t : n_targets - 1;
     V3 x = load3(targets + 3 * tc);
-    V3 acc = sweep<MODE, SYM>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels);
+    V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels, nbuf);
     if (t < n_targets) {
         if (add) store3(out + 3 * t, load3(out + 3 * t) + acc);
         else store3(out + 3 * t, acc);
@@ -621,10 +640,6 @@ struct McPlan {
     long long epoch;
     int spin_limit;
 };
-
-// NB: near-field data staged with the records (sweep), in tiles of NB_TILE
-// panels so that four workgroups still fit a CU's shared memory.
-constexpr int NB_TILE = 128;
 
 template <int MODE, int SYM, bool NB = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_targets_mc(Inst d, const double* __restrict__ targets, long n_targets,
@@ -2733,12 +2748,11 @@ static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_
                                    int add, McBuffers* mc, hipStream_t st) {
     long n_jobs = (n_targets + 255) / 256;
     // near-field data staged in shared memory when the panels have ring-edge geometry
-    constexpr bool can_stage = MODE != MODE_FP32_KAHAN;
-    const bool nb = can_stage && d.segs && !d.corners;
+    const bool nb = d.segs && !d.corners;
     int slots = 0;
     if (mc) {
         if (mc->slots[MODE][SYM][nb] == 0)
-            mc->slots[MODE][SYM][nb] = nb ? mc_slots_for<MODE, SYM, can_stage>() : mc_slots_for<MODE, SYM, false>();
+            mc->slots[MODE][SYM][nb] = nb ? mc_slots_for<MODE, SYM, true>() : mc_slots_for<MODE, SYM, false>();
         slots = mc->slots[MODE][SYM][nb];
     }
     if (mc && slots > 0 && n_jobs > slots && mc->cap_jobs >= n_jobs && n_panels > 0) {
@@ -2752,14 +2766,18 @@ static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_
         pl.spin_limit = 200000;
         long grid = (n_jobs * n_panels + pl.C - 1) / pl.C;
         if (nb)
-            hipLaunchKernelGGL((k_targets_mc<MODE, SYM, can_stage>), dim3((unsigned)grid), dim3(256), 0, st, d, targets,
+            hipLaunchKernelGGL((k_targets_mc<MODE, SYM, true>), dim3((unsigned)grid), dim3(256), 0, st, d, targets,
                                n_targets, out, add, pl);
         else
             hipLaunchKernelGGL((k_targets_mc<MODE, SYM, false>), dim3((unsigned)grid), dim3(256), 0, st, d, targets,
                                n_targets, out, add, pl);
     } else {
-        hipLaunchKernelGGL((k_targets<MODE, SYM>), dim3((unsigned)n_jobs), dim3(256), 0, st, d, targets, n_targets,
-                           n_panels, out, add);
+        if (nb)
+            hipLaunchKernelGGL((k_targets<MODE, SYM, true>), dim3((unsigned)n_jobs), dim3(256), 0, st, d, targets,
+                               n_targets, n_panels, out, add);
+        else
+            hipLaunchKernelGGL((k_targets<MODE, SYM, false>), dim3((unsigned)n_jobs), dim3(256), 0, st, d, targets,
+                               n_targets, n_panels, out, add);
     }
     return hipGetLastError();
 }
