@@ -525,6 +525,72 @@ __device__ inline V3 pair_unit(const Inst& d, const PanelRec& r, int k, V3 x, co
     return panel_unit<MODE, NB>(d, r, k, x, nb);
 }
 
+// Near-field staging (NB): the staged records, the block's bounding box of
+// evaluation points and the per-wave scratch of the tile test.  A tile's
+// near-field records are staged only when some panel of the tile may be within
+// its near-field radius of the box; otherwise no lane of the block can take the
+// vortex-ring branch for that tile and the buffer is not read.
+struct NearShared {
+    double red[6][16];  // per-wave partial bounds
+    double box[6];      // the block's box: min x, y, z, then -max x, y, z
+    int flag[16];       // per-wave: a panel of the tile may be near
+};
+struct NearCtx {
+    double* nbuf = nullptr;       // [TL][NEARD] shared
+    NearShared* sh = nullptr;     // shared
+};
+
+// Bounding box of the block's evaluation points: the targets and, with a
+// symmetry plane, their mirror images (unit_velocity_sym).  Every thread of
+// the block calls it.
+__device__ inline void block_box(const Inst& d, V3 x, NearCtx& nc) {
+    V3 m = x;
+    if (d.symmetry != 0)
+        m = V3{d.symmetry == 1 ? -x.x : x.x, d.symmetry == 2 ? -x.y : x.y, d.symmetry == 3 ? -x.z : x.z};
+    double v[6] = {fmin(x.x, m.x), fmin(x.y, m.y), fmin(x.z, m.z), -fmax(x.x, m.x), -fmax(x.y, m.y), -fmax(x.z, m.z)};
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+        for (int off = WAVE / 2; off > 0; off >>= 1) v[i] = fmin(v[i], __shfl_xor(v[i], off, WAVE));
+    const int nw = (int)blockDim.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+        for (int i = 0; i < 6; ++i) nc.sh->red[i][threadIdx.x / WAVE] = v[i];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double a = nc.sh->red[threadIdx.x][0];
+        for (int q = 1; q < nw; ++q) a = fmin(a, nc.sh->red[threadIdx.x][q]);
+        nc.sh->box[threadIdx.x] = a;
+    }
+    __syncthreads();
+}
+
+// Publishes the record tile (barrier) and stages the tile's near-field records
+// when a panel of it may be near the block's box.  The test keeps a relative
+// margin of 2^-20 on the radius squared, far above the rounding of the lanes'
+// own distance (a lane inside the box is no closer to the panel centre than the
+// box is), and treats NaN as near.
+__device__ inline void stage_near(const Inst& d, NearCtx& nc, long base, int cnt) {
+    int want = 0;
+    const double* bx = nc.sh->box;
+    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
+        const PanelRec& r = d.rec[base + q];
+        double dx = fmax(fmax(bx[0] - r.cx, r.cx + bx[3]), 0.0);
+        double dy = fmax(fmax(bx[1] - r.cy, r.cy + bx[4]), 0.0);
+        double dz = fmax(fmax(bx[2] - r.cz, r.cz + bx[5]), 0.0);
+        double db2 = dx * dx + dy * dy + dz * dz;
+        want |= (r.strength != 0.0) && !(db2 > r.far2 * (1.0 + 0x1p-20));
+    }
+    const unsigned long long bal = __ballot(want);
+    if ((threadIdx.x & (WAVE - 1)) == 0) nc.sh->flag[threadIdx.x / WAVE] = bal != 0ull;
+    __syncthreads();
+    int need = 0;
+    for (int q = 0; q < (int)blockDim.x / WAVE; ++q) need |= nc.sh->flag[q];
+    if (need) {
+        const double* nsrc = d.segs + (long)NEARD * base;
+        for (int q = threadIdx.x; q < cnt * NEARD; q += blockDim.x) nc.nbuf[q] = nsrc[q];
+        __syncthreads();
+    }
+}
+
 // Folds panels [p0, p1) into `acc` for this lane's target x, in ascending
 // panel order: acc = acc + strength * u  (induced_velocity_calc.rs:111-126;
 // a zero-strength panel adds nothing, :202-210).  Every thread of the block
@@ -540,7 +606,7 @@ __device__ inline void kahan_add(float& s, float& c, float v) {
 }
 
 template <int SYM, int TL = TILE, bool NB = false>
-__device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, double* nbuf) {
+__device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, NearCtx& nc) {
     float sx = 0.0f, sy = 0.0f, sz = 0.0f, cx = 0.0f, cy = 0.0f, cz = 0.0f;
     for (long base = p0; base < p1; base += TL) {
         int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
@@ -548,15 +614,12 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
         const double* src = reinterpret_cast<const double*>(d.rec + base);
         double* dst = reinterpret_cast<double*>(tile);
         for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
-        if (NB) {
-            const double* nsrc = d.segs + (long)NEARD * base;
-            for (int q = threadIdx.x; q < cnt * NEARD; q += blockDim.x) nbuf[q] = nsrc[q];
-        }
-        __syncthreads();
+        if (NB) stage_near(d, nc, base, cnt);
+        else __syncthreads();
         for (int p = 0; p < cnt; ++p) {
             const PanelRec& r = tile[p];
             if (r.strength == 0.0) continue;
-            V3 u = pair_unit<MODE_FP32_KAHAN, SYM, NB>(d, r, (int)(base + p), x, NB ? nbuf + NEARD * p : nullptr);
+            V3 u = pair_unit<MODE_FP32_KAHAN, SYM, NB>(d, r, (int)(base + p), x, NB ? nc.nbuf + NEARD * p : nullptr);
             float g = (float)r.strength;
             kahan_add(sx, cx, (float)u.x * g);
             kahan_add(sy, cy, (float)u.y * g);
@@ -570,30 +633,27 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
 // tile also stages each panel's near-field data (NEARD doubles in `nbuf`), so
 // that the lanes taking the vortex-ring branch read it from the shared tile.
 template <int MODE, int SYM, int TL = TILE, bool NB = false>
-__device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, double* nbuf = nullptr) {
-    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL, NB>(d, tile, x, acc, p0, p1, nbuf);
+__device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, NearCtx& nc) {
+    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL, NB>(d, tile, x, acc, p0, p1, nc);
     for (long base = p0; base < p1; base += TL) {
         int cnt = (int)((p1 - base) < TL ? (p1 - base) : TL);
         __syncthreads();
         const double* src = reinterpret_cast<const double*>(d.rec + base);
         double* dst = reinterpret_cast<double*>(tile);
         for (int q = threadIdx.x; q < cnt * RECD; q += blockDim.x) dst[q] = src[q];
-        if (NB) {
-            const double* nsrc = d.segs + (long)NEARD * base;
-            for (int q = threadIdx.x; q < cnt * NEARD; q += blockDim.x) nbuf[q] = nsrc[q];
-        }
-        __syncthreads();
+        if (NB) stage_near(d, nc, base, cnt);
+        else __syncthreads();
         // Two records in flight, in alternating registers: b is read while a
         // is evaluated and the next a while b is evaluated.
         PanelRec a = tile[0];
         for (int p = 0; p < cnt; p += 2) {
             PanelRec b = tile[p + 1 < cnt ? p + 1 : p];
             if (a.strength != 0.0)
-                acc = acc + pair_unit<MODE, SYM, NB>(d, a, (int)(base + p), x, NB ? nbuf + NEARD * p : nullptr) * a.strength;
+                acc = acc + pair_unit<MODE, SYM, NB>(d, a, (int)(base + p), x, NB ? nc.nbuf + NEARD * p : nullptr) * a.strength;
             if (p + 1 >= cnt) break;
             a = tile[p + 2 < cnt ? p + 2 : p + 1];
             if (b.strength != 0.0)
-                acc = acc + pair_unit<MODE, SYM, NB>(d, b, (int)(base + p + 1), x, NB ? nbuf + NEARD * (p + 1) : nullptr) * b.strength;
+                acc = acc + pair_unit<MODE, SYM, NB>(d, b, (int)(base + p + 1), x, NB ? nc.nbuf + NEARD * (p + 1) : nullptr) * b.strength;
         }
     }
     return acc;
@@ -613,10 +673,15 @@ __global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restric
     constexpr int TL = NB ? 64 : TILE;
     __shared__ PanelRec tile[TL];
     __shared__ double nbuf[NB ? TL * NEARD : 1];
+    __shared__ NearShared nsh;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     long tc = t < n_targets ? t : n_targets - 1;
     V3 x = load3(targets + 3 * tc);
-    V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels, nbuf);
+    NearCtx nc;
+    nc.nbuf = nbuf;
+    nc.sh = &nsh;
+    if (NB) block_box(d, x, nc);
+    V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels, nc);
     if (t < n_targets) {
         if (add) store3(out + 3 * t, load3(out + 3 * t) + acc);
         else store3(out + 3 * t, acc);
@@ -647,7 +712,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     constexpr int TL = NB ? NB_TILE : TILE;
     __shared__ PanelRec tile[TL];
     __shared__ double nbuf[NB ? NB_TILE * NEARD : 1];
+    __shared__ NearShared nsh;
     __shared__ int ready;
+    NearCtx nc;
+    nc.nbuf = nbuf;
+    nc.sh = &nsh;
     const long L = pl.L;
     const long g0 = (long)blockIdx.x * pl.C;
     const long gend = pl.n_jobs * L;
@@ -663,6 +732,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const long t = j * 256 + threadIdx.x;
         const long tc = t < n_targets ? t : n_targets - 1;
         V3 x = load3(targets + 3 * tc);
+        if (NB) block_box(d, x, nc);
         V3 acc{0.0, 0.0, 0.0};
         double* part = pl.partial + 3 * t;
         long start = p0;
@@ -686,7 +756,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 start = 0;  // recompute the first panels here: same order, same bits
             }
         }
-        acc = sweep<MODE, SYM, TL, NB>(d, tile, x, acc, start, p1, nbuf);
+        acc = sweep<MODE, SYM, TL, NB>(d, tile, x, acc, start, p1, nc);
         if (head) {
             store3(part, acc);
             __threadfence();
@@ -2361,7 +2431,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const Inst* I, const int2* __restrict__ jobs, int n_jobs, unsigned* head) {
     __shared__ PanelRec tile[TL];
     __shared__ double nbuf[NB ? TL * NEARD : 1];
+    __shared__ NearShared nsh;
     __shared__ int slot;
+    NearCtx nc;
+    nc.nbuf = nbuf;
+    nc.sh = &nsh;
     for (;;) {
         int j = next_job(head, &slot);
         if (j >= n_jobs) break;
@@ -2372,7 +2446,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         long t = (long)jb.y * 256 + threadIdx.x;
         long tc = t < end ? t : end - 1;
         V3 x = load3(d.pts + 3 * tc);
-        V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, (long)d.N * d.nz_rows, nbuf);
+        if (NB) block_box(d, x, nc);
+        V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, (long)d.N * d.nz_rows, nc);
         if (t < end) store3(d.vel + 3 * t, load3(d.vel + 3 * t) + acc);
     }
 }
