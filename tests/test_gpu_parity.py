@@ -314,15 +314,23 @@ def test_load_balanced_sum_bitwise_many_targets(oracle):
     assert same(gpu.induced_velocities(tg), orc.induced_velocities(tg))
 
 
-def test_full_size_free_wake_step_sampled_bitwise(oracle):
+@pytest.mark.parametrize("state", ["synthesized", "rolled_up"])
+def test_full_size_free_wake_step_sampled_bitwise(oracle, state):
     """One EXACT step at the benchmark size (328,164 wake points, 320,000 panels, balanced
     schedule): the wake velocities at 3,000 sampled points equal freestream + the oracle's
-    induced velocity (at the GPU's post-step wake) bit for bit."""
+    induced velocity (at the GPU's post-step wake) bit for bit.  States: a convected sheet
+    with seeded noise (0.3 % near-field pairs) and the stored rolled-up wake of a
+    2,000-step run (17 % near-field pairs: near-field records staged for some tiles of a
+    workgroup and skipped for others)."""
     import bench
 
     js, case = S.four_sail_free_wake(nr_panels=2000, nr_strips=40)
     gpu = sb.Simulation(js)
-    pts, vel, gam, prev, fs = bench.synthesize_developed_state(gpu, case)
+    if state == "synthesized":
+        pts, vel, gam, prev, fs = bench.synthesize_developed_state(gpu, case)
+    else:
+        pts, vel, gam, prev, _ = S.developed_four_sail_state(case)
+        fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
     gpu.set_wake_state(pts, vel, gam, 2000)
     gpu.previous_circulation_strength = prev
     dt = case["time_step"]
@@ -334,6 +342,33 @@ def test_full_size_free_wake_step_sampled_bitwise(oracle):
     u = orc.induced_velocities(w["points"][idx])
     expect = fs[gpu.nr_ctrl_points:][idx] + u
     assert same(w["velocities"][idx], expect)
+
+
+def test_full_size_rolled_up_fast_mode_deviation():
+    """FAST against EXACT (the oracle's bits) for one step from the stored rolled-up wake at
+    the benchmark size: 17 % of the self-induction pairs take the near-field ring, which FAST
+    evaluates with fused operations and refined hardware estimates.  Tolerances: the step's
+    results 1e-10 relative (measured 2e-14); wake-point velocities 2e-9 relative to the
+    largest (measured 3.5e-10: targets close to a ring edge, where the viscous-core distance
+    cancels)."""
+    js, case = S.four_sail_free_wake(nr_panels=2000, nr_strips=40)
+    pts, vel, gam, prev, _ = S.developed_four_sail_state(case)
+    dt = case["time_step"]
+    out = {}
+    for mode in (sb.MODE_EXACT, sb.MODE_FAST):
+        sim = sb.Simulation(js, mode=mode)
+        sim.set_wake_state(pts, vel, gam, 2000)
+        sim.previous_circulation_strength = prev
+        fs = S.uniform_freestream(sim.nr_freestream_points(), case["velocity"])
+        r = sim.do_step(time=2000 * dt, time_step=dt, freestream_velocity=fs)
+        out[mode] = (r.as_arrays(), sim.get_wake_state()["velocities"])
+        del sim
+    (re_, we), (rf, wf) = out[sb.MODE_EXACT], out[sb.MODE_FAST]
+    for f in FIELDS:
+        d = rel_dev(rf[f], re_[f])
+        assert d <= 1e-10, f"fast {f}: {d:.3e}"
+    d = rel_dev(wf, we)
+    assert d <= 2e-9, f"fast wake velocities: {d:.3e}"
 
 
 def test_fp32_kahan_mode_deviation(oracle):
