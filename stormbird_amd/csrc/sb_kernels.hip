@@ -356,12 +356,20 @@ __device__ inline F3 f3cross(F3 a, F3 b) {
 }
 
 // vortex_line.rs:18-106 in f32
-__device__ inline F3 segment_unit_f32(V3 l0, V3 l1, V3 x, float rc) {
+// One vortex segment in fp32.  Where fp32 would cancel more than 8 bits (a
+// target nearly on the segment's line: |r1||r2| + r1.r2 or the perpendicular
+// distance |r1|^2 - par^2 below 2^-8 of its terms) the lane evaluates the
+// segment in f64 (segment_unit) instead, so that no edge loses more than 8
+// bits to cancellation; `rcd` is the core length in f64.
+__device__ inline F3 segment_unit_f32(V3 l0, V3 l1, V3 x, float rc, double rcd) {
     F3 r1 = f3_of(x - l0), r2 = f3_of(x - l1);
     float r1sq = f3dot(r1, r1), r2sq = f3dot(r2, r2);
     float r1l = sqrtf(r1sq), r2l = sqrtf(r2sq);
     float r1r2 = r1l * r2l;
-    float den = r1r2 * (r1r2 + f3dot(r1, r2));
+    float sum = r1r2 + f3dot(r1, r2);
+    bool cancel = sum < r1r2 * 0x1p-8f;
+    float den = r1r2 * sum;
+    F3 out{0.0f, 0.0f, 0.0f};
     if (fabsf(den) > (float)CLOSENESS_ERROR) {
         float core = 1.0f;
         if (rc != 0.0f) {
@@ -369,15 +377,20 @@ __device__ inline F3 segment_unit_f32(V3 l0, V3 l1, V3 x, float rc) {
             float ll = sqrtf(f3dot(rel, rel));
             float par = f3dot(r1, rel) / ll;
             float d2 = par < 0.0f ? r1sq : (par > ll ? r2sq : r1sq - par * par);
+            cancel = cancel || (par >= 0.0f && par <= ll && d2 < r1sq * 0x1p-8f);
             float rc2 = rc * rc;
             float cden = sqrtf(rc2 * rc2 + d2 * d2);
             core = cden > 0.0f ? d2 / cden : 0.0f;
         }
         float kk = (r1l + r2l) / den * (core * (float)FOUR_PI_INVERSE);
         F3 c = f3cross(r1, r2);
-        return F3{c.x * kk, c.y * kk, c.z * kk};
+        out = F3{c.x * kk, c.y * kk, c.z * kk};
     }
-    return F3{0.0f, 0.0f, 0.0f};
+    if (cancel) {
+        V3 u = segment_unit(l0, l1, x, rcd);
+        out = F3{(float)u.x, (float)u.y, (float)u.z};
+    }
+    return out;
 }
 
 template <bool NB = false>
@@ -394,9 +407,10 @@ __device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 
         } else {
             panel_corners(d, r, k, c);
         }
-        float rc = (float)d.core[k];
-        F3 a = segment_unit_f32(c[0], c[1], x, rc), b = segment_unit_f32(c[1], c[2], x, rc);
-        F3 e = segment_unit_f32(c[2], c[3], x, rc), f = segment_unit_f32(c[3], c[0], x, rc);
+        const double rcd = d.core[k];
+        const float rc = (float)rcd;
+        F3 a = segment_unit_f32(c[0], c[1], x, rc, rcd), b = segment_unit_f32(c[1], c[2], x, rc, rcd);
+        F3 e = segment_unit_f32(c[2], c[3], x, rc, rcd), f = segment_unit_f32(c[3], c[0], x, rc, rcd);
         return F3{((a.x + b.x) + e.x) + f.x, ((a.y + b.y) + e.y) + f.y, ((a.z + b.z) + e.z) + f.z};
     }
     F3 t{(float)tx, (float)ty, (float)tz};
