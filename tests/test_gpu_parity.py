@@ -345,17 +345,18 @@ def test_full_size_free_wake_step_sampled_bitwise(oracle, state):
 
 
 def test_full_size_rolled_up_fast_mode_deviation():
-    """FAST against EXACT (the oracle's bits) for one step from the stored rolled-up wake at
-    the benchmark size: 17 % of the self-induction pairs take the near-field ring, which FAST
-    evaluates with fused operations and refined hardware estimates.  Tolerances: the step's
-    results 1e-10 relative (measured 2e-14); wake-point velocities 2e-9 relative to the
-    largest (measured 3.5e-10: targets close to a ring edge, where the viscous-core distance
-    cancels)."""
+    """FAST and FP32-Kahan against EXACT (the oracle's bits) for one step from the stored
+    rolled-up wake at the benchmark size: 17 % of the self-induction pairs take the near-field
+    ring, which FAST evaluates with fused operations and refined hardware estimates.
+    Tolerances, FAST: the step's results 1e-10 relative (measured 2e-14); wake-point
+    velocities 2e-9 relative to the largest (measured 3.5e-10: targets close to a ring edge,
+    where the viscous-core distance cancels).  FP32-Kahan: results 1e-6 (measured 7.7e-8),
+    wake-point velocities 1e-4 (measured 1.6e-5)."""
     js, case = S.four_sail_free_wake(nr_panels=2000, nr_strips=40)
     pts, vel, gam, prev, _ = S.developed_four_sail_state(case)
     dt = case["time_step"]
     out = {}
-    for mode in (sb.MODE_EXACT, sb.MODE_FAST):
+    for mode in (sb.MODE_EXACT, sb.MODE_FAST, sb.MODE_FP32_KAHAN):
         sim = sb.Simulation(js, mode=mode)
         sim.set_wake_state(pts, vel, gam, 2000)
         sim.previous_circulation_strength = prev
@@ -369,6 +370,12 @@ def test_full_size_rolled_up_fast_mode_deviation():
         assert d <= 1e-10, f"fast {f}: {d:.3e}"
     d = rel_dev(wf, we)
     assert d <= 2e-9, f"fast wake velocities: {d:.3e}"
+    r32, w32 = out[sb.MODE_FP32_KAHAN]
+    for f in FIELDS:
+        d = rel_dev(r32[f], re_[f])
+        assert d <= 1e-6, f"fp32 {f}: {d:.3e}"
+    d = rel_dev(w32, we)
+    assert d <= 1e-4, f"fp32 wake velocities: {d:.3e}"
 
 
 def test_fp32_kahan_mode_deviation(oracle):
