@@ -356,34 +356,39 @@ __device__ inline F3 f3cross(F3 a, F3 b) {
 }
 
 // vortex_line.rs:18-106 in f32
-// One vortex segment in fp32.  Where fp32 would cancel more than 8 bits (a
-// target nearly on the segment's line: |r1||r2| + r1.r2 or the perpendicular
-// distance |r1|^2 - par^2 below 2^-8 of its terms) the lane evaluates the
-// segment in f64 (segment_unit) instead, so that no edge loses more than 8
-// bits to cancellation; `rcd` is the core length in f64.
+// One vortex segment in fp32 (vortex_line.rs:18-106), in a form that does not
+// cancel near the segment's line: with sg = l1 - l0 and r2 = r1 - sg,
+// r1 x r2 = sg x r1 (no difference of nearly equal products), and
+// |r1||r2| + r1.r2 = |r1 x r2|^2 / (|r1||r2| - r1.r2) where r1.r2 < 0; the
+// distance to the line is |sg x r1| / |sg|.  The relative error of those terms
+// grows as |r1| / h for a target at distance h from the line, so a lane with a
+// target over the segment closer than |r1| / 256 evaluates it in f64
+// (segment_unit); `rcd` is the core length in f64.
 __device__ inline F3 segment_unit_f32(V3 l0, V3 l1, V3 x, float rc, double rcd) {
-    F3 r1 = f3_of(x - l0), r2 = f3_of(x - l1);
+    F3 r1 = f3_of(x - l0), r2 = f3_of(x - l1), sg = f3_of(l1 - l0);
     float r1sq = f3dot(r1, r1), r2sq = f3dot(r2, r2);
     float r1l = sqrtf(r1sq), r2l = sqrtf(r2sq);
     float r1r2 = r1l * r2l;
-    float sum = r1r2 + f3dot(r1, r2);
-    bool cancel = sum < r1r2 * 0x1p-8f;
+    F3 c = f3cross(sg, r1);
+    float c2 = f3dot(c, c);
+    float dt = f3dot(r1, r2);
+    float sum = dt >= 0.0f ? r1r2 + dt : c2 / (r1r2 - dt);
     float den = r1r2 * sum;
+    float sg2 = f3dot(sg, sg);
+    float ll = sqrtf(sg2);
+    float par = f3dot(r1, sg) / ll;
+    bool over = par >= 0.0f && par <= ll;
+    bool cancel = over && c2 < (r1sq * sg2) * 0x1p-16f;
     F3 out{0.0f, 0.0f, 0.0f};
     if (fabsf(den) > (float)CLOSENESS_ERROR) {
         float core = 1.0f;
         if (rc != 0.0f) {
-            F3 rel = f3_of(l1 - l0);
-            float ll = sqrtf(f3dot(rel, rel));
-            float par = f3dot(r1, rel) / ll;
-            float d2 = par < 0.0f ? r1sq : (par > ll ? r2sq : r1sq - par * par);
-            cancel = cancel || (par >= 0.0f && par <= ll && d2 < r1sq * 0x1p-8f);
+            float d2 = par < 0.0f ? r1sq : (par > ll ? r2sq : c2 / sg2);
             float rc2 = rc * rc;
             float cden = sqrtf(rc2 * rc2 + d2 * d2);
             core = cden > 0.0f ? d2 / cden : 0.0f;
         }
         float kk = (r1l + r2l) / den * (core * (float)FOUR_PI_INVERSE);
-        F3 c = f3cross(r1, r2);
         out = F3{c.x * kk, c.y * kk, c.z * kk};
     }
     if (cancel) {

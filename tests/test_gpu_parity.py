@@ -350,8 +350,8 @@ def test_full_size_rolled_up_fast_mode_deviation():
     ring, which FAST evaluates with fused operations and refined hardware estimates.
     Tolerances, FAST: the step's results 1e-10 relative (measured 2e-14); wake-point
     velocities 2e-9 relative to the largest (measured 3.5e-10: targets close to a ring edge,
-    where the viscous-core distance cancels).  FP32-Kahan: results 1e-6 (measured 7.7e-8),
-    wake-point velocities 1e-4 (measured 1.6e-5)."""
+    where the viscous-core distance cancels).  FP32-Kahan: results 1e-6 (measured 9.7e-8),
+    wake-point velocities 1e-4 (measured 5.1e-6)."""
     js, case = S.four_sail_free_wake(nr_panels=2000, nr_strips=40)
     pts, vel, gam, prev, _ = S.developed_four_sail_state(case)
     dt = case["time_step"]
