@@ -108,11 +108,10 @@ __device__ inline V3 ring_edge(V3 r1, V3 r2, double r1sq, double r2sq, double r1
     return V3{0.0, 0.0, 0.0};
 }
 
-// 1/sqrt(a) from the hardware estimate and one Newton step (~1 ulp) for a in
-// [2^-900, 2^1000]; outside that range (never for wake geometry) the IEEE
-// operations, so that a = 0 gives 0 and no estimate meets a subnormal.
-__device__ inline double rsqrt_fast(double a) {
-    if (a < 0x1p-900 || a > 0x1p1000) return a > 0.0 ? 1.0 / sqrt(a) : 0.0;
+// 1/sqrt(a) from the hardware estimate and one Newton step (~1 ulp); valid for
+// positive normal a, which is and-ed into `ok`.
+__device__ inline double rsqrt_fast(double a, bool& ok) {
+    ok = ok & __builtin_amdgcn_class(a, 0x100);  // positive normal
     double y = __builtin_amdgcn_rsq(a);
     return fma(0.5 * y, fma(-a * y, y, 1.0), y);
 }
@@ -184,35 +183,30 @@ __device__ inline bool ring_sq_ok(double sq) { return (sq >= 0x1p-700) & (sq <= 
 
 // ring_edge in FAST arithmetic: fused dot products, square roots and divisions
 // from the hardware reciprocal (square root) with one Newton step (~1 ulp).
+// Straight-line code like ring_edge_unscaled; operands outside the estimates'
+// ranges clear `ok` (for an edge that contributes).
 __device__ inline V3 ring_edge_fast(V3 r1, V3 r2, double r1sq, double r2sq, double r1l, double r2l,
-                                    const double* seg, double rc) {
+                                    const double* seg, double rc, bool& ok) {
     double r1r2 = r1l * r2l;
     double den = r1r2 * (r1r2 + fma(r1.x, r2.x, fma(r1.y, r2.y, r1.z * r2.z)));
-    if (fabs(den) > CLOSENESS_ERROR) {
-        double core = 1.0;
-        if (rc != 0.0) {
-            double par = fma(r1.x, seg[0], fma(r1.y, seg[1], r1.z * seg[2]));
-            double d2 = par < 0.0 ? r1sq : (par > seg[3] ? r2sq : fma(-par, par, r1sq));
-            double rc2 = rc * rc;
-            double a = fma(rc2, rc2, d2 * d2);
-            core = d2 * rsqrt_fast(a);
-        }
-        // den > CLOSENESS_ERROR here; the estimate is valid below 2^1000
-        double v;
-        if (fabs(den) > 0x1p1000) v = 1.0 / den;
-        else {
-            v = __builtin_amdgcn_rcp(den);
-            v = fma(v, fma(-den, v, 1.0), v);
-        }
-        double kk = (r1l + r2l) * v * FOUR_PI_INVERSE * core;
-        return V3{fma(r1.y, r2.z, -r1.z * r2.y) * kk, fma(r1.z, r2.x, -r1.x * r2.z) * kk,
-                  fma(r1.x, r2.y, -r1.y * r2.x) * kk};
+    const bool take = fabs(den) > CLOSENESS_ERROR;
+    double core = 1.0;
+    bool eok = fabs(den) <= 0x1p1000;
+    if (rc != 0.0) {  // uniform
+        double par = fma(r1.x, seg[0], fma(r1.y, seg[1], r1.z * seg[2]));
+        double d2 = par < 0.0 ? r1sq : (par > seg[3] ? r2sq : fma(-par, par, r1sq));
+        double rc2 = rc * rc;
+        double a = fma(rc2, rc2, d2 * d2);
+        core = d2 * rsqrt_fast(a, eok);
     }
-    return V3{0.0, 0.0, 0.0};
+    double v = __builtin_amdgcn_rcp(den);
+    v = fma(v, fma(-den, v, 1.0), v);
+    ok = ok & (!take | eok);
+    double kk = (r1l + r2l) * v * FOUR_PI_INVERSE * core;
+    V3 w{fma(r1.y, r2.z, -r1.z * r2.y) * kk, fma(r1.z, r2.x, -r1.x * r2.z) * kk, fma(r1.x, r2.y, -r1.y * r2.x) * kk};
+    return take ? w : V3{0.0, 0.0, 0.0};
 }
 
-// |r| in FAST arithmetic (0 at r = 0)
-__device__ inline double length_fast(double sq) { return sq * rsqrt_fast(sq); }
 
 // One vortex ring from its corners c0..c3 (ring order) and its ring-edge
 // geometry g (16 doubles: per edge e, from corner e to e+1, the unit direction
@@ -228,20 +222,25 @@ __device__ inline V3 rings_at(const double* c0, const double* c1, const double* 
     const double* cp[4] = {c0, c1, c2, c3};
     V3 u{0.0, 0.0, 0.0};
     if (MODE == MODE_FAST) {
+        // lanes outside the estimates' ranges (a target at a corner) take the
+        // EXACT ring
+        bool ok = true;
         const V3 r0 = x - load3(cp[0]);
-        const double sq0 = fma(r0.x, r0.x, fma(r0.y, r0.y, r0.z * r0.z)), ln0 = length_fast(sq0);
+        const double sq0 = fma(r0.x, r0.x, fma(r0.y, r0.y, r0.z * r0.z)), ln0 = sq0 * rsqrt_fast(sq0, ok);
         V3 ra = r0;
         double sqa = sq0, lna = ln0;
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
             const V3 rb = x - load3(cp[e + 1]);
-            const double sqb = fma(rb.x, rb.x, fma(rb.y, rb.y, rb.z * rb.z)), lnb = length_fast(sqb);
-            u = u + ring_edge_fast(ra, rb, sqa, sqb, lna, lnb, g + 4 * e, rc);
+            const double sqb = fma(rb.x, rb.x, fma(rb.y, rb.y, rb.z * rb.z)), lnb = sqb * rsqrt_fast(sqb, ok);
+            u = u + ring_edge_fast(ra, rb, sqa, sqb, lna, lnb, g + 4 * e, rc, ok);
             ra = rb;
             sqa = sqb;
             lna = lnb;
         }
-        return u + ring_edge_fast(ra, r0, sqa, sq0, lna, ln0, g + 12, rc);
+        u = u + ring_edge_fast(ra, r0, sqa, sq0, lna, ln0, g + 12, rc, ok);
+        if (ok) return u;
+        return rings_at<MODE_EXACT>(c0, c1, c2, c3, g, rc, x);
     }
     {
         bool ok = (rc == 0.0) | ((fabs(rc) >= 0x1p-150) & (fabs(rc) <= 0x1p69));
