@@ -637,7 +637,14 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
         for (int p = 0; p < cnt; ++p) {
             const PanelRec& r = tile[p];
             if (r.strength == 0.0) continue;
-            V3 u = pair_unit<MODE_FP32_KAHAN, SYM, NB>(d, r, (int)(base + p), x, NB ? nc.nbuf + NEARD * p : nullptr);
+            const double* nb = NB ? nc.nbuf + NEARD * p : nullptr;
+            F3 u;
+            if constexpr (SYM == 0) {
+                u = panel_unit_f32<NB>(d, r, (int)(base + p), x, nb);  // the fp32 terms, no f64 round trip
+            } else {
+                V3 w = pair_unit<MODE_FP32_KAHAN, SYM, NB>(d, r, (int)(base + p), x, nb);
+                u = F3{(float)w.x, (float)w.y, (float)w.z};
+            }
             float g = (float)r.strength;
             kahan_add(sx, cx, (float)u.x * g);
             kahan_add(sy, cy, (float)u.y * g);
