@@ -163,6 +163,14 @@ inline V3 in_rotated_coordinate_system(const V3& self, const V3& r, RotationType
     V3 ax = rotate(V3(1, 0, 0), r, t), ay = rotate(V3(0, 1, 0), r, t), az = rotate(V3(0, 0, 1), r, t);
     return {dot(self, ax), dot(self, ay), dot(self, az)};
 }
+// :157-180: rotate the moment, then add the moment of the force about the new origin (arm = -translation)
+inline V3 moment_in_new_coordinate_system(const V3& self, const V3& r, const V3& translation, const V3& force,
+                                          RotationType t) {
+    V3 rotated = in_rotated_coordinate_system(self, r, t);
+    V3 arm = -translation;
+    V3 from_force = cross(arm, force);
+    return rotated + in_rotated_coordinate_system(from_force, r, t);
+}
 
 // ---------------------------------------------------------------------------
 // Rigid body motion (stormath/src/rigid_body_motion.rs:22-103)
@@ -1829,6 +1837,68 @@ double orc_foil_cl(const char* foil_json, double alpha, int which) {
     }
 }
 double orc_sigmoid(double x, double x0, double r) { return sigmoid_zero_to_one(x, x0, r); }
+
+// stormath transformations (spatial_vector/transformations.rs) and rigid-body motion
+// (rigid_body_motion.rs:22-103); rotation_type 0 XYZ, 1 YawPitchRoll.
+static V3 v3(const double* p) { return V3(p[0], p[1], p[2]); }
+static void put3(double* out, const V3& v) {
+    for (int c = 0; c < 3; ++c) out[c] = v[c];
+}
+static RotationType rtype(int t) { return t == 1 ? RotationType::YawPitchRoll : RotationType::XYZ; }
+static RigidBodyMotion motion_of(const double* m12, int t) {
+    RigidBodyMotion m;
+    m.translation = v3(m12);
+    m.rotation = v3(m12 + 3);
+    m.velocity_linear = v3(m12 + 6);
+    m.velocity_angular = v3(m12 + 9);
+    m.rotation_type = rtype(t);
+    return m;
+}
+void orc_rotate(const double* v, const double* rotation, int rotation_type, double* out) {
+    put3(out, rotate(v3(v), v3(rotation), rtype(rotation_type)));
+}
+void orc_rotate_around_axis(const double* v, double angle, const double* axis, double* out) {
+    put3(out, rotate_around_axis(v3(v), angle, v3(axis)));
+}
+void orc_in_rotated_coordinate_system(const double* v, const double* rotation, int rotation_type, double* out) {
+    put3(out, in_rotated_coordinate_system(v3(v), v3(rotation), rtype(rotation_type)));
+}
+void orc_moment_in_new_coordinate_system(const double* moment, const double* rotation, const double* translation,
+                                         const double* force, int rotation_type, double* out) {
+    put3(out, moment_in_new_coordinate_system(v3(moment), v3(rotation), v3(translation), v3(force),
+                                              rtype(rotation_type)));
+}
+void orc_rigid_body_transform_point(const double* motion12, int rotation_type, const double* p, double* out) {
+    put3(out, motion_of(motion12, rotation_type).transform_point(v3(p)));
+}
+void orc_rigid_body_velocity_at_point(const double* motion12, int rotation_type, const double* p, double* out) {
+    put3(out, motion_of(motion12, rotation_type).velocity_at_point(v3(p)));
+}
+void orc_rigid_body_update_fd(double* motion12, const double* translation, const double* rotation,
+                              double time_step) {
+    RigidBodyMotion m = motion_of(motion12, 0);
+    m.update_translation_fd(v3(translation), time_step);
+    m.update_rotation_fd(v3(rotation), time_step);
+    put3(motion12, m.translation);
+    put3(motion12 + 3, m.rotation);
+    put3(motion12 + 6, m.velocity_linear);
+    put3(motion12 + 9, m.velocity_angular);
+}
+// VelocityCorrections::corrected_velocity (velocity_corrections.rs:28-100); kind 1
+// MaxInducedVelocityMagnitudeRatio(ratio), 2 FixedMagnitudeEqualToFreestream, else freestream + induced.
+void orc_corrected_velocity(int kind, double ratio, const double* fs, const double* ui, size_t n, double* out) {
+    std::vector<V3> f(n), u(n);
+    for (size_t i = 0; i < n; ++i) {
+        f[i] = v3(fs + 3 * i);
+        u[i] = v3(ui + 3 * i);
+    }
+    std::vector<V3> r(n);
+    if (kind == 1) r = corrected_velocity(VelCorr::MaxRatio, ratio, f, u);
+    else if (kind == 2) r = corrected_velocity(VelCorr::FixedMagnitude, ratio, f, u);
+    else
+        for (size_t i = 0; i < n; ++i) r[i] = f[i] + u[i];
+    for (size_t i = 0; i < n; ++i) put3(out + 3 * i, r[i]);
+}
 
 int orc_set_threads(int n) {  // OpenMP threads of the target loop (0: the runtime default)
 #ifdef _OPENMP

@@ -106,6 +106,29 @@ void orc_panel_unit_velocity(const double* corners12, double far_ratio,
                              double* out);
 double orc_foil_cl(const char* foil_json, double alpha, int which);
 double orc_sigmoid(double x, double x0, double range);
+/* stormath transformations (spatial_vector/transformations.rs:29-180);
+ * rotation_type 0 XYZ, 1 YawPitchRoll. */
+void orc_rotate(const double* v, const double* rotation, int rotation_type, double* out);
+void orc_rotate_around_axis(const double* v, double angle, const double* axis, double* out);
+void orc_in_rotated_coordinate_system(const double* v, const double* rotation, int rotation_type,
+                                      double* out);
+void orc_moment_in_new_coordinate_system(const double* moment, const double* rotation,
+                                         const double* translation, const double* force,
+                                         int rotation_type, double* out);
+/* RigidBodyMotion (stormath/src/rigid_body_motion.rs:22-103); motion12 =
+ * translation, rotation, velocity_linear, velocity_angular. */
+void orc_rigid_body_transform_point(const double* motion12, int rotation_type, const double* p,
+                                    double* out);
+void orc_rigid_body_velocity_at_point(const double* motion12, int rotation_type, const double* p,
+                                      double* out);
+/* update_translation_/update_rotation_with_velocity_using_finite_difference, in place. */
+void orc_rigid_body_update_fd(double* motion12, const double* translation, const double* rotation,
+                              double time_step);
+/* VelocityCorrections::corrected_velocity (solvers/velocity_corrections.rs:28-100):
+ * kind 1 MaxInducedVelocityMagnitudeRatio(ratio), 2 FixedMagnitudeEqualToFreestream,
+ * 0 none (freestream + induced). fs, ui, out: [n][3]. */
+void orc_corrected_velocity(int kind, double ratio, const double* fs, const double* ui, size_t n,
+                            double* out);
 /* OpenMP threads of the per-target loops (n > 0 sets it); returns the current count. */
 int orc_set_threads(int n);
 /* 0: host libm, 1: portable libm shared with the device code */

@@ -76,6 +76,14 @@ def lib(variant: str = "glibc"):
         L.orc_sim_get_line_model.argtypes = [vp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long), _dp, _dp]
         L.orc_sim_restart.argtypes = [vp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_rotate.argtypes = [_dp, _dp, ctypes.c_int, _dp]
+        L.orc_rotate_around_axis.argtypes = [_dp, ctypes.c_double, _dp, _dp]
+        L.orc_in_rotated_coordinate_system.argtypes = [_dp, _dp, ctypes.c_int, _dp]
+        L.orc_moment_in_new_coordinate_system.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, _dp]
+        L.orc_rigid_body_transform_point.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        L.orc_rigid_body_velocity_at_point.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+        L.orc_rigid_body_update_fd.argtypes = [_dp, _dp, _dp, ctypes.c_double]
+        L.orc_corrected_velocity.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_size_t, _dp]
         L.orc_set_threads.restype = ctypes.c_int
         _libs[variant] = L
     return _libs[variant]
@@ -258,3 +266,69 @@ def horseshoe_unit_velocity(start, end, wake_start, wake_end, x, core=0.0):
 def set_threads(n: int, variant: str = "glibc") -> int:
     """OpenMP threads of the oracle's per-target loops (n > 0 sets it); returns the count."""
     return lib(variant).orc_set_threads(int(n))
+
+
+# ---- stormath transformations and rigid-body motion (rotation_type 0 XYZ, 1 YawPitchRoll) ----
+_ROT = {"XYZ": 0, "YawPitchRoll": 1}
+
+
+def _vec_call(fn, *args):
+    out = np.zeros(3)
+    fn(*args, _p(out))
+    return out
+
+
+def rotate(v, rotation, rotation_type="XYZ"):
+    """SpatialVector::rotate (transformations.rs:43-48)."""
+    return _vec_call(lib().orc_rotate, _p(_f64(v)), _p(_f64(rotation)), _ROT[rotation_type])
+
+
+def rotate_around_axis(v, angle, axis):
+    """SpatialVector::rotate_around_axis, Rodrigues' formula (transformations.rs:29-41)."""
+    return _vec_call(lib().orc_rotate_around_axis, _p(_f64(v)), float(angle), _p(_f64(axis)))
+
+
+def in_rotated_coordinate_system(v, rotation, rotation_type="XYZ"):
+    """SpatialVector::in_rotated_coordinate_system (transformations.rs:145-155)."""
+    return _vec_call(lib().orc_in_rotated_coordinate_system, _p(_f64(v)), _p(_f64(rotation)),
+                     _ROT[rotation_type])
+
+
+def moment_in_new_coordinate_system(moment, rotation, translation, force, rotation_type="XYZ"):
+    """SpatialVector::moment_in_new_coordinate_system (transformations.rs:157-180)."""
+    return _vec_call(lib().orc_moment_in_new_coordinate_system, _p(_f64(moment)), _p(_f64(rotation)),
+                     _p(_f64(translation)), _p(_f64(force)), _ROT[rotation_type])
+
+
+def _motion12(translation, rotation, velocity_linear, velocity_angular):
+    return np.concatenate([_f64(translation), _f64(rotation), _f64(velocity_linear), _f64(velocity_angular)])
+
+
+def rigid_body_transform_point(motion, point, rotation_type="XYZ"):
+    """RigidBodyMotion::transform_point (rigid_body_motion.rs:32-34); motion = (t, r, v, w)."""
+    m = _motion12(*motion)
+    return _vec_call(lib().orc_rigid_body_transform_point, _p(m), _ROT[rotation_type], _p(_f64(point)))
+
+
+def rigid_body_velocity_at_point(motion, point, rotation_type="XYZ"):
+    """RigidBodyMotion::velocity_at_point (rigid_body_motion.rs:57-65); motion = (t, r, v, w)."""
+    m = _motion12(*motion)
+    return _vec_call(lib().orc_rigid_body_velocity_at_point, _p(m), _ROT[rotation_type], _p(_f64(point)))
+
+
+def rigid_body_update_fd(motion, translation, rotation, time_step):
+    """update_{translation,rotation}_with_velocity_using_finite_difference (rigid_body_motion.rs:67-103).
+    Returns the new (translation, rotation, velocity_linear, velocity_angular)."""
+    m = _motion12(*motion)
+    lib().orc_rigid_body_update_fd(_p(m), _p(_f64(translation)), _p(_f64(rotation)), float(time_step))
+    return m[0:3].copy(), m[3:6].copy(), m[6:9].copy(), m[9:12].copy()
+
+
+def corrected_velocity(kind, freestream, induced, ratio=0.0):
+    """VelocityCorrections::corrected_velocity (velocity_corrections.rs:28-100); kind
+    "MaxInducedVelocityMagnitudeRatio", "FixedMagnitudeEqualToFreestream" or "NoCorrection"."""
+    k = {"NoCorrection": 0, "MaxInducedVelocityMagnitudeRatio": 1, "FixedMagnitudeEqualToFreestream": 2}[kind]
+    fs, ui = _f64(freestream, (-1, 3)), _f64(induced, (-1, 3))
+    out = np.zeros_like(fs)
+    lib().orc_corrected_velocity(k, float(ratio), _p(fs), _p(ui), fs.shape[0], _p(out))
+    return out
