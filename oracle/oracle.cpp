@@ -1884,6 +1884,16 @@ void orc_rigid_body_update_fd(double* motion12, const double* translation, const
     put3(motion12 + 6, m.velocity_linear);
     put3(motion12 + 9, m.velocity_angular);
 }
+// FrozenWake::update_induced_velocities_at_control_points (frozen_wake.rs:268-301) on given arrays.
+void orc_frozen_matvec(const double* fixed, const double* factors, const double* strengths, size_t n,
+                       double* out) {
+    FrozenWake fw;
+    fw.init(n);
+    for (size_t r = 0; r < n; ++r) fw.fixed[r] = v3(fixed + 3 * r);
+    for (size_t k = 0; k < n * n; ++k) fw.factors[k] = v3(factors + 3 * k);
+    fw.matvec(std::vector<double>(strengths, strengths + n));
+    for (size_t r = 0; r < n; ++r) put3(out + 3 * r, fw.induced[r]);
+}
 // VelocityCorrections::corrected_velocity (velocity_corrections.rs:28-100); kind 1
 // MaxInducedVelocityMagnitudeRatio(ratio), 2 FixedMagnitudeEqualToFreestream, else freestream + induced.
 void orc_corrected_velocity(int kind, double ratio, const double* fs, const double* ui, size_t n, double* out) {

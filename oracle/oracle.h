@@ -129,6 +129,10 @@ void orc_rigid_body_update_fd(double* motion12, const double* translation, const
  * 0 none (freestream + induced). fs, ui, out: [n][3]. */
 void orc_corrected_velocity(int kind, double ratio, const double* fs, const double* ui, size_t n,
                             double* out);
+/* FrozenWake::update_induced_velocities_at_control_points (frozen_wake.rs:268-301):
+ * fixed [3n], factors [n][n][3] row-major, strengths [n], out [3n]. */
+void orc_frozen_matvec(const double* fixed, const double* factors, const double* strengths, size_t n,
+                       double* out);
 /* OpenMP threads of the per-target loops (n > 0 sets it); returns the current count. */
 int orc_set_threads(int n);
 /* 0: host libm, 1: portable libm shared with the device code */

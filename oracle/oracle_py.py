@@ -83,6 +83,7 @@ def lib(variant: str = "glibc"):
         L.orc_rigid_body_transform_point.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.orc_rigid_body_velocity_at_point.argtypes = [_dp, ctypes.c_int, _dp, _dp]
         L.orc_rigid_body_update_fd.argtypes = [_dp, _dp, _dp, ctypes.c_double]
+        L.orc_frozen_matvec.argtypes = [_dp, _dp, _dp, ctypes.c_size_t, _dp]
         L.orc_corrected_velocity.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_size_t, _dp]
         L.orc_set_threads.restype = ctypes.c_int
         _libs[variant] = L
@@ -331,4 +332,12 @@ def corrected_velocity(kind, freestream, induced, ratio=0.0):
     fs, ui = _f64(freestream, (-1, 3)), _f64(induced, (-1, 3))
     out = np.zeros_like(fs)
     lib().orc_corrected_velocity(k, float(ratio), _p(fs), _p(ui), fs.shape[0], _p(out))
+    return out
+
+
+def frozen_matvec(fixed, factors, strengths, variant: str = "glibc"):
+    """FrozenWake::update_induced_velocities_at_control_points (frozen_wake.rs:268-301)."""
+    g = _f64(strengths)
+    out = np.zeros((g.size, 3))
+    lib(variant).orc_frozen_matvec(_p(_f64(fixed)), _p(_f64(factors)), _p(g), g.size, _p(out))
     return out

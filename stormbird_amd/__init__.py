@@ -66,6 +66,8 @@ def lib():
         L.sb_sim_get_wake_state.argtypes = [vp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_long),
                                             ctypes.POINTER(ctypes.c_int)]
         L.sb_sim_set_wake_state.argtypes = [vp, _dp, _dp, _dp, ctypes.c_long, ctypes.c_int]
+        L.sb_sim_set_wake_state_ex.argtypes = [vp, _dp, ctypes.c_size_t, _dp, ctypes.c_size_t, _dp,
+                                               ctypes.c_size_t, ctypes.c_long, ctypes.c_int]
         L.sb_sim_get_panels.argtypes = [vp, _dp]
         L.sb_sim_get_frozen_wake.argtypes = [vp, _dp, _dp]
         L.sb_sim_last_kernel_ms.restype = ctypes.c_double
@@ -75,6 +77,7 @@ def lib():
         L.sb_wake_induced_velocities.argtypes = [_dp, ctypes.c_size_t, _dp, ctypes.c_size_t, ctypes.c_int,
                                                  ctypes.c_int, _dp]
         L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
+        L.sb_frozen_matvec.argtypes = [_dp, _dp, _dp, ctypes.c_size_t, _dp]
         L.sb_debug_linearized_in_shared_memory.argtypes = [ctypes.c_long]
         L.sb_setup_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), ctypes.c_size_t]
         L.sb_debug_libm_host.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
@@ -238,6 +241,8 @@ class Simulation:
     def set_translation_and_rotation_with_finite_difference_for_the_velocity(self, time_step, translation,
                                                                              rotation):
         t, r = _f64(translation), _f64(rotation)
+        if t.shape != (3,) or r.shape != (3,):
+            raise ValueError(f"translation and rotation must be 3-vectors, got shapes {t.shape} and {r.shape}")
         rc = lib().sb_sim_set_translation_and_rotation_fd(self._h, time_step, _p(t), _p(r))
         if rc != 0:
             _err("set_translation_and_rotation", rc)
@@ -261,9 +266,19 @@ class Simulation:
                 "first_step_done": bool(first.value)}
 
     def set_wake_state(self, points, velocities, strengths, steps_completed, first_step_done=True):
-        pts, vel, st = _f64(points), _f64(velocities), _f64(strengths)
-        rc = lib().sb_sim_set_wake_state(self._h, _p(pts), _p(vel), _p(st), int(steps_completed),
-                                         1 if first_step_done else 0)
+        """points, velocities: nr_wake_points x 3 values; strengths: nr_panels values (any shape
+        holding exactly that many; ValueError otherwise)."""
+        arrays = []
+        for name, a, shape in (("points", points, (self.nr_wake_points, 3)),
+                               ("velocities", velocities, (self.nr_wake_points, 3)),
+                               ("strengths", strengths, (self.nr_panels,))):
+            a = _f64(a)
+            if a.size != int(np.prod(shape)):
+                raise ValueError(f"set_wake_state: {name} has shape {a.shape}, the simulation needs {shape}")
+            arrays.append(a.reshape(shape))
+        pts, vel, st = arrays
+        rc = lib().sb_sim_set_wake_state_ex(self._h, _p(pts), pts.shape[0], _p(vel), vel.shape[0], _p(st), st.size,
+                                            int(steps_completed), 1 if first_step_done else 0)
         if rc != 0:
             _err("set_wake_state", rc)
 
@@ -367,6 +382,22 @@ def setup_info(setup_string: str) -> dict:
     keys = ("nr_ctrl_points", "nr_wings", "nr_span_points", "nr_wake_rows", "nr_freestream_points",
             "nr_panels", "quasi_steady")
     return dict(zip(keys, list(out)))
+
+
+def frozen_matvec(fixed, factors, strengths) -> np.ndarray:
+    """FrozenWake::update_induced_velocities_at_control_points (frozen_wake.rs:268-301) on the GPU:
+    fixed [n][3] + factors [n][n][3] (row-major) times strengths [n]."""
+    g = _f64(strengths)
+    n = g.size
+    fx = _f64(fixed)
+    fa = _f64(factors)
+    if fx.size != 3 * n or fa.size != 3 * n * n:
+        raise ValueError(f"frozen_matvec: fixed {fx.shape} and factors {fa.shape} do not fit {n} strengths")
+    out = np.zeros((n, 3))
+    rc = lib().sb_frozen_matvec(_p(fx), _p(fa), _p(g), n, _p(out))
+    if rc != 0:
+        _err("frozen_matvec", rc)
+    return out
 
 
 def libm_device(fn: int, x, y=None) -> np.ndarray:

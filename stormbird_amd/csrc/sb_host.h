@@ -61,6 +61,12 @@ void write_wake_step_file(const sb_sim& s, double time, double dt);
 using namespace sb;
 using namespace sb_host;
 
+// Limits of this library on a setup the reference accepts (applied at create and by
+// sb_setup_info, so that a setup the check accepts also builds).
+inline void check_limits(const Setup& s) {
+    if (s.lfm.n() > 1024) throw std::runtime_error("more than 1024 strips per simulation is not supported");
+}
+
 struct sb_sim {
     Setup setup;
     int device = 0;
@@ -265,7 +271,7 @@ struct sb_sim {
         W = m.nw();
         P = (int)m.span_points.size();
         M = setup.wake.M;
-        if (N > 1024) throw std::runtime_error("more than 1024 strips per simulation is not supported");
+        check_limits(setup);
         nr_points = setup.quasi ? 0 : (long)P * (M + 1);  // quasi-steady: no wake points
         nr_panels = (long)N * M;
         layout();
@@ -574,6 +580,7 @@ namespace sb_host {
 // Sizes of a parsed setup: strips, wings, span points, wake rows, freestream
 // points, wake panels, 1 for quasi-steady.
 inline void setup_counts(const Setup& s, long* out) {
+    check_limits(s);
     const long N = s.lfm.n(), W = s.lfm.nw(), P = (long)s.lfm.span_points.size(), M = s.wake.M;
     const long points = s.quasi ? 0 : P * (M + 1);
     out[0] = N;

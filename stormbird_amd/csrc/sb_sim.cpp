@@ -164,6 +164,19 @@ int sb_sim_get_wake_state(const sb_sim* s, double* pts, double* vel, double* str
     return rc;
 }
 
+int sb_sim_set_wake_state_ex(sb_sim* s, const double* pts, size_t n_pts, const double* vel, size_t n_vel,
+                             const double* str, size_t n_str, long steps, int first) {
+    if (!s) return -1;
+    if ((pts && n_pts != (size_t)s->nr_points) || (vel && n_vel != (size_t)s->nr_points) ||
+        (str && n_str != (size_t)s->nr_panels)) {
+        g_error = "wake state sizes (" + std::to_string(n_pts) + ", " + std::to_string(n_vel) + ", " +
+                  std::to_string(n_str) + ") do not match the simulation's (" + std::to_string(s->nr_points) +
+                  " points, " + std::to_string(s->nr_panels) + " panels)";
+        return -3;
+    }
+    return sb_sim_set_wake_state(s, pts, vel, str, steps, first);
+}
+
 int sb_sim_set_wake_state(sb_sim* s, const double* pts, const double* vel, const double* str, long steps,
                           int first) {
     if (!s) return -1;

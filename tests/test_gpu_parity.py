@@ -396,3 +396,43 @@ def test_fp32_kahan_mode_deviation(oracle):
     print(f"fp32-kahan worst relative deviation over 25 steps: {worst:.3e}")
     assert worst <= 1e-5
     assert worst > 0.0  # it is a different arithmetic
+
+
+@pytest.mark.parametrize("n", [1, 7, 160, 384, 1024])
+def test_frozen_matvec_bitwise(oracle, n):
+    """sb_frozen_matvec against the oracle's FrozenWake matvec (frozen_wake.rs:268-301), on
+    random data and on the frozen wake of a stepped simulation."""
+    rng = np.random.default_rng(n)
+    fixed = rng.normal(size=(n, 3))
+    fac = rng.normal(size=(n, n, 3)) * 10.0 ** rng.uniform(-6, 2, size=(n, n, 1))
+    g = rng.normal(size=n) * 5.0
+    assert same(sb.frozen_matvec(fixed, fac, g), oracle.frozen_matvec(fixed, fac, g))
+    if n == 160:
+        js, case = S.four_sail_free_wake(nr_panels=20, nr_strips=40)
+        orc = oracle.Simulation(js, "pm")
+        fs = S.uniform_freestream(orc.nr_freestream_points(), case["velocity"])
+        for k in range(4):
+            orc.do_step(time=k * case["time_step"], time_step=case["time_step"], freestream_velocity=fs)
+        fixed, fac = orc.get_frozen_wake()
+        g = orc.get_previous_circulation()
+        got = sb.frozen_matvec(fixed, fac, g)
+        assert same(got, oracle.frozen_matvec(fixed, fac, g, "pm"))
+        assert np.max(np.abs(got - fixed - np.einsum("rcx,c->rx", fac, g))) < 1e-9 * np.max(np.abs(got))
+
+
+def test_set_wake_state_checks_sizes():
+    """sb_sim_set_wake_state_ex refuses arrays whose length is not the simulation's (-3)."""
+    js, case = S.four_sail_free_wake(nr_panels=10, nr_strips=4)
+    sim = sb.Simulation(js)
+    st = sim.get_wake_state()
+    with pytest.raises(ValueError):
+        sim.set_wake_state(st["points"][:-1], st["velocities"], st["strengths"], 3)
+    with pytest.raises(ValueError):
+        sim.set_wake_state(st["points"], st["velocities"], st["strengths"][:-2], 3)
+    L = sb.lib()
+    pts = np.ascontiguousarray(st["points"])
+    rc = L.sb_sim_set_wake_state_ex(sim._h, pts.ctypes.data_as(sb._dp), pts.shape[0] - 1, None, 0, None, 0, 3, 1)
+    assert rc == -3
+    sim.set_wake_state(st["points"], st["velocities"], st["strengths"], 3)
+    st2 = sim.get_wake_state()
+    assert same(st2["points"], st["points"]) and st2["steps_completed"] == 3

@@ -194,3 +194,13 @@ def test_stored_developed_state_loads():
     u = np.asarray(case["velocity"])
     downstream = (pts - pts[:164].mean(axis=0)) @ (u / np.linalg.norm(u))
     assert downstream.max() > 4000.0  # 2,000 steps x 0.208 s x 12 m/s = 5 km of wake
+
+
+def test_setup_check_applies_the_create_time_limits():
+    """sb_setup_info rejects what sb_sim_create would reject: more than 1024 strips."""
+    import stormbird_amd.setups as S
+
+    ok = sb.setup_info(S.rectangular_wing(nr_strips=1024, nr_panels=2))
+    assert ok["nr_ctrl_points"] == 1024
+    with pytest.raises(ValueError, match="1024"):
+        sb.setup_info(S.rectangular_wing(nr_strips=1025, nr_panels=2))
