@@ -196,6 +196,25 @@ int sb_wake_induced_velocities(const double* panel_records, size_t n_panels,
                                const double* targets_xyz, size_t n_targets, int symmetry,
                                int mode, double* out_xyz);
 
+/* Allocation-free form of sb_wake_induced_velocities.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The
+ * records [n_panels][22], targets [3 n_targets] and output [3 n_targets] are in
+ * device memory on the calling thread's current device; the work is queued on
+ * `stream` (a hipStream_t; NULL: the null stream) and not waited for.  The
+ * panel records are staged in a fixed 32 MiB device array that the library
+ * holds on each device it runs on until the process ends; no memory is
+ * allocated or freed.  Limit: n_panels <= sb_noalloc_max_panels() (-4 with a
+ * message beyond it).  Every call on a device reuses that device's array, so
+ * calls that overlap on one GPU (other streams or threads) race.  Results
+ * equal sb_wake_induced_velocities's bit for bit.  mode: SB_MODE_EXACT or
+ * SB_MODE_FAST. */
+int sb_wake_induced_velocities_noalloc(const double* panel_records_device, size_t n_panels,
+                                       const double* targets_xyz_device, size_t n_targets, int symmetry,
+                                       int mode, double* out_xyz_device, void* stream);
+/* The largest n_panels sb_wake_induced_velocities_noalloc takes. */
+long sb_noalloc_max_panels(void);
+
 /* Frozen-wake matrix-vector product (kernel-level entry for parity tests):
  * out[r] = fixed[r] + sum_c factors[r][c] * strengths[c], the x, y and z sums
  * each taken over the columns in ascending order, as

@@ -78,6 +78,10 @@ def lib():
                                                  ctypes.c_int, _dp]
         L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
         L.sb_frozen_matvec.argtypes = [_dp, _dp, _dp, ctypes.c_size_t, _dp]
+        L.sb_wake_induced_velocities_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
+                                                         ctypes.c_size_t, ctypes.c_int, ctypes.c_int,
+                                                         ctypes.c_void_p, ctypes.c_void_p]
+        L.sb_noalloc_max_panels.restype = ctypes.c_long
         L.sb_debug_linearized_in_shared_memory.argtypes = [ctypes.c_long]
         L.sb_setup_info.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_long), ctypes.c_size_t]
         L.sb_debug_libm_host.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]

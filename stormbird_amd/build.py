@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libstormbird_amd.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["sb_kernels.hip", "sb_mlp.hip", "sb_standalone.hip", "sb_sim.cpp", "sb_batch.cpp", "sb_export.cpp", "sb_model.cpp", "sb_csm.cpp"]
+SOURCES = ["sb_kernels.hip", "sb_mlp.hip", "sb_standalone.hip", "sb_noalloc.hip", "sb_sim.cpp", "sb_batch.cpp", "sb_export.cpp", "sb_model.cpp", "sb_csm.cpp"]
 ARCH = os.environ.get("SB_OFFLOAD_ARCH", "gfx950")
 
 COMMON = [
