@@ -152,6 +152,13 @@ int sb_sim_set_local_wing_angles(sb_sim* sim, const double* angles, size_t n);
 /* LineForceModel::local_wing_angles (line_force_model/mod.rs:92), n = number of wings. */
 int sb_sim_get_local_wing_angles(const sb_sim* sim, double* out, size_t n);
 
+/* LineForceModel::wing_indices (line_force_model/mod.rs:69) as [start, end) pairs
+ * [2 * nr_wings]; -3 when cap < 2 * nr_wings. */
+int sb_sim_get_wing_ranges(const sb_sim* sim, long* ranges, size_t cap);
+/* LineForceModel::rigid_body_motion (line_force_model/mod.rs:88; stormath/src/rigid_body_motion.rs:22-29):
+ * translation, rotation, velocity_linear, velocity_angular [12]. */
+int sb_sim_get_rigid_body_motion(const sb_sim* sim, double* motion12);
+
 /* DynamicWake public fields points / velocity_at_points / strengths /
  * number_of_time_steps_completed (dynamic_wake/mod.rs:41-64) and
  * Simulation::first_time_step_completed (simulation.rs:27). */
@@ -181,6 +188,14 @@ double sb_sim_last_kernel_ms(const sb_sim* sim, const char* name);
 /* Pair evaluations with nonzero panel strength in the last step:
  * [0] self-induction, [1] fixed velocities, [2] unit-strength factor pairs. */
 int sb_sim_last_pair_counts(const sb_sim* sim, double* counts3);
+
+/* Order in which the free-wake self-induction visits its targets (no counterpart
+ * in the reference; results do not depend on it, since every target keeps its
+ * ascending panel order): waves of `span_points_per_wave` consecutive span points
+ * of 64 / span_points_per_wave consecutive wake rows (a divisor of 64), or 0 for
+ * the flat point order (the default, and the fastest measured). */
+int sb_sim_set_target_tiling(sb_sim* sim, int span_points_per_wave);
+int sb_sim_get_target_tiling(const sb_sim* sim);
 
 /* Near-field (vortex-ring branch) and total pair counts, with nonzero panel
  * strength, of the free-wake self-induction sum at the current wake state

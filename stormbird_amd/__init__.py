@@ -74,6 +74,8 @@ def lib():
         L.sb_sim_last_kernel_ms.argtypes = [vp, ctypes.c_char_p]
         L.sb_sim_last_pair_counts.argtypes = [vp, _dp]
         L.sb_sim_count_self_induction_pairs.argtypes = [vp, _dp]
+        L.sb_sim_set_target_tiling.argtypes = [vp, ctypes.c_int]
+        L.sb_sim_get_target_tiling.argtypes = [vp]
         L.sb_wake_induced_velocities.argtypes = [_dp, ctypes.c_size_t, _dp, ctypes.c_size_t, ctypes.c_int,
                                                  ctypes.c_int, _dp]
         L.sb_debug_libm_device.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_size_t]
@@ -285,6 +287,17 @@ class Simulation:
                                             int(steps_completed), 1 if first_step_done else 0)
         if rc != 0:
             _err("set_wake_state", rc)
+
+    @property
+    def target_tiling(self) -> int:
+        """Span points per wave of the self-induction's target tiles (0: flat order)."""
+        return lib().sb_sim_get_target_tiling(self._h)
+
+    @target_tiling.setter
+    def target_tiling(self, span_points_per_wave: int):
+        rc = lib().sb_sim_set_target_tiling(self._h, int(span_points_per_wave))
+        if rc != 0:
+            _err("set_target_tiling", rc)
 
     def get_panels(self) -> np.ndarray:
         out = np.zeros((self.nr_panels, 22))

@@ -119,6 +119,13 @@ struct McBuffers {
     long cap_jobs = 0;
     long long epoch = 0;          // incremented per launch; flags compare against it
     int slots[3][2][2] = {};  // resident workgroups per (mode, symmetry, near-field staging)
+    // Order of the self-induction targets: slot j * 256 + l of job j holds wake point
+    // tmap[slot], or -(i + 1) for a padding slot that evaluates wake point i and stores
+    // nothing.  Null: flat order (slot = wake point).  Each target's own panel order is
+    // the same in either order, so the results are too.
+    int* tmap = nullptr;
+    long tmap_jobs = 0;
+    int tmap_span = 0;  // span points per wave of the map (0: flat order)
 };
 
 // Scratch of the two-pass few-target sum (k_terms + k_fold_terms): one chunk
@@ -155,7 +162,7 @@ hipError_t launch_velocity_copy(const Inst& d, hipStream_t st);
 hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStream_t st);
 // Induced velocity at `targets` from panels [0, n_panels): out (+)= sum.
 hipError_t launch_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode, double* out,
-                          int add, McBuffers* mc, hipStream_t st);
+                          int add, McBuffers* mc, hipStream_t st, const int* tmap = nullptr, long map_jobs = 0);
 hipError_t launch_forces(const Inst& d, double dt, hipStream_t st);
 hipError_t launch_flow_update(const Inst& d, hipStream_t st);
 // Induced velocity at arbitrary targets from panels [0, n_panels): out = sum.

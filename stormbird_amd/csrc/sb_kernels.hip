@@ -283,8 +283,9 @@ __device__ inline V3 rings_at(const double* c0, const double* c1, const double* 
 // near-field record (corners and edge geometry, NEARD doubles) comes from
 // Inst::segs, or with NB from its copy `nb` in the shared tile.
 template <int MODE = MODE_EXACT, bool NB = false>
-__device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
-    double rc = d.core[k];
+__device__ inline V3 panel_rings(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr,
+                                 const double* nbc = nullptr) {
+    double rc = NB ? *nbc : d.core[k];
     if (NB || (d.segs && !d.corners)) {
         const double* q = NB ? nb : d.segs + (long)NEARD * k;
         return rings_at<MODE>(q, q + 3, q + 6, q + 9, q + 12, rc, x);
@@ -398,7 +399,8 @@ __device__ inline F3 segment_unit_f32(V3 l0, V3 l1, V3 x, float rc, double rcd) 
 }
 
 template <bool NB = false>
-__device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
+__device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr,
+                                    const double* nbc = nullptr) {
     double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
     double d2 = tx * tx + ty * ty + tz * tz;
     if (d2 <= r.far2) {
@@ -411,7 +413,7 @@ __device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 
         } else {
             panel_corners(d, r, k, c);
         }
-        const double rcd = d.core[k];
+        const double rcd = NB ? *nbc : d.core[k];
         const float rc = (float)rcd;
         F3 a = segment_unit_f32(c[0], c[1], x, rc, rcd), b = segment_unit_f32(c[1], c[2], x, rc, rcd);
         F3 e = segment_unit_f32(c[2], c[3], x, rc, rcd), f = segment_unit_f32(c[3], c[0], x, rc, rcd);
@@ -427,16 +429,17 @@ __device__ inline F3 panel_unit_f32(const Inst& d, const PanelRec& r, int k, V3 
 }
 
 template <int MODE, bool NB = false>
-__device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
+__device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr,
+                                const double* nbc = nullptr) {
     if (MODE == MODE_FP32_KAHAN) {
-        F3 u = panel_unit_f32<NB>(d, r, k, x, nb);
+        F3 u = panel_unit_f32<NB>(d, r, k, x, nb, nbc);
         return V3{(double)u.x, (double)u.y, (double)u.z};
     }
     // panel.rs:53-76: near/far switch on |x - c|^2 <= far^2
     double tx = x.x - r.cx, ty = x.y - r.cy, tz = x.z - r.cz;
     if (MODE == MODE_EXACT) {
         double d2 = tx * tx + ty * ty + tz * tz;
-        if (d2 <= r.far2) return panel_rings<MODE_EXACT, NB>(d, r, k, x, nb);
+        if (d2 <= r.far2) return panel_rings<MODE_EXACT, NB>(d, r, k, x, nb, nbc);
         // panel.rs:82-96 point doublet
         double nh = tx * r.nx + ty * r.ny + tz * r.nz;
         double h3 = 3.0 * nh;
@@ -445,7 +448,7 @@ __device__ inline V3 panel_unit(const Inst& d, const PanelRec& r, int k, V3 x, c
         return doublet_quotients(d2, num);
     } else {
         double d2 = fma(tx, tx, fma(ty, ty, tz * tz));
-        if (d2 <= r.far2) return panel_rings<MODE_FAST, NB>(d, r, k, x, nb);
+        if (d2 <= r.far2) return panel_rings<MODE_FAST, NB>(d, r, k, x, nb, nbc);
         double y = __builtin_amdgcn_rsq(d2);
         // one Newton step on 1/sqrt(d2): y <- y (3 - d2 y^2) / 2
         double e = fma(-d2 * y, y, 1.0);
@@ -518,14 +521,15 @@ __device__ inline bool far_fast(const PanelRec& r, V3 x, V3& out) {
 // go through one evaluation site: with two separately inlined near-field
 // evaluations the gfx950 compiler returned the first result for both.
 template <int MODE, bool NB = false>
-__device__ inline V3 unit_velocity_sym(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
+__device__ inline V3 unit_velocity_sym(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr,
+                                       const double* nbc = nullptr) {
     int nq = d.symmetry == 0 ? 1 : 2;
     V3 s{d.symmetry == 1 ? -1.0 : 1.0, d.symmetry == 2 ? -1.0 : 1.0, d.symmetry == 3 ? -1.0 : 1.0};
     V3 out{0.0, 0.0, 0.0};
 #pragma nounroll
     for (int q = 0; q < nq; ++q) {
         V3 p = q == 0 ? x : V3{x.x * s.x, x.y * s.y, x.z * s.z};
-        V3 v = panel_unit<MODE, NB>(d, r, k, p, nb);
+        V3 v = panel_unit<MODE, NB>(d, r, k, p, nb, nbc);
         out = q == 0 ? v : V3{out.x + s.x * v.x, out.y + s.y * v.y, out.z + s.z * v.z};
     }
     return out;
@@ -538,9 +542,10 @@ constexpr int TILE = 256;  // panel records per LDS tile (20 KiB)
 constexpr int RECD = (int)(sizeof(PanelRec) / sizeof(double));
 
 template <int MODE, int SYM, bool NB = false>
-__device__ inline V3 pair_unit(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr) {
-    if (SYM) return unit_velocity_sym<MODE, NB>(d, r, k, x, nb);
-    return panel_unit<MODE, NB>(d, r, k, x, nb);
+__device__ inline V3 pair_unit(const Inst& d, const PanelRec& r, int k, V3 x, const double* nb = nullptr,
+                               const double* nbc = nullptr) {
+    if (SYM) return unit_velocity_sym<MODE, NB>(d, r, k, x, nb, nbc);
+    return panel_unit<MODE, NB>(d, r, k, x, nb, nbc);
 }
 
 // Near-field staging (NB): the staged records, the block's bounding box of
@@ -555,6 +560,7 @@ struct NearShared {
 };
 struct NearCtx {
     double* nbuf = nullptr;       // [TL][NEARD] shared
+    double* ncore = nullptr;      // [TL] shared: the panels' viscous core lengths
     NearShared* sh = nullptr;     // shared
 };
 
@@ -605,6 +611,7 @@ __device__ inline void stage_near(const Inst& d, NearCtx& nc, long base, int cnt
     if (need) {
         const double* nsrc = d.segs + (long)NEARD * base;
         for (int q = threadIdx.x; q < cnt * NEARD; q += blockDim.x) nc.nbuf[q] = nsrc[q];
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) nc.ncore[q] = d.core[base + q];
         __syncthreads();
     }
 }
@@ -638,11 +645,12 @@ __device__ inline V3 sweep_kahan(const Inst& d, PanelRec* tile, V3 x, V3 acc, lo
             const PanelRec& r = tile[p];
             if (r.strength == 0.0) continue;
             const double* nb = NB ? nc.nbuf + NEARD * p : nullptr;
+            const double* nbc = NB ? nc.ncore + p : nullptr;
             F3 u;
             if constexpr (SYM == 0) {
-                u = panel_unit_f32<NB>(d, r, (int)(base + p), x, nb);  // the fp32 terms, no f64 round trip
+                u = panel_unit_f32<NB>(d, r, (int)(base + p), x, nb, nbc);  // the fp32 terms, no f64 round trip
             } else {
-                V3 w = pair_unit<MODE_FP32_KAHAN, SYM, NB>(d, r, (int)(base + p), x, nb);
+                V3 w = pair_unit<MODE_FP32_KAHAN, SYM, NB>(d, r, (int)(base + p), x, nb, nbc);
                 u = F3{(float)w.x, (float)w.y, (float)w.z};
             }
             float g = (float)r.strength;
@@ -674,14 +682,27 @@ __device__ inline V3 sweep(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0,
         for (int p = 0; p < cnt; p += 2) {
             PanelRec b = tile[p + 1 < cnt ? p + 1 : p];
             if (a.strength != 0.0)
-                acc = acc + pair_unit<MODE, SYM, NB>(d, a, (int)(base + p), x, NB ? nc.nbuf + NEARD * p : nullptr) * a.strength;
+                acc = acc + pair_unit<MODE, SYM, NB>(d, a, (int)(base + p), x, NB ? nc.nbuf + NEARD * p : nullptr,
+                                                     NB ? nc.ncore + p : nullptr) *
+                                a.strength;
             if (p + 1 >= cnt) break;
             a = tile[p + 2 < cnt ? p + 2 : p + 1];
             if (b.strength != 0.0)
-                acc = acc + pair_unit<MODE, SYM, NB>(d, b, (int)(base + p + 1), x, NB ? nc.nbuf + NEARD * (p + 1) : nullptr) * b.strength;
+                acc = acc + pair_unit<MODE, SYM, NB>(d, b, (int)(base + p + 1), x,
+                                                     NB ? nc.nbuf + NEARD * (p + 1) : nullptr,
+                                                     NB ? nc.ncore + p + 1 : nullptr) *
+                                b.strength;
         }
     }
     return acc;
+}
+
+// The target a lane of a pair-sum launch evaluates: with a target map (McBuffers::tmap)
+// the map's entry for the slot, else the slot itself; a negative value -(i + 1) marks a
+// padding slot that evaluates target i and stores nothing.
+__device__ inline long target_of_slot(const int* tmap, long slot, long n_targets) {
+    if (tmap) return (long)tmap[slot];
+    return slot < n_targets ? slot : -n_targets;
 }
 
 // NB: near-field data staged with the records (sweep), in tiles of NB_TILE
@@ -693,21 +714,24 @@ constexpr int NB_TILE = 128;
 // out[t] = acc.
 template <int MODE, int SYM, bool NB = false>
 __global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restrict__ targets, long n_targets,
-                                                 long n_panels, double* __restrict__ out, int add) {
+                                                 long n_panels, double* __restrict__ out, int add,
+                                                 const int* __restrict__ tmap) {
     // staged tiles of 64 panels (19.5 KB): eight workgroups per CU, as without staging
     constexpr int TL = NB ? 64 : TILE;
     __shared__ PanelRec tile[TL];
     __shared__ double nbuf[NB ? TL * NEARD : 1];
+    __shared__ double ncore[NB ? TL : 1];
     __shared__ NearShared nsh;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long tc = t < n_targets ? t : n_targets - 1;
-    V3 x = load3(targets + 3 * tc);
+    const long slot = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long t = target_of_slot(tmap, slot, n_targets);
+    V3 x = load3(targets + 3 * (t >= 0 ? t : -t - 1));
     NearCtx nc;
     nc.nbuf = nbuf;
+    nc.ncore = ncore;
     nc.sh = &nsh;
     if (NB) block_box(d, x, nc);
     V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels, nc);
-    if (t < n_targets) {
+    if (t >= 0) {
         if (add) store3(out + 3 * t, load3(out + 3 * t) + acc);
         else store3(out + 3 * t, acc);
     }
@@ -729,6 +753,7 @@ struct McPlan {
     long long* flags;    // [n_jobs], == epoch once the partial sums are published
     long long epoch;
     int spin_limit;
+    const int* tmap;     // target map (McBuffers::tmap) or null for flat order
 };
 
 template <int MODE, int SYM, bool NB = false>
@@ -737,10 +762,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     constexpr int TL = NB ? NB_TILE : TILE;
     __shared__ PanelRec tile[TL];
     __shared__ double nbuf[NB ? NB_TILE * NEARD : 1];
+    __shared__ double ncore[NB ? NB_TILE : 1];
     __shared__ NearShared nsh;
     __shared__ int ready;
     NearCtx nc;
     nc.nbuf = nbuf;
+    nc.ncore = ncore;
     nc.sh = &nsh;
     const long L = pl.L;
     const long g0 = (long)blockIdx.x * pl.C;
@@ -754,12 +781,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const bool tail = b < je;  // last panels of job j (its first panels start the next range)
         const long p0 = tail ? L - (b - a) : 0;
         const long p1 = head ? (b - a) : L;
-        const long t = j * 256 + threadIdx.x;
-        const long tc = t < n_targets ? t : n_targets - 1;
-        V3 x = load3(targets + 3 * tc);
+        const long slot = j * 256 + threadIdx.x;
+        const long t = target_of_slot(pl.tmap, slot, n_targets);
+        V3 x = load3(targets + 3 * (t >= 0 ? t : -t - 1));
         if (NB) block_box(d, x, nc);
         V3 acc{0.0, 0.0, 0.0};
-        double* part = pl.partial + 3 * t;
+        double* part = pl.partial + 3 * slot;
         long start = p0;
         if (tail) {
             if (threadIdx.x == 0) {
@@ -787,7 +814,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             __threadfence();
             __syncthreads();
             if (threadIdx.x == 0) __hip_atomic_store(pl.flags + j, pl.epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else if (t < n_targets) {
+        } else if (t >= 0) {
             if (add) store3(out + 3 * t, load3(out + 3 * t) + acc);
             else store3(out + 3 * t, acc);
         }
@@ -2456,10 +2483,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const Inst* I, const int2* __restrict__ jobs, int n_jobs, unsigned* head) {
     __shared__ PanelRec tile[TL];
     __shared__ double nbuf[NB ? TL * NEARD : 1];
+    __shared__ double ncore[NB ? TL : 1];
     __shared__ NearShared nsh;
     __shared__ int slot;
     NearCtx nc;
     nc.nbuf = nbuf;
+    nc.ncore = ncore;
     nc.sh = &nsh;
     for (;;) {
         int j = next_job(head, &slot);
@@ -2845,8 +2874,8 @@ static int mc_slots_for() {
 
 template <int MODE, int SYM>
 static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_targets, long n_panels, double* out,
-                                   int add, McBuffers* mc, hipStream_t st) {
-    long n_jobs = (n_targets + 255) / 256;
+                                   int add, McBuffers* mc, hipStream_t st, const int* tmap, long map_jobs) {
+    long n_jobs = tmap ? map_jobs : (n_targets + 255) / 256;
     // near-field data staged in shared memory when the panels have ring-edge geometry
     const bool nb = d.segs && !d.corners;
     int slots = 0;
@@ -2864,6 +2893,7 @@ static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_
         pl.flags = mc->flags;
         pl.epoch = ++mc->epoch;
         pl.spin_limit = 200000;
+        pl.tmap = tmap;
         long grid = (n_jobs * n_panels + pl.C - 1) / pl.C;
         if (nb)
             hipLaunchKernelGGL((k_targets_mc<MODE, SYM, true>), dim3((unsigned)grid), dim3(256), 0, st, d, targets,
@@ -2874,26 +2904,28 @@ static hipError_t launch_targets_t(const Inst& d, const double* targets, long n_
     } else {
         if (nb)
             hipLaunchKernelGGL((k_targets<MODE, SYM, true>), dim3((unsigned)n_jobs), dim3(256), 0, st, d, targets,
-                               n_targets, n_panels, out, add);
+                               n_targets, n_panels, out, add, tmap);
         else
             hipLaunchKernelGGL((k_targets<MODE, SYM, false>), dim3((unsigned)n_jobs), dim3(256), 0, st, d, targets,
-                               n_targets, n_panels, out, add);
+                               n_targets, n_panels, out, add, tmap);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode, double* out,
-                          int add, McBuffers* mc, hipStream_t st) {
+                          int add, McBuffers* mc, hipStream_t st, const int* tmap, long map_jobs) {
     if (n_targets <= 0) return hipSuccess;
     bool sym = d.symmetry != 0;
     if (mode == MODE_FAST)
-        return sym ? launch_targets_t<MODE_FAST, 1>(d, targets, n_targets, n_panels, out, add, mc, st)
-                   : launch_targets_t<MODE_FAST, 0>(d, targets, n_targets, n_panels, out, add, mc, st);
+        return sym ? launch_targets_t<MODE_FAST, 1>(d, targets, n_targets, n_panels, out, add, mc, st, tmap, map_jobs)
+                   : launch_targets_t<MODE_FAST, 0>(d, targets, n_targets, n_panels, out, add, mc, st, tmap, map_jobs);
     if (mode == MODE_FP32_KAHAN)  // one workgroup per 256 targets (the Kahan state is not carried across pieces)
-        return sym ? launch_targets_t<MODE_FP32_KAHAN, 1>(d, targets, n_targets, n_panels, out, add, nullptr, st)
-                   : launch_targets_t<MODE_FP32_KAHAN, 0>(d, targets, n_targets, n_panels, out, add, nullptr, st);
-    return sym ? launch_targets_t<MODE_EXACT, 1>(d, targets, n_targets, n_panels, out, add, mc, st)
-               : launch_targets_t<MODE_EXACT, 0>(d, targets, n_targets, n_panels, out, add, mc, st);
+        return sym ? launch_targets_t<MODE_FP32_KAHAN, 1>(d, targets, n_targets, n_panels, out, add, nullptr, st, tmap,
+                                                          map_jobs)
+                   : launch_targets_t<MODE_FP32_KAHAN, 0>(d, targets, n_targets, n_panels, out, add, nullptr, st, tmap,
+                                                          map_jobs);
+    return sym ? launch_targets_t<MODE_EXACT, 1>(d, targets, n_targets, n_panels, out, add, mc, st, tmap, map_jobs)
+               : launch_targets_t<MODE_EXACT, 0>(d, targets, n_targets, n_panels, out, add, mc, st, tmap, map_jobs);
 }
 
 hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStream_t st) {
@@ -2901,7 +2933,9 @@ hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStre
     long end = d.end_index < np ? d.end_index : np;
     long n_panels = (long)d.N * d.nz_rows;
     if (end <= 0 || n_panels <= 0) return hipSuccess;
-    return launch_targets(d, d.pts, end, n_panels, mode, d.vel, 1, mc, st);
+    const bool map = mc && mc->tmap && mc->tmap_jobs > 0;
+    return launch_targets(d, d.pts, end, n_panels, mode, d.vel, 1, mc, st, map ? mc->tmap : nullptr,
+                          map ? mc->tmap_jobs : 0);
 }
 
 hipError_t launch_forces(const Inst& d, double dt, hipStream_t st) {

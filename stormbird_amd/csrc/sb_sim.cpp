@@ -149,6 +149,41 @@ int sb_sim_get_local_wing_angles(const sb_sim* s, double* out, size_t n) {
     return 0;
 }
 
+int sb_sim_set_target_tiling(sb_sim* s, int span_points_per_wave) {
+    if (!s) return -1;
+    if (span_points_per_wave != 0 &&
+        (span_points_per_wave < 1 || span_points_per_wave > 64 || 64 % span_points_per_wave != 0))
+        return -4;
+    return guarded([&] {
+        check(hipSetDevice(s->device), "hipSetDevice");
+        s->set_target_tiling(span_points_per_wave);
+    });
+}
+
+int sb_sim_get_target_tiling(const sb_sim* s) { return s ? s->mc.tmap_span : -1; }
+
+int sb_sim_get_wing_ranges(const sb_sim* s, long* ranges, size_t cap) {
+    if (!s || !ranges) return -1;
+    if (cap < 2 * (size_t)s->W) return -3;
+    for (int w = 0; w < s->W; ++w) {
+        ranges[2 * w] = s->setup.lfm.wing_start[w];
+        ranges[2 * w + 1] = s->setup.lfm.wing_end[w];
+    }
+    return 0;
+}
+
+int sb_sim_get_rigid_body_motion(const sb_sim* s, double* motion12) {
+    if (!s || !motion12) return -1;
+    const RigidBody& r = s->setup.lfm.rbm;
+    const V3 v[4] = {r.translation, r.rotation, r.velocity_linear, r.velocity_angular};
+    for (int k = 0; k < 4; ++k) {
+        motion12[3 * k] = v[k].x;
+        motion12[3 * k + 1] = v[k].y;
+        motion12[3 * k + 2] = v[k].z;
+    }
+    return 0;
+}
+
 int sb_sim_get_wake_state(const sb_sim* s, double* pts, double* vel, double* str, long* steps, int* first) {
     if (!s) return -1;
     int rc = guarded([&] {

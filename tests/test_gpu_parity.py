@@ -436,3 +436,38 @@ def test_set_wake_state_checks_sizes():
     sim.set_wake_state(st["points"], st["velocities"], st["strengths"], 3)
     st2 = sim.get_wake_state()
     assert same(st2["points"], st["points"]) and st2["steps_completed"] == 3
+
+
+def test_target_tiling_leaves_results_unchanged(oracle):
+    """The order in which the self-induction visits its targets (flat, or tiles of 8, 16, 64
+    span points per wave) changes no bit: one full-size EXACT step from the rolled-up wake,
+    and a small free wake over several steps against the oracle."""
+    js, case = S.four_sail_free_wake(nr_panels=2000, nr_strips=40)
+    pts, vel, gam, prev, _ = S.developed_four_sail_state(case)
+    dt = case["time_step"]
+    ref = None
+    for ws in (16, 0, 8, 64):
+        sim = sb.Simulation(js)
+        sim.target_tiling = ws
+        assert sim.target_tiling == ws
+        sim.set_wake_state(pts, vel, gam, 2000)
+        sim.previous_circulation_strength = prev
+        fs = S.uniform_freestream(sim.nr_freestream_points(), case["velocity"])
+        r = sim.do_step(time=2000 * dt, time_step=dt, freestream_velocity=fs)
+        got = (r.as_arrays(), sim.get_wake_state()["velocities"])
+        if ref is None:
+            ref = got
+        else:
+            for f in FIELDS:
+                assert same(got[0][f], ref[0][f]), f"tiling {ws}: {f}"
+            assert same(got[1], ref[1]), f"tiling {ws}: wake velocities"
+        del sim
+    js, case = S.four_sail_free_wake(nr_panels=40, nr_strips=10)
+    for ws in (1, 4, 32):
+        gpu = sb.Simulation(js)
+        gpu.target_tiling = ws
+        orc = oracle.Simulation(js, "pm")
+        fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+        step_both(gpu, orc, fs, 12, case["time_step"], wake_check_every=4)
+    with pytest.raises(RuntimeError):
+        sb.Simulation(js).target_tiling = 3
