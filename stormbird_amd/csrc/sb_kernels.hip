@@ -705,6 +705,74 @@ __device__ inline long target_of_slot(const int* tmap, long slot, long n_targets
     return slot < n_targets ? slot : -n_targets;
 }
 
+typedef const __attribute__((address_space(4))) PanelRec* ScalarRecs;
+
+// Record k read through the constant address space: a wave-uniform k gives
+// scalar loads into SGPRs.
+__device__ inline PanelRec scalar_rec(ScalarRecs recs, long k) {
+    PanelRec o;
+    o.cx = recs[k].cx;
+    o.cy = recs[k].cy;
+    o.cz = recs[k].cz;
+    o.nx = recs[k].nx;
+    o.ny = recs[k].ny;
+    o.nz = recs[k].nz;
+    o.far2 = recs[k].far2;
+    o.area_term = recs[k].area_term;
+    o.strength = recs[k].strength;
+    o.cbase = recs[k].cbase;
+    return o;
+}
+
+// sweep with the panel records read by scalar loads (the panel index is
+// wave-uniform) instead of through a shared tile: the records take no LDS
+// bandwidth, a tile barrier is needed only for the near-field staging, and the
+// zero-strength skip is a scalar branch.  The same pairs in the same order, so
+// the same bits as sweep.
+template <int MODE, int SYM, int TL, bool NB>
+__device__ inline V3 sweep_scalar(const Inst& d, V3 x, V3 acc, long p0l, long p1l, NearCtx& nc) {
+    const ScalarRecs recs = (ScalarRecs)d.rec;
+    const int p0 = __builtin_amdgcn_readfirstlane((int)p0l), p1 = __builtin_amdgcn_readfirstlane((int)p1l);
+    for (int base = p0; base < p1; base += TL) {
+        const int cnt = p1 - base < TL ? p1 - base : TL;
+        if (NB) {
+            __syncthreads();  // every wave is done with the previous tile's near-field records
+            stage_near(d, nc, base, cnt);
+        }
+        // Scalar loads return out of order, so waiting for one waits for all: each record
+        // is taken (its strength tested) before the next one's load is issued, and the
+        // scheduler barrier keeps the load behind the test; the load then runs under the
+        // current record's evaluation.
+        PanelRec a = scalar_rec(recs, base);
+        for (int p = 0; p < cnt; p += 2) {
+            const bool live_a = a.strength != 0.0;
+            __builtin_amdgcn_sched_barrier(0);
+            PanelRec b = scalar_rec(recs, base + (p + 1 < cnt ? p + 1 : p));
+            if (live_a)
+                acc = acc + pair_unit<MODE, SYM, NB>(d, a, base + p, x, NB ? nc.nbuf + NEARD * p : nullptr,
+                                                     NB ? nc.ncore + p : nullptr) *
+                                a.strength;
+            if (p + 1 >= cnt) break;
+            const bool live_b = b.strength != 0.0;
+            __builtin_amdgcn_sched_barrier(0);
+            a = scalar_rec(recs, base + (p + 2 < cnt ? p + 2 : p + 1));
+            if (live_b)
+                acc = acc + pair_unit<MODE, SYM, NB>(d, b, base + p + 1, x, NB ? nc.nbuf + NEARD * (p + 1) : nullptr,
+                                                     NB ? nc.ncore + p + 1 : nullptr) *
+                                b.strength;
+        }
+    }
+    return acc;
+}
+
+// The pair sum of the single-instance kernels: scalar-loaded records for the f64
+// modes, the shared record tile for FP32-Kahan.
+template <int MODE, int SYM, int TL, bool NB>
+__device__ inline V3 sweep_targets(const Inst& d, PanelRec* tile, V3 x, V3 acc, long p0, long p1, NearCtx& nc) {
+    if constexpr (MODE == MODE_FP32_KAHAN) return sweep_kahan<SYM, TL, NB>(d, tile, x, acc, p0, p1, nc);
+    else return sweep_scalar<MODE, SYM, TL, NB>(d, x, acc, p0, p1, nc);
+}
+
 // NB: near-field data staged with the records (sweep), in tiles of NB_TILE
 // panels so that four workgroups still fit a CU's shared memory.
 constexpr int NB_TILE = 128;
@@ -718,7 +786,7 @@ __global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restric
                                                  const int* __restrict__ tmap) {
     // staged tiles of 64 panels (19.5 KB): eight workgroups per CU, as without staging
     constexpr int TL = NB ? 64 : TILE;
-    __shared__ PanelRec tile[TL];
+    __shared__ PanelRec tile[MODE == MODE_FP32_KAHAN ? TL : 1];
     __shared__ double nbuf[NB ? TL * NEARD : 1];
     __shared__ double ncore[NB ? TL : 1];
     __shared__ NearShared nsh;
@@ -730,7 +798,7 @@ __global__ __launch_bounds__(256) void k_targets(Inst d, const double* __restric
     nc.ncore = ncore;
     nc.sh = &nsh;
     if (NB) block_box(d, x, nc);
-    V3 acc = sweep<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels, nc);
+    V3 acc = sweep_targets<MODE, SYM, TL, NB>(d, tile, x, V3{0.0, 0.0, 0.0}, 0, n_panels, nc);
     if (t >= 0) {
         if (add) store3(out + 3 * t, load3(out + 3 * t) + acc);
         else store3(out + 3 * t, acc);
@@ -760,7 +828,7 @@ template <int MODE, int SYM, bool NB = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_targets_mc(Inst d, const double* __restrict__ targets, long n_targets,
                                                     double* __restrict__ out, int add, McPlan pl) {
     constexpr int TL = NB ? NB_TILE : TILE;
-    __shared__ PanelRec tile[TL];
+    __shared__ PanelRec tile[MODE == MODE_FP32_KAHAN ? TL : 1];
     __shared__ double nbuf[NB ? NB_TILE * NEARD : 1];
     __shared__ double ncore[NB ? NB_TILE : 1];
     __shared__ NearShared nsh;
@@ -808,7 +876,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 start = 0;  // recompute the first panels here: same order, same bits
             }
         }
-        acc = sweep<MODE, SYM, TL, NB>(d, tile, x, acc, start, p1, nc);
+        acc = sweep_targets<MODE, SYM, TL, NB>(d, tile, x, acc, start, p1, nc);
         if (head) {
             store3(part, acc);
             __threadfence();
@@ -2345,24 +2413,6 @@ __device__ inline int next_job(unsigned* head, int* slot) {
 // queue by persistent workgroups.
 constexpr int COOP_WAVES = 16;
 constexpr int FX_COMPUTE = 15;  // compute waves; wave 15 folds
-typedef const __attribute__((address_space(4))) PanelRec* ScalarRecs;
-
-// Record k read through the constant address space: a wave-uniform k gives
-// scalar loads into SGPRs.
-__device__ inline PanelRec scalar_rec(ScalarRecs recs, long k) {
-    PanelRec o;
-    o.cx = recs[k].cx;
-    o.cy = recs[k].cy;
-    o.cz = recs[k].cz;
-    o.nx = recs[k].nx;
-    o.ny = recs[k].ny;
-    o.nz = recs[k].nz;
-    o.far2 = recs[k].far2;
-    o.area_term = recs[k].area_term;
-    o.strength = recs[k].strength;
-    o.cbase = recs[k].cbase;
-    return o;
-}
 
 // PER panels per compute wave per block (1, 2 or 3; the autotuner picks it):
 // blocks of 15 PER panels, a double-buffered term array of 2 x 15 PER x 1.5 KiB.
