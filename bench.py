@@ -145,10 +145,14 @@ def develop_wake(sim, steps_completed, rng):
     sim.set_wake_state(pts.reshape(-1, 3), st["velocities"], gam.ravel(), steps_completed)
 
 
-def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
+def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1, mode=0, mlp=None):
     """512 polar-sweep instances (seeded: 1-6 sails, 16-64 strips, chords 5-15 m, AR 2-6,
     200-3000 wake rows, AWA 20-180 deg, AWS 6-18 m/s) from a developed wake: one batched
-    step (sb_batch, graph-replayed) against the plain path (each instance stepped alone)."""
+    step (sb_batch, graph-replayed) against the plain path (each instance stepped alone),
+    both in the same arithmetic mode (0 EXACT, 1 FAST).  With a trained warm-start network,
+    the same developed steps are then timed with the judged warm start (per instance the
+    solver's first evaluation keeps the better of the previous circulation and the
+    network's guess)."""
     import torch
 
     import stormbird_amd as sb
@@ -159,7 +163,7 @@ def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
     setups = [m[0] for m in made]
     cases = [m[1] for m in made]
     t_create = time.perf_counter()
-    batch = sb.Batch(setups, mode=sb.MODE_EXACT)
+    batch = sb.Batch(setups, mode=mode)
     t_create = time.perf_counter() - t_create
     dts = np.array([c["time_step"] for c in cases])
     fs_all = np.concatenate([S.uniform_freestream(s.nr_freestream_points(), c["velocity"])
@@ -198,16 +202,42 @@ def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
                          want_result=False)
     k0 += 1
     torch.cuda.synchronize()
-    dev_ms = []
+    dev_ms, iters = [], []
     t0 = time.perf_counter()
     for k in range(steps):
-        batch.do_step_device(times=(k0 + k) * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(),
-                             n_points=n_pts, want_result=True)
+        res = batch.do_step_device(times=(k0 + k) * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(),
+                                   n_points=n_pts, want_result=True)
         dev_ms.append(batch.last_step_ms())
+        iters.append(float(np.mean(res.iterations)))
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    k0 += steps
     plain_rate = n_inst * plain_steps / t_plain
     batch_rate = n_inst * steps / t_batch
+    warm = None
+    if mlp is not None:
+        batch.set_warm_start(mlp, 3)
+        batch.do_step_device(times=k0 * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(), n_points=n_pts,
+                             want_result=False)  # captures the warm-start step graph
+        k0 += 1
+        torch.cuda.synchronize()
+        wms, witers, taken = [], [], 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            res = batch.do_step_device(times=(k0 + k) * dts, time_steps=dts, freestream_ptr=fs_dev.data_ptr(),
+                                       n_points=n_pts, want_result=True)
+            wms.append(batch.last_step_ms())
+            witers.append(float(np.mean(res.iterations)))
+            taken += batch.last_warm_taken()
+        torch.cuda.synchronize()
+        t_warm = time.perf_counter() - t0
+        warm_rate = n_inst * steps / t_warm
+        warm = {"mode": "judged (per instance the better of the previous circulation and the guess)",
+                "batched_warm_instance_steps_per_s": warm_rate, "batched_warm_ms_per_step": 1000.0 * t_warm / steps,
+                "batched_warm_device_ms_per_step": float(np.mean(wms)),
+                "iterations_mean_warm": float(np.mean(witers)), "iterations_mean_without": float(np.mean(iters)),
+                "guess_taken_fraction": taken / (n_inst * steps), "batched_warm_vs_plain": warm_rate / plain_rate}
+        batch.set_warm_start(None, 0)
     return {
         "workload": (f"{n_inst} polar-sweep instances (seed {seed}): 1-6 sails, 16-64 strips per sail, chords "
                      "5-15 m, AR 2-6, 200-3000 wake rows, AWA 20-180 deg, AWS 6-18 m/s, convected wakes, "
@@ -225,9 +255,11 @@ def batched_polar_sweep(n_inst=512, seed=2, steps=3, plain_steps=1):
         "steps": steps,
         "plain_steps": plain_steps,
         "create_s": t_create,
+        "mode": {0: "exact", 1: "fast"}[mode],
+        "iterations_mean": float(np.mean(iters)),
         "autotuner": {"fixed_per": tuning[0], "targets_tile": tuning[1],
                       "fixed_ms_per_panels_per_wave_1_2_3": [float(x) for x in tune_ms[:3]]},
-        "warm_start": None,
+        "warm_start": warm,
     }
 
 
@@ -749,7 +781,8 @@ def main():
 
     if args.only:
         if args.only == "batched":
-            out = batched_polar_sweep(n_inst=args.batch_instances, steps=args.steps)
+            out = {"exact": batched_polar_sweep(n_inst=args.batch_instances, steps=args.steps),
+                   "fast": batched_polar_sweep(n_inst=args.batch_instances, steps=args.steps, mode=1)}
         elif args.only == "warm":
             out, _ = warm_start_polar(n_inst=args.batch_instances, train_conditions=args.train_conditions)
         elif args.only == "quasi":
@@ -881,15 +914,20 @@ def main():
         line["quasi_steady"] = quasi_steady()
     if not args.no_csm:
         line["complete_sail_model"] = complete_sail_model_section()
-    if not args.no_batch:
-        del sim
-        line["batched"] = batched_polar_sweep(n_inst=args.batch_instances)
+    del sim
     if not args.no_warm:
         import gc
 
         gc.collect()
         line["warm_start"], mlp = warm_start_polar(n_inst=args.batch_instances,
                                                    train_conditions=args.train_conditions)
+    if not args.no_batch:
+        import gc
+
+        gc.collect()
+        line["batched"] = batched_polar_sweep(n_inst=args.batch_instances, mlp=mlp)
+        gc.collect()
+        line["batched_fast"] = batched_polar_sweep(n_inst=args.batch_instances, mode=1, mlp=mlp)
     if not args.no_route:
         import gc
 

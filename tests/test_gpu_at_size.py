@@ -4,7 +4,8 @@
   rows), developed as the benchmark develops it: every instance of the batched step
   equals the same instance stepped alone from the same state (results and wake state),
   and a seeded sample of 8 instances, among them the one with the most strips and the
-  one with the longest wake, equals the oracle.
+  one with the longest wake, equals the oracle; in FAST arithmetic the same 8 stay within
+  1e-10 of it.
 * The 64-ship route (4 sails x 48 strips, 1,500-row free wake, gusts, roll), developed
   the same way: a sample of ships equals the same ships stepped alone (results and wake
   state), and sampled wake points of each equal freestream + the oracle's induced
@@ -16,7 +17,7 @@ import pytest
 import stormbird_amd as sb
 from stormbird_amd import setups as S
 
-from test_gpu_parity import compare_results, same
+from test_gpu_parity import FIELDS, compare_results, rel_dev, same
 
 pytestmark = pytest.mark.gpu
 
@@ -80,6 +81,17 @@ def test_polar_sweep_512_instances_at_size(oracle):
         ro = orc.do_step(time=k0 * dts[i], time_step=dts[i], freestream_velocity=parts[i])
         compare_results(rb[i], ro, exact=True, where=f"instance {i} vs oracle")
         wake_equal(batch.instances[i].get_wake_state(), orc.get_wake_state(), f"instance {i} vs oracle")
+        # FAST arithmetic (the batched FAST step equals FAST alone bit for bit): within 1e-10
+        # of the oracle from the same state
+        fast = sb.Simulation(setups[i], mode=sb.MODE_FAST)
+        fast.set_wake_state(st["points"], st["velocities"], st["strengths"], st["steps_completed"])
+        fast.previous_circulation_strength = prev
+        rf = fast.do_step(time=k0 * dts[i], time_step=dts[i], freestream_velocity=parts[i])
+        af, ao = rf.as_arrays(), ro.as_arrays()
+        for f in FIELDS:
+            assert rel_dev(af[f], ao[f]) <= 1e-10, f"instance {i} FAST {f}: {rel_dev(af[f], ao[f]):.3e}"
+        assert abs(rf.iterations - ro.iterations) <= 1  # a 1e-14 difference may cross a stopping test
+        del fast
 
 
 @pytest.mark.timeout(900)
