@@ -274,12 +274,55 @@ def _perturbed(cases, rng):
     return out
 
 
-def warm_start_polar(n_inst=512, seed=3, train_conditions=16, train_steps=3000, minibatch=65536):
-    """Learned warm start on the polar-sweep distribution (seed 3): first-step solves of
-    512 instances under perturbed inflow give the training set; the network is trained on
-    the GPU (bf16 matrix-core forward/backward, Adam); held-out conditions are then solved
-    to a 1e-6 relative tolerance (max |Gamma_est - Gamma| < 1e-6 max |Gamma*|) from zero
-    (cold) and from the network's guess (warm), batched, and cold one instance at a time."""
+def polar_training_set(n_solves=200_000, seed=30, n_geometries=4096, per_batch=512):
+    """Training rows of the learned warm start: first-step circulation solves of fresh
+    polar-sweep geometries (n_geometries drawn from the polar-sweep distribution, in
+    batches of per_batch; seeds apart from the held-out instances), each under
+    ceil(n_solves / n_geometries) perturbed inflows, solved tight (max |Gamma_est - Gamma|
+    < 1e-9 |u| c / 2).  The first step sheds one wake row, so the solve does not depend on
+    the wake length: the training batches carry 8 rows.  Returns (features [rows, 64],
+    targets Gamma / scale [rows], solves, seconds)."""
+    import stormbird_amd as sb
+    from stormbird_amd import setups as S
+
+    n_batches = max(1, n_geometries // per_batch)
+    conditions = -(-n_solves // (n_batches * per_batch))
+    t0 = time.perf_counter()
+    X, T = [], []
+    solves = 0
+    for b in range(n_batches):
+        grng = np.random.default_rng([seed, b])
+        made = [S.polar_sweep_instance(grng, max_panels=8, min_panels=8) for _ in range(per_batch)]
+        cases = [m[1] for m in made]
+        batch = sb.Batch([m[0] for m in made], mode=sb.MODE_EXACT)
+        dts = np.array([c["time_step"] for c in cases])
+        scale0 = np.array([0.5 * math.sqrt(sum(x * x for x in c["velocity"])) * 10.0 for c in cases])
+        for s, t in zip(batch.instances, 1e-9 * scale0):
+            s.set_solver_settings(max_iterations=5000, damping=0.05, residual_tolerance=0.0, strength_tolerance=t)
+        batch.set_record_features(True)
+        npts = [s.nr_freestream_points() for s in batch.instances]
+        crng = np.random.default_rng([seed, b, 1])
+        for _ in range(conditions):
+            vel = _perturbed(cases, crng)
+            batch.reset()
+            fs = np.repeat(np.asarray(vel), npts, axis=0)
+            res = batch.do_step(times=None, time_steps=dts, freestream_velocity=fs)
+            x, sc = batch.get_features()
+            g = np.concatenate([r.force_input.circulation_strength for r in res])
+            X.append(x)
+            T.append((g / sc).astype(np.float32))
+            solves += per_batch
+        del batch
+    return np.concatenate(X), np.concatenate(T), solves, time.perf_counter() - t0
+
+
+def warm_start_polar(n_inst=512, seed=3, train_solves=200_000, train_steps=3000, minibatch=65536):
+    """Learned warm start on the polar-sweep distribution: the network is trained on the GPU
+    (bf16 matrix-core forward/backward, Adam) on `train_solves` first-step solves of fresh
+    geometries (polar_training_set); the held-out 512 instances (seed 3) are then solved
+    under perturbed inflow to a 1e-6 relative tolerance (max |Gamma_est - Gamma| < 1e-6
+    max |Gamma*|) from zero (cold) and from the network's guess (warm), batched, and cold one
+    instance at a time."""
     import torch
 
     import stormbird_amd as sb
@@ -302,20 +345,7 @@ def warm_start_polar(n_inst=512, seed=3, train_conditions=16, train_steps=3000, 
         for s, t in zip(batch.instances, tol):
             s.set_solver_settings(max_iterations=5000, damping=0.05, residual_tolerance=0.0, strength_tolerance=t)
 
-    # training set: tight first-step solves
-    t0 = time.perf_counter()
-    settings(1e-9 * scale0)
-    batch.set_record_features(True)
-    drng = np.random.default_rng(seed + 1)
-    X, T = [], []
-    for _ in range(train_conditions):
-        res = solve(_perturbed(cases, drng))
-        x, sc = batch.get_features()
-        g = np.concatenate([r.force_input.circulation_strength for r in res])
-        X.append(x)
-        T.append((g / sc).astype(np.float32))
-    X, T = np.concatenate(X), np.concatenate(T)
-    t_data = time.perf_counter() - t0
+    X, T, n_solves, t_data = polar_training_set(train_solves)
     # training on the GPU
     t0 = time.perf_counter()
     mlp = sb.Mlp(seed=seed)
@@ -375,8 +405,10 @@ def warm_start_polar(n_inst=512, seed=3, train_conditions=16, train_steps=3000, 
     result = {
         "workload": (f"{n_inst} polar-sweep instances (seed {seed}), first-step circulation solves under "
                      "perturbed inflow (+-6 deg, x0.8-1.2), damping 0.05, stop at max |Gamma_est - Gamma| < "
-                     "1e-6 max |Gamma*| (residual test off)"),
-        "train_solves": int(n_inst * train_conditions),
+                     "1e-6 max |Gamma*| (residual test off); network trained on first-step solves of 4096 fresh "
+                     "geometries of the same distribution"),
+        "train_solves": int(n_solves),
+        "train_geometries": 4096,
         "train_rows": int(X.shape[0]),
         "train_steps": train_steps,
         "minibatch": nb,
@@ -404,31 +436,31 @@ def warm_start_polar(n_inst=512, seed=3, train_conditions=16, train_steps=3000, 
 
 def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
     """Route simulation: 64 ships x 4 wing sails x 48 strips, free wake of 1,500 rows
-    (self-induction on every wake point), von Karman gusts (15 %, 200 m), roll 10 deg at
-    12 s; timed from a synthesized developed wake (3 real steps, then every row shed) and
-    extrapolated to 10,000 steps.  With a trained network, one more step runs with the
-    judged per-step warm start."""
+    (self-induction on every wake point), in 2 wind conditions (power-law true wind, ship
+    speed, von Karman gusts along and across the wind: 15 %, 200 m), each ship with a
+    turbulence realization of its own, roll 10 deg at 12 s.  Every ship starts from the
+    rolled-up wake of its condition after 1,500 steps from the start (points, velocities and
+    strengths stored in stormbird_amd/data/route_developed_wakes_1500rows.npz, made by
+    `bench.py --only route-rollup`); the timed steps run from there with the default kernel
+    variants, and are extrapolated to 10,000 steps.  With a trained network, one more step
+    runs with the judged per-step warm start."""
     import torch
 
-    from stormbird_amd.route import RouteSimulation
+    from stormbird_amd.route import DEVELOPED_ROUTE, RouteSimulation
 
     t0 = time.perf_counter()
     route = RouteSimulation(n_ships=n_ships, seed=seed)
-    for _ in range(3):
-        route.step(want_result=False)
-    drng = np.random.default_rng(seed + 100)
-    for s in route.batch.instances:
-        develop_wake(s, 3, drng)
+    route.load_developed(DEVELOPED_ROUTE)
     t_setup = time.perf_counter() - t0
     pairs = 0.0
     for s in route.batch.instances:
         n, P = s.nr_ctrl_points, s.nr_span_points
         M = s.nr_panels // n
         pairs += float(P * (M + 1)) * float(n * M) + float(n * n * (M - 1)) + float(n * n)
-    route.step(want_result=False)
-    # the autotuner on a sample of the queues (one resident wave of jobs per variant)
-    _, tuning, tune_ms = route.batch.autotune(None, reps=1, max_jobs=1024)
-    route.step(want_result=False)  # captures the step graph with the chosen variants
+    near = []
+    for c in range(len(route.conditions)):
+        nn, tot = route.batch.instances[c].count_self_induction_pairs()
+        near.append(nn / tot)
     torch.cuda.synchronize()
     dev_ms, iters = [], []
     t0 = time.perf_counter()
@@ -440,32 +472,68 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
     t = (time.perf_counter() - t0) / steps
     out = {
         "workload": (f"{n_ships} ships (seed {seed}) x 4 wing sails 30x10 m x 48 strips, free wake of 1500 rows, "
-                     "all wake points self-induced, von Karman gusts 15 % / 200 m, roll 10 deg at 12 s, "
-                     "developed-wake steady state"),
+                     "all wake points self-induced, 2 wind conditions (power-law true wind 8-14 m/s, ship "
+                     "4-7 m/s), von Karman gusts along and across the wind 15 % / 200 m (1000 linspace "
+                     "frequencies, 0.0005-0.5 Hz), roll 10 deg at 12 s, rolled-up wakes of a 1,500-step run "
+                     "from the start"),
         "n_ships": n_ships,
         "biot_savart_pairs_per_batched_step": pairs,
+        "near_field_pair_fraction_per_condition": near,
         "batched_s_per_step": t,
         "batched_device_ms_per_step": float(np.mean(dev_ms)),
         "ship_steps_per_s": n_ships / t,
         "pairs_per_s": pairs / t,
-        "fp64_tflops_far_field_estimate": pairs * FLOP_FAR / t / 1e12,
         "extrapolated_10000_steps_h": 10000 * t / 3600.0,
         "iterations_mean": float(np.mean(iters)),
         "setup_s": t_setup,
-        "freestream": "resident in device memory, refilled per ship per step (inside the timed step)",
-        "autotuner": {"fixed_per": tuning[0], "targets_tile": tuning[1],
-                      "sample_ms": {"fixed_1_2_3": [float(x) for x in tune_ms[:3]],
-                                    "targets_64_128_256": [float(x) for x in tune_ms[3:]]}},
+        "freestream": ("WindEnvironment semantics evaluated by the batch: control points on the host (gusts, "
+                       "profile, ship speed), every wake point on the device, inside the timed step"),
+        "kernel_variants": dict(zip(("fixed_per", "targets_tile"), route.batch.get_tuning())),
     }
     if mlp is not None:
         route.batch.set_warm_start(mlp, 3)
-        route.step(want_result=False)  # captures the warm-start graph
         res = route.step(want_result=True)
         out["judged_warm_start"] = {"iterations_mean": float(np.mean([r.iterations for r in res])),
                                     "taken_fraction": route.batch.last_warm_taken() / n_ships,
                                     "fallbacks": route.batch.last_fallbacks(),
                                     "device_ms_per_step": route.batch.last_step_ms()}
     return out
+
+
+def route_rollup(steps=1500, n_conditions=2, seed=5, dump=None, resume=None, budget_s=None):
+    """The route's ships 0..n_conditions-1 (one per wind condition) run from the start for
+    `steps` steps in FAST arithmetic (within 1e-10 of EXACT per step), so that their wakes
+    roll up; `dump` stores them (RouteSimulation.store_developed).  With `budget_s`, the run
+    stops after that many seconds, and `resume` continues from what it stored.  Progress on
+    stderr every 100 steps or 60 s."""
+    import stormbird_amd as sb
+    from stormbird_amd.route import RouteSimulation
+
+    route = RouteSimulation(n_ships=n_conditions, n_conditions=n_conditions, seed=seed, mode=sb.MODE_FAST)
+    if resume:
+        route.load_developed(resume)
+    t0 = last = time.perf_counter()
+    res = None
+    while route.k < steps:
+        k = route.k
+        res = route.step(want_result=True)
+        now = time.perf_counter()
+        if (k + 1) % 100 == 0 or k == steps - 1 or now - last > 60.0:
+            last = now
+            print(f"route rollup: step {k + 1}, {now - t0:.1f} s, iterations "
+                  f"{[int(x) for x in res.iterations]}", file=sys.stderr, flush=True)
+        if budget_s is not None and time.perf_counter() - t0 > budget_s:
+            break
+    total = time.perf_counter() - t0
+    done = route.k >= steps
+    near = []
+    for c in range(n_conditions):
+        nn, tot = route.batch.instances[c].count_self_induction_pairs()
+        near.append(nn / tot)
+    if dump:
+        route.store_developed(dump)
+    return {"steps_completed": route.k, "done": done, "total_s": total,
+            "near_field_pair_fraction_per_condition": near}
 
 
 def single_wing_sail(steps=200):
@@ -757,12 +825,17 @@ def main():
     ap.add_argument("--no-warm", action="store_true")
     ap.add_argument("--no-route", action="store_true")
     ap.add_argument("--route-ships", type=int, default=64)
-    ap.add_argument("--train-conditions", type=int, default=16, help="inflow conditions per instance for training")
+    ap.add_argument("--train-solves", type=int, default=200_000, help="first-step solves in the warm start's training set")
     ap.add_argument("--no-quasi", action="store_true")
     ap.add_argument("--no-single", action="store_true")
     ap.add_argument("--no-csm", action="store_true")
-    ap.add_argument("--dump-state", default=None, help="with --only full: write the final wake state (.npz)")
-    ap.add_argument("--only", choices=("batched", "warm", "route", "quasi", "csm", "full", "single"), default=None,
+    ap.add_argument("--resume", default=None, help="with --only route-rollup: continue from this checkpoint")
+    ap.add_argument("--budget-s", type=float, default=None,
+                    help="with --only route-rollup: stop after this many seconds and write a checkpoint")
+    ap.add_argument("--dump-state", default=None,
+                    help="with --only full or route-rollup: write the final wake state (.npz)")
+    ap.add_argument("--only", choices=("batched", "warm", "route", "route-rollup", "quasi", "csm", "full", "single"),
+                    default=None,
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -784,7 +857,7 @@ def main():
             out = {"exact": batched_polar_sweep(n_inst=args.batch_instances, steps=args.steps),
                    "fast": batched_polar_sweep(n_inst=args.batch_instances, steps=args.steps, mode=1)}
         elif args.only == "warm":
-            out, _ = warm_start_polar(n_inst=args.batch_instances, train_conditions=args.train_conditions)
+            out, _ = warm_start_polar(n_inst=args.batch_instances, train_solves=args.train_solves)
         elif args.only == "quasi":
             out = quasi_steady()
         elif args.only == "single":
@@ -793,6 +866,9 @@ def main():
             out = complete_sail_model_section()
         elif args.only == "full":
             out = full_run(steps=args.panels, nr_strips=args.strips, dump=args.dump_state)
+        elif args.only == "route-rollup":
+            out = route_rollup(steps=args.panels, dump=args.dump_state, resume=args.resume,
+                               budget_s=args.budget_s)
         else:
             out = route_simulation(n_ships=args.route_ships, steps=args.steps)
         print(json.dumps({"section": args.only, **out}))
@@ -919,8 +995,7 @@ def main():
         import gc
 
         gc.collect()
-        line["warm_start"], mlp = warm_start_polar(n_inst=args.batch_instances,
-                                                   train_conditions=args.train_conditions)
+        line["warm_start"], mlp = warm_start_polar(n_inst=args.batch_instances, train_solves=args.train_solves)
     if not args.no_batch:
         import gc
 

@@ -108,7 +108,9 @@ long sb_sim_get_freestream_points(const sb_sim* sim, double* xyz, size_t cap);
  * the freestream at every point of sb_sim_get_freestream_points, host memory. */
 int sb_sim_do_step(sb_sim* sim, double time, double time_step, const double* freestream_xyz,
                    size_t n_points, sb_step_result* out);
-/* Same with `freestream_xyz` already in device memory (HBM-resident input). */
+/* Same with `freestream_xyz` already in device memory (HBM-resident input),
+ * read after the work queued so far on the null stream (where the caller wrote
+ * it, e.g. torch's default stream). */
 int sb_sim_do_step_device(sb_sim* sim, double time, double time_step,
                           const double* freestream_xyz_device, size_t n_points,
                           sb_step_result* out);
@@ -276,11 +278,30 @@ sb_sim* sb_batch_instance(sb_batch* b, long i);
 long sb_batch_freestream_offsets(const sb_batch* b, long* offsets, size_t cap);
 /* One time step of every instance.  times/dts: [n] (times may be NULL);
  * fs: the concatenated freestream, n_points points (host memory, or device
- * memory for the _device variant); results: [n] or NULL. */
+ * memory for the _device variant, read after the work queued so far on the
+ * null stream); results: [n] or NULL. */
 int sb_batch_do_step(sb_batch* b, const double* times, const double* dts, const double* fs, size_t n_points,
                      sb_step_result* results);
 int sb_batch_do_step_device(sb_batch* b, const double* times, const double* dts, const double* fs,
                             size_t n_points, sb_step_result* results);
+/* Wind steps (no counterpart in the reference for a batch; per instance they
+ * are CompleteSailModel::freestream_velocity, complete_sail_model/mod.rs:215-255,
+ * followed by the step, :174-191, without a controller): a wind environment
+ * (environment.rs:21-34 serde form; NULL or "{}" for the defaults) and one wind
+ * condition per instance (wind_condition/mod.rs:18-39 serde form, with its
+ * discretized gust spectra), parsed once; replaced by the next call.  -3 for a
+ * quasi-steady batch, -4 for a string that does not parse. */
+int sb_batch_set_wind(sb_batch* b, const char* environment_json, const char* const* wind_conditions_json);
+/* One time step of every instance in the wind set by sb_batch_set_wind, at
+ * times[i] with ship_velocities[i] (along the environment's zero direction):
+ * the unsteady apparent wind at the control points (with the inflow
+ * corrections, on the host) and at every wake point where it is before the
+ * step (on the device, straight into the batch's freestream). */
+int sb_batch_do_step_wind(sb_batch* b, const double* times, const double* dts, const double* ship_velocities,
+                          sb_step_result* results);
+/* The freestream of the last batched step, n_points points (the concatenation
+ * that sb_batch_freestream_offsets describes), into host memory. */
+int sb_batch_get_freestream(const sb_batch* b, double* out, size_t n_points);
 /* Device time of the last batched step (ms, HIP events around the step). */
 double sb_batch_last_step_ms(const sb_batch* b);
 /* sb_sim_reset on every instance. */
@@ -323,7 +344,10 @@ void sb_mlp_destroy(sb_mlp* m);
 long sb_mlp_nr_params(const sb_mlp* m);
 int sb_mlp_get_params(const sb_mlp* m, float* params, size_t cap);
 int sb_mlp_set_params(sb_mlp* m, const float* params, size_t n);
-/* Device pointers: x [n][64] fp32, y/t [n] fp32. */
+/* Device pointers: x [n][64] fp32, y/t [n] fp32.  The calls read their device
+ * inputs after the work queued so far on the null stream (where the caller
+ * wrote them, e.g. torch's default stream), and return when their outputs are
+ * written. */
 int sb_mlp_forward(sb_mlp* m, const float* x, long n, float* y);
 /* Mean squared error of the predictions against t and its gradient with
  * respect to every parameter (host array, parameter layout). */

@@ -145,6 +145,9 @@ pub mod ffi {
         pub fn sb_batch_do_step_device(b: *mut sb_batch, times: *const f64, dts: *const f64, fs: *const f64, n_points: usize, results: *mut sb_step_result) -> c_int;
         pub fn sb_batch_last_step_ms(b: *const sb_batch) -> f64;
         pub fn sb_batch_reset(b: *mut sb_batch) -> c_int;
+        pub fn sb_batch_set_wind(b: *mut sb_batch, environment_json: *const c_char, wind_conditions_json: *const *const c_char) -> c_int;
+        pub fn sb_batch_do_step_wind(b: *mut sb_batch, times: *const f64, dts: *const f64, ship_velocities: *const f64, results: *mut sb_step_result) -> c_int;
+        pub fn sb_batch_get_freestream(b: *const sb_batch, out: *mut f64, n_points: usize) -> c_int;
         pub fn sb_batch_set_tuning(b: *mut sb_batch, fixed_per: c_int, targets_tile: c_int) -> c_int;
         pub fn sb_batch_get_tuning(b: *const sb_batch, fixed_per: *mut c_int, targets_tile: *mut c_int) -> c_int;
         pub fn sb_batch_autotune(b: *mut sb_batch, cache_path: *const c_char, flags: c_int, reps: c_int, max_jobs: c_long, timings: *mut f64, cap: usize) -> c_int;

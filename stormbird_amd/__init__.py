@@ -14,6 +14,7 @@ built or no GPU is present, constructing a Simulation raises.
 from __future__ import annotations
 
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -98,6 +99,9 @@ def lib():
         L.sb_batch_freestream_offsets.argtypes = [vp, ctypes.POINTER(ctypes.c_long), ctypes.c_size_t]
         L.sb_batch_do_step.argtypes = [vp, _dp, _dp, _dp, ctypes.c_size_t, vp]
         L.sb_batch_do_step_device.argtypes = [vp, _dp, _dp, ctypes.c_void_p, ctypes.c_size_t, vp]
+        L.sb_batch_set_wind.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p)]
+        L.sb_batch_do_step_wind.argtypes = [vp, _dp, _dp, _dp, vp]
+        L.sb_batch_get_freestream.argtypes = [vp, _dp, ctypes.c_size_t]
         L.sb_batch_last_step_ms.restype = ctypes.c_double
         L.sb_batch_last_step_ms.argtypes = [vp]
         L.sb_batch_reset.argtypes = [vp]
@@ -522,6 +526,43 @@ class Batch:
         if rc != 0:
             _err("sb_batch_do_step_device", rc)
         return self._collect() if want_result else None
+
+    def set_wind(self, environment, conditions):
+        """Wind of every instance: a WindEnvironment (JSON string or dict; None for the
+        defaults) and one WindCondition per instance (JSON strings or dicts), as the
+        reference's serde forms (wind/environment.rs:21-34, wind_condition/mod.rs:18-39)."""
+        if len(conditions) != len(self.instances):
+            raise ValueError(f"{len(conditions)} wind conditions for {len(self.instances)} instances")
+        env = environment if isinstance(environment, (str, type(None))) else json.dumps(environment)
+        enc = [(c if isinstance(c, str) else json.dumps(c)).encode() for c in conditions]
+        arr = (ctypes.c_char_p * len(enc))(*enc)
+        rc = lib().sb_batch_set_wind(self._h, env.encode() if env else None, arr)
+        if rc != 0:
+            _err("sb_batch_set_wind", rc)
+
+    def do_step_wind(self, *, times, time_steps, ship_velocities, want_result: bool = True):
+        """One step of every instance in its wind (set_wind) at times[i], the ship moving
+        at ship_velocities[i]: CompleteSailModel::freestream_velocity then the step
+        (complete_sail_model/mod.rs:174-255), without a controller."""
+        if times is None:
+            raise ValueError("a wind step needs the times")
+        ts, dts = self._arrays(times, time_steps)
+        sv = _f64(ship_velocities, (-1,))
+        if sv.shape[0] != len(self.instances):
+            raise ValueError(f"{sv.shape[0]} ship velocities for {len(self.instances)} instances")
+        res = ctypes.cast(self._res, ctypes.c_void_p) if want_result else None
+        rc = lib().sb_batch_do_step_wind(self._h, _p(ts), _p(dts), _p(sv), res)
+        if rc != 0:
+            _err("sb_batch_do_step_wind", rc)
+        return self._collect() if want_result else None
+
+    def get_freestream(self) -> np.ndarray:
+        """The freestream of the last step [points, 3] (split by freestream_offsets)."""
+        out = np.zeros((int(self.freestream_offsets[-1]), 3))
+        rc = lib().sb_batch_get_freestream(self._h, _p(out), out.shape[0])
+        if rc != 0:
+            _err("sb_batch_get_freestream", rc)
+        return out
 
     def last_step_ms(self) -> float:
         return lib().sb_batch_last_step_ms(self._h)

@@ -91,6 +91,7 @@ struct sb_batch {
     hipGraphExec_t exec = nullptr;
     long graph_sig[3] = {-1, -1, -1};
     hipEvent_t ev[2] = {};
+    hipEvent_t ev_input = nullptr;  // orders device inputs written on the null stream
     double last_ms = 0.0;
     int use_graph = 1;
     // learned warm start
@@ -98,6 +99,7 @@ struct sb_batch {
     int warm_mode = 0;          // 0 off, 1 first step only, 2 every step, 3 every step judged
     bool record_features = false;
     long n_strips = 0;
+    std::vector<long> h_strip_off;  // [n + 1]
     long* d_strip_off = nullptr;
     float *d_feat = nullptr, *d_scale = nullptr, *d_pred = nullptr;
     unsigned *d_fallbacks = nullptr, *h_fallbacks = nullptr;
@@ -110,20 +112,29 @@ struct sb_batch {
     // work of the last dynamic step, for the tuning key: pairs of the fixed-velocity and
     // self-induction sums as the kernels evaluate them
     double fixed_work = 0.0, target_work = 0.0;
+    // wind steps: one wind environment, one wind condition per instance
+    bool has_wind = false;
+    WindEnvironmentCfg wind_env;
+    std::vector<WindConditionCfg> wind_cond;
+    long* d_wind_blk = nullptr;  // [n + 1] 256-point blocks of each instance's freestream
+    long wind_blocks = 0;
+    WindJob *h_wind = nullptr, *d_wind = nullptr;
+    double *h_wind_ctrl = nullptr, *d_wind_ctrl = nullptr;  // control-point freestream [n_strips][3]
 
     ~sb_batch() {
         (void)hipSetDevice(device);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         void* dev[] = {d_geo, d_fs, d_out, d_inst, d_off, d_jobs, d_counters, d_strip_off, d_feat, d_scale, d_pred,
-                       d_fallbacks, d_singular};
+                       d_fallbacks, d_singular, d_wind_blk, d_wind, d_wind_ctrl};
         for (void* p : dev)
             if (p) (void)hipFree(p);
-        void* host[] = {h_geo, h_out, h_inst, h_jobs, h_fallbacks, h_singular};
+        void* host[] = {h_geo, h_out, h_inst, h_jobs, h_fallbacks, h_singular, h_wind, h_wind_ctrl};
         for (void* p : host)
             if (p) (void)hipHostFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (ev_input) (void)hipEventDestroy(ev_input);
         if (stream) (void)hipStreamDestroy(stream);
         sims.clear();
     }
@@ -149,6 +160,7 @@ struct sb_batch {
         }
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        check(hipEventCreateWithFlags(&ev_input, hipEventDisableTiming), "hipEventCreate");
         quasi = sims[0]->setup.quasi;
         bool any_lin = false;
         for (auto& s : sims) any_lin = any_lin || s->setup.solver.kind == 1 || s->setup.solver.start_linearized;
@@ -241,6 +253,7 @@ struct sb_batch {
         std::vector<long> so(n + 1, 0);
         for (int i = 0; i < n; ++i) so[i + 1] = so[i] + sims[i]->N;
         n_strips = so[n];
+        h_strip_off = so;
         d_strip_off = dalloc<long>(n + 1);
         check(hipMemcpyAsync(d_strip_off, so.data(), sizeof(long) * (n + 1), hipMemcpyHostToDevice, stream), "upload");
         d_feat = dalloc<float>(n_strips * MLP_FEATURES);
@@ -256,6 +269,36 @@ struct sb_batch {
         bd.pred = d_pred;
         bd.fallbacks = d_fallbacks;
         check(hipStreamSynchronize(stream), "batch create");
+    }
+
+    // complete_sail_model/mod.rs:215-255 for every instance: parsed once, evaluated
+    // each wind step.
+    void set_wind(const char* environment_json, const char* const* conditions_json) {
+        if (quasi) throw std::invalid_argument("wind steps run on dynamic batches");
+        if (!conditions_json) throw std::invalid_argument("null wind condition list");
+        WindEnvironmentCfg e = parse_wind_environment(environment_json ? environment_json : "{}");
+        std::vector<WindConditionCfg> c;
+        for (int i = 0; i < n; ++i) {
+            if (!conditions_json[i]) throw std::invalid_argument("null wind condition string");
+            c.push_back(parse_wind_condition(conditions_json[i]));
+        }
+        check(hipSetDevice(device), "hipSetDevice");
+        if (!d_wind) {
+            std::vector<long> blk(n + 1, 0);
+            for (int i = 0; i < n; ++i) blk[i + 1] = blk[i] + (fs_off[i + 1] / 3 - fs_off[i] / 3 + 255) / 256;
+            wind_blocks = blk[n];
+            d_wind_blk = dalloc<long>(n + 1);
+            check(hipMemcpyAsync(d_wind_blk, blk.data(), sizeof(long) * (n + 1), hipMemcpyHostToDevice, stream),
+                  "upload");
+            h_wind = halloc<WindJob>(n);
+            d_wind = dalloc<WindJob>(n);
+            h_wind_ctrl = halloc<double>(3 * n_strips);
+            d_wind_ctrl = dalloc<double>(3 * n_strips);
+            check(hipStreamSynchronize(stream), "wind setup");
+        }
+        wind_env = std::move(e);
+        wind_cond = std::move(c);
+        has_wind = true;
     }
 
     void reset() {
@@ -275,10 +318,15 @@ struct sb_batch {
         return d;
     }
 
+    // ship_velocities != nullptr: a wind step (fs unused), the freestream of every
+    // instance from its wind condition at times[i] (set_wind).
     void step(const double* times, const double* dts, const double* fs, size_t n_fs, bool fs_on_device,
-              sb_step_result* results) {
+              sb_step_result* results, const double* ship_velocities = nullptr) {
         check(hipSetDevice(device), "hipSetDevice");
-        if ((long)n_fs != fs_total / 3) throw std::invalid_argument("freestream length mismatch");
+        const bool wind = ship_velocities != nullptr;
+        if (wind && (!has_wind || !times))
+            throw std::invalid_argument(!has_wind ? "no wind conditions set" : "a wind step needs the times");
+        if (!wind && (long)n_fs != fs_total / 3) throw std::invalid_argument("freestream length mismatch");
         if (!dts) throw std::invalid_argument("null time steps");
         for (int i = 0; i < n; ++i) {
             if (sims[i]->first_done != first_done || sims[i]->steps_completed != sims[0]->steps_completed)
@@ -289,8 +337,19 @@ struct sb_batch {
         parallel_for(n, [&](int i) {
             sims[i]->setup.lfm.update_global();
             sims[i]->pack_geometry(h_geo + geo_off[i]);
+            if (wind) {
+                const sb_sim& s = *sims[i];
+                WindJob& j = h_wind[i];
+                j.w = freestream_at_ctrl_points(wind_env, wind_cond[i], ship_velocities[i], times[i], s.setup.lfm,
+                                                reinterpret_cast<V3*>(h_wind_ctrl + 3 * h_strip_off[i]));
+                j.ctrl = d_wind_ctrl + 3 * h_strip_off[i];
+                j.dst = d_fs + fs_off[i];
+                j.n_ctrl = s.N;
+                j.n_pts = (fs_off[i + 1] - fs_off[i]) / 3 - s.N;
+            }
         });
         if (quasi) {
+            if (wind) throw std::invalid_argument("wind steps run on dynamic batches");
             step_quasi(first, times, dts, fs, fs_on_device, results);
             return;
         }
@@ -337,9 +396,20 @@ struct sb_batch {
         }
 
         // freestream into the slab (outside the graph: the caller's pointer may change)
-        check(hipMemcpyAsync(d_fs, fs, sizeof(double) * fs_total,
-                             fs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
-              "upload freestream");
+        if (wind) {
+            // the wake points where they are before this step (stage 0)
+            for (int i = 0; i < n; ++i) h_wind[i].pts = h_inst[i].pts;
+            check(hipMemcpyAsync(d_wind_ctrl, h_wind_ctrl, sizeof(double) * 3 * n_strips, hipMemcpyHostToDevice,
+                                 stream),
+                  "upload control-point wind");
+            check(hipMemcpyAsync(d_wind, h_wind, sizeof(WindJob) * n, hipMemcpyHostToDevice, stream), "upload wind");
+            check(launch_batch_wind(d_wind, d_wind_blk, n, wind_blocks, stream), "wind at every point");
+        } else {
+            if (fs_on_device) wait_for_null_stream(stream, ev_input);
+            check(hipMemcpyAsync(d_fs, fs, sizeof(double) * fs_total,
+                                 fs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
+                  "upload freestream");
+        }
         check(hipEventRecord(ev[0], stream), "event");
         const bool apply = mlp && (warm_mode >= 2 || (warm_mode == 1 && first));
         const bool feats = apply || record_features;
@@ -402,6 +472,7 @@ struct sb_batch {
         }
         if (any_lin_solver && any_iter)
             throw std::invalid_argument("a quasi-steady batch uses one solver (Linearized or SimpleIterative)");
+        if (fs_on_device) wait_for_null_stream(stream, ev_input);
         check(hipMemcpyAsync(d_fs, fs, sizeof(double) * fs_total,
                              fs_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
               "upload freestream");
@@ -779,6 +850,28 @@ int sb_batch_do_step_device(sb_batch* b, const double* times, const double* dts,
                             sb_step_result* results) {
     if (!b || !fs) return -1;
     return guarded([&] { b->step(times, dts, fs, n_points, true, results); });
+}
+
+int sb_batch_set_wind(sb_batch* b, const char* environment_json, const char* const* wind_conditions_json) {
+    if (!b || !wind_conditions_json) return -1;
+    return guarded([&] { b->set_wind(environment_json, wind_conditions_json); });
+}
+
+int sb_batch_do_step_wind(sb_batch* b, const double* times, const double* dts, const double* ship_velocities,
+                          sb_step_result* results) {
+    if (!b || !times || !dts || !ship_velocities) return -1;
+    return guarded([&] { b->step(times, dts, nullptr, 0, false, results, ship_velocities); });
+}
+
+int sb_batch_get_freestream(const sb_batch* b, double* out, size_t n_points) {
+    if (!b || !out) return -1;
+    if (n_points != (size_t)(b->fs_total / 3)) return -3;
+    return guarded([&] {
+        check(hipSetDevice(b->device), "hipSetDevice");
+        check(hipMemcpyAsync(out, b->d_fs, sizeof(double) * b->fs_total, hipMemcpyDeviceToHost, b->stream),
+              "download freestream");
+        check(hipStreamSynchronize(b->stream), "get freestream");
+    });
 }
 
 double sb_batch_last_step_ms(const sb_batch* b) { return b ? b->last_ms : -1.0; }

@@ -167,6 +167,22 @@ void controller_output(const SetPointsCfg& sp, const ControllerInput& in, double
 
 }  // namespace
 
+namespace sb_host {
+
+WindField freestream_at_ctrl_points(const WindEnvironmentCfg& e, const WindConditionCfg& c, double ship_velocity,
+                                    double time, const LineModel& m, V3* out) {
+    WindField w = wind_field(e, c, ship_velocity, time);
+    for (int i = 0; i < m.n(); ++i) out[i] = apparent_wind_at(w, m.ctrl_points[i]);
+    if (e.has_inflow) {
+        std::vector<std::pair<long, long>> ranges;
+        for (int wing = 0; wing < m.nw(); ++wing) ranges.push_back({m.wing_start[wing], m.wing_end[wing]});
+        apply_inflow_corrections(e, apparent_wind_direction(e, w, 10.0), out, m.ctrl_points.data(), ranges);
+    }
+    return w;
+}
+
+}  // namespace sb_host
+
 struct sb_csm {
     std::unique_ptr<sb_sim> sim;
     WindEnvironmentCfg env;
@@ -184,24 +200,10 @@ struct sb_csm {
 
     const LineModel& lfm() const { return sim->setup.lfm; }
 
-    // complete_sail_model/mod.rs:213-255, control-point part: the apparent wind
-    // at every control point, then the inflow corrections of each sail at the
-    // apparent wind direction of the 10 m reference height.
-    void ctrl_freestream(const WindField& w) {
-        sim->setup.lfm.update_global();
-        const LineModel& m = lfm();
-        for (int i = 0; i < m.n(); ++i) fs_ctrl[i] = apparent_wind_at(w, m.ctrl_points[i]);
-        if (!env.has_inflow) return;
-        std::vector<std::pair<long, long>> ranges;
-        for (int wing = 0; wing < m.nw(); ++wing) ranges.push_back({m.wing_start[wing], m.wing_end[wing]});
-        apply_inflow_corrections(env, apparent_wind_direction(env, w, 10.0), fs_ctrl.data(), m.ctrl_points.data(),
-                                 ranges);
-    }
-
     // Freestream of every point into the simulation's buffer (on its stream).
     void freestream(const WindConditionCfg& wc, double ship_velocity, double time) {
-        WindField w = wind_field(env, wc, ship_velocity, time);
-        ctrl_freestream(w);
+        sim->setup.lfm.update_global();
+        WindField w = freestream_at_ctrl_points(env, wc, ship_velocity, time, lfm(), fs_ctrl.data());
         check(hipSetDevice(sim->device), "hipSetDevice");
         check(hipMemcpyAsync(sim->d_fs, fs_ctrl.data(), sizeof(V3) * fs_ctrl.size(), hipMemcpyHostToDevice,
                              sim->stream),
@@ -285,7 +287,8 @@ struct sb_csm {
 
     void apply_controller_wind(double time, double dt, const WindConditionCfg& wc, double ship_velocity,
                                double loading) {  // complete_sail_model/mod.rs:193-211, 257-288
-        ctrl_freestream(wind_field(env, wc, ship_velocity, time));
+        sim->setup.lfm.update_global();
+        freestream_at_ctrl_points(env, wc, ship_velocity, time, lfm(), fs_ctrl.data());
         update(time, dt, input_from_velocity(loading, fs_ctrl));
     }
 

@@ -172,6 +172,19 @@ hipError_t launch_count_pairs(const Inst& d, const double* targets, long n_targe
                               double* partial, long* n_blocks, hipStream_t st);
 // Wind-environment freestream at the wake points (after the N control points of d.fs).
 hipError_t launch_wind_points(const Inst& d, const WindField& w, hipStream_t st);
+
+// Freestream of one batched instance from its wind field: the control points'
+// values come from the host (inflow corrections are applied there), the wake
+// points' are evaluated where the points are.
+struct WindJob {
+    WindField w;
+    const double* ctrl;  // [n_ctrl][3]
+    const double* pts;   // [n_pts][3] wake points
+    double* dst;         // [n_ctrl + n_pts][3]
+    long n_ctrl, n_pts;
+};
+// blk_off[n + 1]: 256-point blocks per instance, prefix sums (device)
+hipError_t launch_batch_wind(const WindJob* jobs, const long* blk_off, int n, long blocks, hipStream_t st);
 hipError_t launch_libm(int fn, const double* x, const double* y, double* out, long n, hipStream_t st);
 
 // ---------------------------------------------------------------------------

@@ -34,6 +34,25 @@ inline void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e), (int)e);
 }
 
+// Device inputs handed over by pointer are written by the caller on the null
+// stream (the default stream of most callers, torch's included), and the
+// library's own streams (hipStreamNonBlocking) do not synchronize with it: work
+// that reads such an input first waits, on the device, for everything the null
+// stream holds so far.  The host does not block.
+inline void wait_for_null_stream(hipStream_t st, hipEvent_t ev) {
+    check(hipEventRecord(ev, nullptr), "event on the null stream");
+    check(hipStreamWaitEvent(st, ev, 0), "wait for the null stream");
+}
+
+// complete_sail_model/mod.rs:215-255, control-point part: the wind field of
+// condition c at `time` for a ship at `ship_velocity` (environment.rs:73-190),
+// and the unsteady apparent wind at the control points of m (global positions)
+// with the environment's inflow corrections at the apparent wind direction of
+// the 10 m reference height.  The wake points take the returned field on the
+// device (k_wind_points, kb_wind).
+WindField freestream_at_ctrl_points(const WindEnvironmentCfg& e, const WindConditionCfg& c, double ship_velocity,
+                                    double time, const LineModel& m, V3* out);
+
 template <class T>
 T* dalloc(size_t n) {
     void* p = nullptr;
@@ -99,6 +118,7 @@ struct sb_sim {
 
     std::vector<double> h_geo, h_out;
     hipEvent_t ev[kTimers + 1] = {};
+    hipEvent_t ev_input = nullptr;  // orders device inputs written on the null stream
     double pairs[3] = {0, 0, 0};
     double timer_ms[kTimers] = {0};
 
@@ -112,6 +132,7 @@ struct sb_sim {
             if (p) (void)hipFree(p);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (ev_input) (void)hipEventDestroy(ev_input);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -323,6 +344,7 @@ struct sb_sim {
         check(hipSetDevice(device), "hipSetDevice");
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : ev) check(hipEventCreate(&e), "hipEventCreate");
+        check(hipEventCreateWithFlags(&ev_input, hipEventDisableTiming), "hipEventCreate");
         const LineModel& m = setup.lfm;
         N = m.n();
         W = m.nw();
@@ -428,6 +450,7 @@ struct sb_sim {
 
     void upload_freestream(const double* fs, size_t n, bool on_device) {
         size_t bytes = sizeof(double) * 3 * n;
+        if (on_device) wait_for_null_stream(stream, ev_input);
         check(hipMemcpyAsync(d_fs, fs, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
               "upload freestream");
     }

@@ -2303,6 +2303,17 @@ __device__ inline int owner_of(const long* off, int n, long b) {
     return lo;
 }
 
+// Freestream of batched instances from their wind fields (environment.rs:182-190 at
+// every point): one 256-point block per thread block of the flat range.
+__global__ void kb_wind(const WindJob* J, const long* blk_off, int n) {
+    const int i = owner_of(blk_off, n, (long)blockIdx.x);
+    const WindJob& j = J[i];
+    const long q = (long)(blockIdx.x - blk_off[i]) * 256 + threadIdx.x;
+    if (q < j.n_ctrl) store3(j.dst + 3 * q, load3(j.ctrl + 3 * q));
+    else if (q < j.n_ctrl + j.n_pts) store3(j.dst + 3 * q, apparent_wind_at(j.w, load3(j.pts + 3 * (q - j.n_ctrl))));
+}
+
+
 __global__ void kb_prepare(const Inst* I, int first_step) {
     const Inst d = I[blockIdx.x];
     prepare_body(d, first_step);
@@ -3029,6 +3040,12 @@ hipError_t launch_wind_points(const Inst& d, const WindField& w, hipStream_t st)
     long n = (long)d.P * (d.M + 1);
     if (d.quasi || n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_wind_points, dim3(blocks_for(n, 256)), dim3(256), 0, st, d, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_wind(const WindJob* jobs, const long* blk_off, int n, long blocks, hipStream_t st) {
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(kb_wind, dim3((unsigned)blocks), dim3(256), 0, st, jobs, blk_off, n);
     return hipGetLastError();
 }
 

@@ -242,6 +242,7 @@ unsigned nblk(long n, int b) { return (unsigned)((n + b - 1) / b); }
 struct sb_mlp {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t ev_input = nullptr;  // orders device inputs written on the null stream
     long n_params = 0;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
     uint16_t* Wb[kLayers] = {};
@@ -270,6 +271,7 @@ struct sb_mlp {
             if (Z[l]) (void)hipFree(Z[l]);
             if (Hb[l]) (void)hipFree(Hb[l]);
         }
+        if (ev_input) (void)hipEventDestroy(ev_input);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -277,6 +279,7 @@ struct sb_mlp {
         device = dev;
         check(hipSetDevice(dev), "hipSetDevice");
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreateWithFlags(&ev_input, hipEventDisableTiming), "hipEventCreate");
         n_params = param_offset(kLayers);
         p = dalloc<float>(n_params);
         g = dalloc<float>(n_params);
@@ -488,6 +491,8 @@ int sb_mlp_forward(sb_mlp* m, const float* x_dev, long n, float* y_dev) {
     if (!m || !x_dev || !y_dev) return -1;
     if (n <= 0) return -3;
     return guarded([&] {
+        check(hipSetDevice(m->device), "hipSetDevice");
+        wait_for_null_stream(m->stream, m->ev_input);
         m->forward(x_dev, n, nullptr);
         check(hipMemcpyAsync(y_dev, m->y, sizeof(float) * n, hipMemcpyDeviceToDevice, m->stream), "copy");
         check(hipStreamSynchronize(m->stream), "forward");
@@ -498,6 +503,8 @@ int sb_mlp_gradients(sb_mlp* m, const float* x_dev, const float* t_dev, long n, 
     if (!m || !x_dev || !t_dev) return -1;
     if (n <= 0) return -3;
     return guarded([&] {
+        check(hipSetDevice(m->device), "hipSetDevice");
+        wait_for_null_stream(m->stream, m->ev_input);
         float l = m->gradients(x_dev, t_dev, n);
         if (loss) *loss = l;
         if (grads_host) {
@@ -512,6 +519,8 @@ int sb_mlp_train_step(sb_mlp* m, const float* x_dev, const float* t_dev, long n,
     if (!m || !x_dev || !t_dev) return -1;
     if (n <= 0) return -3;
     return guarded([&] {
+        check(hipSetDevice(m->device), "hipSetDevice");
+        wait_for_null_stream(m->stream, m->ev_input);
         float l = m->train_step(x_dev, t_dev, n, lr);
         if (loss) *loss = l;
     });

@@ -6,10 +6,11 @@
   and a seeded sample of 8 instances, among them the one with the most strips and the
   one with the longest wake, equals the oracle; in FAST arithmetic the same 8 stay within
   1e-10 of it.
-* The 64-ship route (4 sails x 48 strips, 1,500-row free wake, gusts, roll), developed
-  the same way: a sample of ships equals the same ships stepped alone (results and wake
-  state), and sampled wake points of each equal freestream + the oracle's induced
-  velocity at the GPU's post-step wake.
+* The 64-ship route (4 sails x 48 strips, 1,500-row free wake, gusts, roll) from the
+  stored rolled-up wakes: the freestream the batch evaluates at the control points and at
+  sampled wake points equals the oracle's WindEnvironment, a sample of ships equals the
+  same ships stepped alone (results and wake state), and sampled wake points of each
+  equal freestream + the oracle's induced velocity at the GPU's post-step wake.
 """
 import numpy as np
 import pytest
@@ -96,47 +97,53 @@ def test_polar_sweep_512_instances_at_size(oracle):
 
 @pytest.mark.timeout(900)
 def test_route_64_ships_at_size(oracle):
-    import bench
-    from stormbird_amd.route import RouteSimulation, ship_setup
+    import oracle_csm
 
-    print("route: building 64 ships", flush=True)
+    from stormbird_amd.route import RouteSimulation, wind_environment
+
+    print("route: building 64 ships, loading the rolled-up wakes", flush=True)
     route = RouteSimulation(n_ships=64, seed=5)
-    for _ in range(3):
-        route.step(want_result=False)
-    drng = np.random.default_rng(105)
-    for s in route.batch.instances:
-        bench.develop_wake(s, 3, drng)
+    route.load_developed()
+    route.step(want_result=False)  # the first step from the stored wakes
     sample = [0, 21, 42, 63]
     before = {i: (route.batch.instances[i].get_wake_state(), route.batch.instances[i].previous_circulation_strength.copy())
               for i in sample}
     k = route.k
-    print("route: batched step of 64 ships from the developed state", flush=True)
+    print("route: batched step of 64 ships", flush=True)
     res = route.step(want_result=True)
+    fs_all = route.batch.get_freestream()
+    off = route.batch.freestream_offsets
+    m = oracle_csm.Libm("pm")
+    env = oracle_csm.WindEnvironment(wind_environment(), m)
     for i in sample:
         print(f"route: ship {i} alone", flush=True)
         dt = float(route.dt[i])
-        t_prev, t = (k - 1) * dt, k * dt
-        fs = route.freestream_host(i)
-        js, _ = ship_setup(route.awa[i])
-        alone = sb.Simulation(js, mode=sb.MODE_EXACT)
-        # the motion state of step k: rotation at t_prev, then at t (angular velocity by difference)
-        alone.set_translation_and_rotation_with_finite_difference_for_the_velocity(dt, [0, 0, 0],
-                                                                                   [route.roll(t_prev), 0, 0])
-        alone.set_translation_and_rotation_with_finite_difference_for_the_velocity(dt, [0, 0, 0],
-                                                                                   [route.roll(t), 0, 0])
+        fs = fs_all[off[i]:off[i + 1]]
+        alone = sb.Simulation(route.setups[i], mode=sb.MODE_EXACT)
+        # the motion state of step k: rotation at (k - 1) dt, then at k dt (angular velocity by difference)
+        for kk in (k - 1, k):
+            alone.set_translation_and_rotation_with_finite_difference_for_the_velocity(
+                dt, [0, 0, 0], [route.roll(kk * dt), 0, 0])
         st, prev = before[i]
         alone.set_wake_state(st["points"], st["velocities"], st["strengths"], st["steps_completed"])
         alone.previous_circulation_strength = prev
-        ra = alone.do_step(time=t, time_step=dt, freestream_velocity=fs)
+        # the wind at the control points and at sampled wake points, as the reference evaluates it
+        pts = alone.get_freestream_velocity_points()
+        n = alone.nr_ctrl_points
+        idx = np.concatenate([np.arange(n), n + np.random.default_rng(i).choice(pts.shape[0] - n, 600, replace=False)])
+        c = oracle_csm.WindCondition(route.wind_conditions[i], m)
+        linear = oracle_csm.scale(env.zero, float(route.ship_speed[i]))
+        want = np.array([env.unsteady_apparent(c, tuple(pts[q]), linear, k * dt) for q in idx])
+        assert same(fs[idx], want), f"ship {i}: freestream"
+        ra = alone.do_step(time=k * dt, time_step=dt, freestream_velocity=fs)
         compare_results(res[i], ra, exact=True, where=f"ship {i}")
         w = route.batch.instances[i].get_wake_state()
         wake_equal(w, alone.get_wake_state(), f"ship {i}")
         del alone
         print(f"route: ship {i} sampled wake points against the oracle", flush=True)
-        orc = oracle.Simulation(js, "pm")
+        orc = oracle.Simulation(route.setups[i], "pm")
         orc.set_wake_state(w["points"], w["velocities"], w["strengths"], w["steps_completed"])
-        idx = np.random.default_rng(i).choice(w["points"].shape[0], 600, replace=False)
-        u = orc.induced_velocities(w["points"][idx])
-        n = route.batch.instances[i].nr_ctrl_points
-        assert same(w["velocities"][idx], fs[n:][idx] + u), f"ship {i}: sampled wake velocities"
+        wi = np.random.default_rng(100 + i).choice(w["points"].shape[0], 600, replace=False)
+        u = orc.induced_velocities(w["points"][wi])
+        assert same(w["velocities"][wi], fs[n:][wi] + u), f"ship {i}: sampled wake velocities"
         assert np.max(np.abs(u)) > 0.0

@@ -96,30 +96,48 @@ def test_batch_rejects_bad_input():
         batch.do_step(times=None, time_steps=0.1, freestream_velocity=np.zeros((3, 3)))
 
 
-def test_route_simulation_batch_equals_alone():
-    """Rolling ships with gusts and free wakes: batched steps equal each ship stepped alone."""
-    from stormbird_amd.route import RouteSimulation
+def test_route_simulation_batch_equals_alone(oracle):
+    """Rolling ships in gusty wind with free wakes: the batch's freestream is the reference's
+    WindEnvironment at every point (the oracle's restatement, bit for bit: power-law profile,
+    gusts along and across the wind, ship speed), and the batched steps equal each ship
+    stepped alone with that freestream."""
+    import oracle_csm
 
-    route = RouteSimulation(n_ships=3, seed=5, nr_strips=8, nr_panels=12)
-    alone = [sb.Simulation(js, mode=sb.MODE_EXACT) for js in
-             [__import__("stormbird_amd.route", fromlist=["ship_setup"]).ship_setup(a, 8, 12)[0] for a in route.awa]]
+    from stormbird_amd.route import RouteSimulation, wind_environment
+
+    route = RouteSimulation(n_ships=3, n_conditions=2, seed=5, nr_strips=8, nr_panels=12, nr_freqs=50)
+    alone = [sb.Simulation(js, mode=sb.MODE_EXACT) for js in route.setups]
+    m = oracle_csm.Libm("pm")
+    env = oracle_csm.WindEnvironment(wind_environment(), m)
+    conds = [oracle_csm.WindCondition(wc, m) for wc in route.wind_conditions]
+    off = route.batch.freestream_offsets
     for k in range(7):
         print(f"route step {k}", flush=True)
-        rb = route.step()
+        pts = []
         for i, sim in enumerate(alone):
-            t = (route.k - 1) * route.dt[i]
+            dt = float(route.dt[i])
             sim.set_translation_and_rotation_with_finite_difference_for_the_velocity(
-                float(route.dt[i]), [0.0, 0.0, 0.0], [route.roll(t), 0.0, 0.0])
-            fs = route.freestream_host(i)
-            ra = sim.do_step(time=t, time_step=float(route.dt[i]), freestream_velocity=fs)
+                dt, [0.0, 0.0, 0.0], [route.roll(route.k * dt), 0.0, 0.0])
+            pts.append(sim.get_freestream_velocity_points())
+        t = route.k * route.dt
+        rb = route.step()
+        fs_all = route.batch.get_freestream()
+        for i, sim in enumerate(alone):
+            fs = fs_all[off[i]:off[i + 1]]
+            linear = oracle_csm.scale(env.zero, float(route.ship_speed[i]))
+            want = np.array([env.unsteady_apparent(conds[i], tuple(p), linear, float(t[i])) for p in pts[i]])
+            assert same(fs, want), f"step {k} ship {i}: freestream"
+            ra = sim.do_step(time=float(t[i]), time_step=float(route.dt[i]), freestream_velocity=fs)
             compare_results(rb[i], ra, exact=True, where=f"step {k} ship {i}")
     # a judged warm start (untrained network) changes only the initial guess
-    m = sb.Mlp(seed=1)
-    route.batch.set_warm_start(m, 3)
+    net = sb.Mlp(seed=1)
+    route.batch.set_warm_start(net, 3)
     res = route.step()
     assert route.batch.last_warm_taken() + route.batch.last_fallbacks() <= 3
     for r in res:
         assert np.all(np.isfinite(r.force_input.circulation_strength))
+    with pytest.raises(ValueError):
+        route.batch.set_wind(wind_environment(), route.wind_conditions[:2])
 
 
 @pytest.mark.parametrize("solver", [None, {"start_with_linearized_solution": True,

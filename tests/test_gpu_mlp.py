@@ -7,7 +7,9 @@
 - training with the GPU's own forward/backward/Adam reduces the loss;
 - on a small polar sweep, a network trained on first-step solves lowers the
   fixed-point iterations to a 1e-6 relative tolerance, and the solve still
-  ends at the same circulation.
+  ends at the same circulation;
+- a first-step solve does not depend on the wake length (the training set's
+  instances carry 8 wake rows).
 """
 import math
 
@@ -18,6 +20,7 @@ import stormbird_amd as sb
 from stormbird_amd import setups as S
 
 import mlp_reference as R
+from test_gpu_parity import compare_results
 
 pytestmark = pytest.mark.gpu
 
@@ -131,3 +134,33 @@ def test_warm_start_lowers_iterations():
         scale = np.max(np.abs(g))
         assert np.max(np.abs(rw.force_input.circulation_strength - g)) <= 1e-4 * scale
         assert np.max(np.abs(rc.force_input.circulation_strength - g)) <= 1e-4 * scale
+
+
+def test_first_step_does_not_depend_on_wake_length():
+    """The training set is made of first-step solves of instances that carry 8 wake rows:
+    the first step sheds one row, so its circulation, forces and features equal those of the
+    same geometry with its full wake length, bit for bit."""
+    import json
+
+    rng = np.random.default_rng(2)
+    made = [S.polar_sweep_instance(rng) for _ in range(6)]
+    long_setups = [m[0] for m in made]
+    short_setups = []
+    for js in long_setups:
+        d = json.loads(js)
+        d["simulation_settings"]["Dynamic"]["wake"]["nr_panels_per_line_element"] = 8
+        short_setups.append(json.dumps(d))
+    cases = [m[1] for m in made]
+    dts = np.array([c["time_step"] for c in cases])
+    out = []
+    for setups in (long_setups, short_setups):
+        b = sb.Batch(setups)
+        b.set_record_features(True)
+        fs = np.repeat(np.array([c["velocity"] for c in cases]), [s.nr_freestream_points() for s in b.instances],
+                       axis=0)
+        res = b.do_step(times=None, time_steps=dts, freestream_velocity=fs)
+        out.append((res, b.get_features()))
+    (rl, (xl, sl)), (rs, (xs, ss)) = out
+    for i in range(6):
+        compare_results(rs[i], rl[i], exact=True, where=f"instance {i}")
+    assert np.array_equal(xl, xs) and np.array_equal(sl, ss)
