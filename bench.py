@@ -371,6 +371,7 @@ def warm_start_polar(n_inst=512, seed=3, train_solves=200_000, train_steps=3000,
     tol = np.array([1e-6 * np.max(np.abs(g)) for g in gref])
     settings(tol)
     batch.set_warm_start(None, 0)
+    batch.set_record_features(True)
     cold = solve(vel)
     ms_cold = batch.last_step_ms()
     x, sc = batch.get_features()
@@ -452,6 +453,7 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
     route = RouteSimulation(n_ships=n_ships, seed=seed)
     route.load_developed(DEVELOPED_ROUTE)
     t_setup = time.perf_counter() - t0
+    print(f"route: {n_ships} ships loaded in {t_setup:.1f} s", file=sys.stderr, flush=True)
     pairs = 0.0
     for s in route.batch.instances:
         n, P = s.nr_ctrl_points, s.nr_span_points
@@ -468,6 +470,7 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
         res = route.step(want_result=True)
         dev_ms.append(route.batch.last_step_ms())
         iters.append(np.mean([r.iterations for r in res]))
+        print(f"route: step {route.k}, {dev_ms[-1]:.0f} ms on the device", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / steps
     out = {
@@ -498,6 +501,94 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
                                     "fallbacks": route.batch.last_fallbacks(),
                                     "device_ms_per_step": route.batch.last_step_ms()}
     return out
+
+
+def route_warm_start(n_ships=64, train_seed=11, eval_seed=5, nr_panels=60, warmup=40, record=120, eval_steps=40,
+                     train_steps=1500, minibatch=65536):
+    """The judged per-step warm start of the route, trained on route steps: the features
+    include the solver's default initial guess (the previous step's circulation), so the
+    network predicts this step's circulation from the previous one and this step's inflow
+    (gusts, roll).  Training data: `record` steps of a route of other wind conditions and
+    turbulence (seed train_seed, 8 conditions) with short wakes (`nr_panels` rows), each
+    solved tight (max |Gamma_est - Gamma| < 1e-9 |u| c / 2 or 400 iterations).  Evaluation:
+    the route's own conditions (seed eval_seed) with short wakes and the default solver
+    settings, `eval_steps` steps without and then with the judged warm start, from the same
+    state."""
+    import torch
+
+    import stormbird_amd as sb
+    from stormbird_amd.route import RouteSimulation
+
+    t0 = time.perf_counter()
+    tr = RouteSimulation(n_ships=n_ships, n_conditions=8, seed=train_seed, nr_panels=nr_panels)
+    for s in tr.batch.instances:
+        s.set_solver_settings(max_iterations=400, damping=0.05, residual_tolerance=0.0, strength_tolerance=1e-9 * 60.0)
+    for _ in range(warmup):
+        tr.step(want_result=False)
+    tr.batch.set_record_features(True)
+    X, T = [], []
+    for _ in range(record):
+        res = tr.step(want_result=True)
+        x, sc = tr.batch.get_features()
+        g = np.concatenate([r.force_input.circulation_strength for r in res])
+        X.append(x)
+        T.append((g / sc).astype(np.float32))
+    X, T = np.concatenate(X), np.concatenate(T)
+    del tr
+    t_data = time.perf_counter() - t0
+    print(f"route warm start: {X.shape[0]} training rows in {t_data:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    mlp = sb.Mlp(seed=train_seed)
+    xd = torch.from_numpy(X).to("cuda")
+    td = torch.from_numpy(T).to("cuda")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(train_seed)
+    nb = min(minibatch, X.shape[0])
+    loss = None
+    for it in range(train_steps):
+        idx = torch.randint(0, X.shape[0], (nb,), device="cuda", generator=gen)
+        xb = xd.index_select(0, idx).contiguous()
+        tb = td.index_select(0, idx).contiguous()
+        loss = mlp.train_step(xb.data_ptr(), tb.data_ptr(), nb, 1e-3 if it < 0.7 * train_steps else 2e-4)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"route warm start: trained in {t_train:.1f} s (mse {loss:.3e})", file=sys.stderr, flush=True)
+    # evaluation: the same steps from the same state, without and with the judged warm start
+    ev = RouteSimulation(n_ships=n_ships, seed=eval_seed, nr_panels=nr_panels)
+    for _ in range(warmup):
+        ev.step(want_result=False)
+    saved = [(s.get_wake_state(), s.previous_circulation_strength.copy()) for s in ev.batch.instances]
+    k0 = ev.k
+
+    def run(warm):
+        ev.batch.set_warm_start(mlp if warm else None, 3 if warm else 0)
+        for s, (st, prev) in zip(ev.batch.instances, saved):
+            s.set_wake_state(st["points"], st["velocities"], st["strengths"], st["steps_completed"])
+            s.previous_circulation_strength = prev
+        ev.set_motion(k0 - 2)
+        ev.set_motion(k0 - 1)
+        ev.k = k0
+        its, resid, taken = [], [], 0
+        for _ in range(eval_steps):
+            r = ev.step(want_result=True)
+            its.append(np.asarray(r.iterations, dtype=np.float64))
+            resid.append(np.asarray(r.residual, dtype=np.float64))
+            taken += ev.batch.last_warm_taken() if warm else 0
+        return np.concatenate(its), np.concatenate(resid), taken
+
+    it_cold, res_cold, _ = run(False)
+    it_warm, res_warm, taken = run(True)
+    return {
+        "workload": (f"route steps with short wakes ({nr_panels} rows), {n_ships} ships; trained on {record} steps of "
+                     f"another route (seed {train_seed}, 8 wind conditions), evaluated on {eval_steps} steps of the "
+                     f"benchmark route's conditions (seed {eval_seed}) with the default solver settings "
+                     "(at most 40 iterations, residual 1e-4)"),
+        "train_rows": int(X.shape[0]), "train_steps": train_steps, "final_train_mse": loss,
+        "data_s": t_data, "train_s": t_train,
+        "iterations_mean_without": float(it_cold.mean()), "iterations_mean_judged": float(it_warm.mean()),
+        "residual_mean_without": float(res_cold.mean()), "residual_mean_judged": float(res_warm.mean()),
+        "guess_taken_fraction": taken / float(n_ships * eval_steps),
+    }, mlp
 
 
 def route_rollup(steps=1500, n_conditions=2, seed=5, dump=None, resume=None, budget_s=None):
@@ -534,6 +625,44 @@ def route_rollup(steps=1500, n_conditions=2, seed=5, dump=None, resume=None, bud
         route.store_developed(dump)
     return {"steps_completed": route.k, "done": done, "total_s": total,
             "near_field_pair_fraction_per_condition": near}
+
+
+def mlp_kernels(rows=65536, steps=20, infer_rows=73468):
+    """The warm-start network's GEMMs alone: `steps` Adam steps on a minibatch of `rows`
+    random rows, and `steps` forward passes over `infer_rows` rows (the strips of the
+    512-instance polar sweep), timed with HIP events around the stream's work (the library
+    synchronizes each call).  Algorithmic flops per row: forward 2 (64*256 + 3*256*256 +
+    256); a training step adds the weight gradients of every layer and the input gradients
+    of layers 1-3, 2 (64*256 + 3*256*256) + 2 (3*256*256)."""
+    import torch
+
+    import stormbird_amd as sb
+
+    rng = np.random.default_rng(0)
+    X = torch.from_numpy(rng.standard_normal((max(rows, infer_rows), 64)).astype(np.float32)).to("cuda")
+    T = torch.from_numpy(rng.standard_normal(rows).astype(np.float32)).to("cuda")
+    Y = torch.zeros(infer_rows, dtype=torch.float32, device="cuda")
+    m = sb.Mlp(seed=1)
+    fwd = 2.0 * (64 * 256 + 3 * 256 * 256 + 256)
+    train = fwd + 2.0 * (64 * 256 + 3 * 256 * 256) + 2.0 * (3 * 256 * 256)
+    for _ in range(2):
+        m.train_step(X.data_ptr(), T.data_ptr(), rows, 1e-4)
+        m.forward(X.data_ptr(), infer_rows, Y.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.train_step(X.data_ptr(), T.data_ptr(), rows, 1e-4)
+    t_train = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.forward(X.data_ptr(), infer_rows, Y.data_ptr())
+    t_fwd = (time.perf_counter() - t0) / steps
+    return {"train_rows": rows, "train_step_ms": 1e3 * t_train,
+            "train_tflops_host_timed": train * rows / t_train / 1e12,
+            "forward_rows": infer_rows, "forward_ms": 1e3 * t_fwd,
+            "forward_tflops_host_timed": fwd * infer_rows / t_fwd / 1e12,
+            "note": "host-timed whole calls (launch and synchronization included); per-kernel times in the "
+                    "rocprofv3 summary of `bench.py --only mlp`"}
 
 
 def single_wing_sail(steps=200):
@@ -834,7 +963,8 @@ def main():
                     help="with --only route-rollup: stop after this many seconds and write a checkpoint")
     ap.add_argument("--dump-state", default=None,
                     help="with --only full or route-rollup: write the final wake state (.npz)")
-    ap.add_argument("--only", choices=("batched", "warm", "route", "route-rollup", "quasi", "csm", "full", "single"),
+    ap.add_argument("--only", choices=("batched", "warm", "route", "route-rollup", "route-warm", "quasi", "csm",
+                                       "full", "single", "mlp"),
                     default=None,
                     help="run one extra section alone and print its JSON (for profiling)")
     args = ap.parse_args()
@@ -866,6 +996,10 @@ def main():
             out = complete_sail_model_section()
         elif args.only == "full":
             out = full_run(steps=args.panels, nr_strips=args.strips, dump=args.dump_state)
+        elif args.only == "route-warm":
+            out, _ = route_warm_start()
+        elif args.only == "mlp":
+            out = mlp_kernels()
         elif args.only == "route-rollup":
             out = route_rollup(steps=args.panels, dump=args.dump_state, resume=args.resume,
                                budget_s=args.budget_s)
@@ -1007,7 +1141,9 @@ def main():
         import gc
 
         gc.collect()
-        line["route"] = route_simulation(n_ships=args.route_ships, mlp=mlp)
+        route_warm, route_mlp = route_warm_start(n_ships=args.route_ships)
+        line["route"] = route_simulation(n_ships=args.route_ships, mlp=route_mlp)
+        line["route"]["warm_start_short_wakes"] = route_warm
     print(json.dumps(line))
 
 

@@ -889,7 +889,6 @@ int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode) {
         return -4;
     }
     return guarded([&] {
-        if (m) mlp_reserve(m, b->n_strips);
         b->mlp = mode > 0 ? m : nullptr;
         b->warm_mode = mode;
     });

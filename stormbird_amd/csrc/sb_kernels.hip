@@ -2578,8 +2578,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // force_calculations.rs:50-69); twist = chord angle about the span axis
 // relative to the wing's first strip; then the wing's centre relative to the
 // centre of all strips along and across the mean felt velocity (in 4 mean
-// chords).  scale = c |u| / 2, so that
-// circulation = prediction * scale.
+// chords); then the solver's default initial guess (the previous step's
+// circulation, zero at the first step) of strips i - 4 .. i + 4 over their
+// scales.  scale = c |u| / 2, so that circulation = prediction * scale.
 constexpr int WS_FEATURES = 64;
 constexpr int WS_BASE = 5;
 
@@ -2652,7 +2653,14 @@ __device__ inline void features_body(const Inst& d, float* X, float* scale) {
         V3 across = normalize(cross(load3(geo + d.gl.dir_g + 3 * i), along));
         x[47] = (float)(dot(off, along) / cm / 4.0);
         x[48] = (float)(dot(off, across) / cm / 4.0);
-        for (int q = 49; q < WS_FEATURES; ++q) x[q] = 0.0f;
+        // the solver's default initial guess (the previous step's circulation; zero at the
+        // first step) of strips i - 4 .. i + 4 of the wing, in units of this step's scale
+        for (int o = -4; o <= 4; ++o) {
+            int j = i + o;
+            double sc = j >= a && j < b ? (double)scale[j] : 0.0;
+            x[53 + o] = sc > 0.0 ? (float)(d.prev_gamma[j] / sc) : 0.0f;
+        }
+        for (int q = 58; q < WS_FEATURES; ++q) x[q] = 0.0f;
     }
 }
 

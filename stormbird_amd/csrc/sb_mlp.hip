@@ -6,11 +6,18 @@
 // lifting_line/simulation.rs:91); the network only supplies that initial
 // guess, and the unchanged fixed-point solve finishes from it.
 //
-// GEMM convention: C[M][N] = A[M][K] * Bt[N][K]^T, both operands bf16 with K
-// contiguous (K a multiple of 16), fp32 accumulation in
-// v_mfma_f32_16x16x16_bf16.  Fragment layout of that instruction (wave64):
-// lane l holds A[l%16][4(l/16)+v] and Bt[l%16][4(l/16)+v] for v = 0..3, and
-// returns D[4(l/16)+v][l%16].
+// GEMMs on v_mfma_f32_16x16x32_bf16 (gfx950): fragment layout (wave64) lane l
+// holds A[l%16][8(l/16)+v] and B[l%16][8(l/16)+v] for v = 0..7 (K = 32) and
+// returns D[4(l/16)+v][l%16], v = 0..3, D = A B^T.  The kernels pass the output's
+// column operand as A and its row operand as B, so that a lane's four results
+// are four adjacent columns of one output row (one 16-byte store).  Kernels
+// staging 32-deep K slices of 128-row operand tiles through double-buffered LDS:
+// - k_gemm_nt: C[M][N] = A[M][K] * Bt[N][K]^T (K contiguous in both), with the
+//   layer's epilogue fused (bias and GELU forward; GELU derivative backward);
+// - k_gemm_tn: C[O][I] = sum over rows r of A[r][O] B[r][I] (the weight
+//   gradient, rows contiguous in neither operand: each tile is transposed on
+//   its way into LDS), split over row slices, with the bias gradient (column
+//   sums of A) taken from the same staged tile.
 //
 // Parameter layout (fp32, flat): for layer l = 0..4, W_l[out][in] then b_l[out],
 // sizes 256x64, 256x256 (x3), 1x256.
@@ -30,7 +37,8 @@
 
 namespace sb {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __host__ inline uint16_t f2bf(float f) {  // round to nearest even
@@ -47,58 +55,338 @@ __device__ __host__ inline float bf2f(uint16_t h) {
     return f;
 }
 
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// e^x on the hardware's base-2 exponential (relative error ~1e-7; 0 below e^-87)
+__device__ inline float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
+// erf(x) by Abramowitz & Stegun 7.1.26 (Hastings' rational approximation in
+// t = 1 / (1 + p |x|) times e^(-x^2)), |error| <= 1.5e-7: a dozen instructions
+// and no branch, against ~10^2 for a correctly rounded erff; the GELU values it
+// feeds are rounded to bf16 (relative step 2^-8) right after.
+__device__ inline float erf_as(float x) {
+    const float ax = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
+    float p = fmaf(1.061405429f, t, -1.453152027f);
+    p = fmaf(p, t, 1.421413741f);
+    p = fmaf(p, t, -0.284496736f);
+    p = fmaf(p, t, 0.254829592f);
+    const float r = 1.0f - p * t * exp_hw(-ax * ax);
+    return copysignf(r, x);
+}
+
+__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float x) {
-    float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    float pdf = 0.39894228040143268f * expf(-0.5f * x * x);
+    float cdf = 0.5f * (1.0f + erf_as(x * 0.70710678118654752f));
+    float pdf = 0.39894228040143268f * exp_hw(-0.5f * x * x);
     return cdf + x * pdf;
 }
 
-struct GemmArgs {
+__device__ inline f32x4 mfma_bf16_k32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
+
+constexpr int GT = 128;   // output tile edge (both kernels)
+constexpr int GK = 32;    // K depth of one LDS stage
+constexpr int GLD = 40;   // LDS row stride in bf16 (80 B: 16-byte aligned fragment reads)
+
+enum { EPI_PLAIN = 0, EPI_FWD = 1, EPI_BACK = 2 };
+
+struct NtArgs {
     const uint16_t* A;   // [M][K]
     const uint16_t* Bt;  // [N][K]
-    int M, N, K;
-    int k_split;         // K range per z-slice (multiple of 16)
-    float* C;            // fp32 out [M][N] (z-slice s writes C + s*M*N when split)
-    const float* bias;   // [N] or null
-    uint16_t* Hb;        // bf16 of gelu(C + bias) [M][N], or null
+    int M, N, K;         // K a multiple of 32, N a multiple of 16
+    int epi;
+    const float* bias;   // [N] (EPI_FWD)
+    float* Z;            // fp32 [M][N]: the product (EPI_PLAIN) or product + bias (EPI_FWD)
+    uint16_t* Hb;        // bf16 [M][N]: gelu(product + bias) (EPI_FWD), product * gelu'(Zin) (EPI_BACK)
+    const float* Zin;    // [M][N] (EPI_BACK)
 };
 
-// 64x64 tile per workgroup (2x2 waves of 32x32, each 2x2 MFMA 16x16 tiles).
-__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
-    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    const int m0 = blockIdx.y * 64 + (wave / 2) * 32;
-    const int n0 = blockIdx.x * 64 + (wave % 2) * 32;
-    const int kb = blockIdx.z * g.k_split;
-    const int ke = kb + g.k_split < g.K ? kb + g.k_split : g.K;
-    const int r = lane % 16, kq = 4 * (lane / 16);
-    f32x4 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const bf16x4 zero = {0, 0, 0, 0};
-    for (int k = kb; k < ke; k += 16) {
-        bf16x4 a[2], b[2];
+// 128x128 output tile per workgroup of 4 waves (each 64x64: 4x4 MFMA tiles).
+__global__ __launch_bounds__(256) void k_gemm_nt(NtArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][GT][GLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][GT][GLD];
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+    const s16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    s16x8 ra[2], rb[2];
+    // thread t moves 16-byte pieces t and t + 256 of each 128 x 32 operand slice
+    auto load = [&](int k) {
+#pragma unroll
         for (int i = 0; i < 2; ++i) {
-            int row = m0 + 16 * i + r;
-            a[i] = row < g.M ? *reinterpret_cast<const bf16x4*>(g.A + (long)row * g.K + k + kq) : zero;
-            int col = n0 + 16 * i + r;
-            b[i] = col < g.N ? *reinterpret_cast<const bf16x4*>(g.Bt + (long)col * g.K + k + kq) : zero;
+            const int seg = t + 256 * i, row = seg / 4, part = seg % 4;
+            ra[i] = m0 + row < g.M ? *reinterpret_cast<const s16x8*>(g.A + (long)(m0 + row) * g.K + k + 8 * part)
+                                   : zero;
+            rb[i] = n0 + row < g.N ? *reinterpret_cast<const s16x8*>(g.Bt + (long)(n0 + row) * g.K + k + 8 * part)
+                                   : zero;
         }
-        for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int seg = t + 256 * i, row = seg / 4, part = seg % 4;
+            *reinterpret_cast<s16x8*>(&As[buf][row][8 * part]) = ra[i];
+            *reinterpret_cast<s16x8*>(&Bs[buf][row][8 * part]) = rb[i];
+        }
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int nk = g.K / GK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int s = 0; s < nk; ++s) {
+        const int buf = s & 1;
+        if (s + 1 < nk) load((s + 1) * GK);  // in flight while this stage computes
+        s16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const s16x8*>(&As[buf][wm + 16 * i + lane % 16][8 * (lane / 16)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const s16x8*>(&Bs[buf][wn + 16 * j + lane % 16][8 * (lane / 16)]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = mfma_bf16_k32(b[j], a[i], acc[i][j]);
+        if (s + 1 < nk) store(buf ^ 1);  // that half was last read before the previous barrier
+        __syncthreads();
     }
-    float* C = g.C + (long)blockIdx.z * g.M * g.N;
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j)
-            for (int v = 0; v < 4; ++v) {
-                int row = m0 + 16 * i + kq + v;
-                int col = n0 + 16 * j + r;
-                if (row >= g.M || col >= g.N) continue;
-                float z = acc[i][j][v];
-                if (g.bias) z += g.bias[col];
-                C[(long)row * g.N + col] = z;
-                if (g.Hb) g.Hb[(long)row * g.N + col] = f2bf(gelu(z));
+    // lane: row m0 + wm + 16 i + lane % 16, columns n0 + wn + 16 j + 4 (lane / 16) + (0..3)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int row = m0 + wm + 16 * i + lane % 16;
+            const int col = n0 + wn + 16 * j + 4 * (lane / 16);
+            if (row >= g.M || col >= g.N) continue;  // N is a multiple of 16: all four columns or none
+            const long q = (long)row * g.N + col;
+            f32x4 z = acc[i][j];
+            if (g.epi == EPI_BACK) {
+                const f32x4 zi = *reinterpret_cast<const f32x4*>(g.Zin + q);
+                uint16_t h[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) h[v] = f2bf(z[v] * gelu_grad(zi[v]));
+                *reinterpret_cast<uint2*>(g.Hb + q) = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+            } else {
+                if (g.epi == EPI_FWD) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) z[v] += g.bias[col + v];
+                }
+                *reinterpret_cast<f32x4*>(g.Z + q) = z;
+                if (g.epi == EPI_FWD) {
+                    uint16_t h[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) h[v] = f2bf(gelu(z[v]));
+                    *reinterpret_cast<uint2*>(g.Hb + q) =
+                        make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+                }
             }
+        }
+}
+
+struct TnArgs {
+    const uint16_t* A;    // [M][O]
+    const uint16_t* B;    // [M][I]
+    int M, O, I;          // I a multiple of 16
+    int rows_per_slice;   // a multiple of 32; slice z covers rows [z rows_per_slice, ...)
+    float* part;          // [slices][O][I]
+    float* part_bias;     // [slices][O]: column sums of A over the slice (tiles of column 0)
+};
+
+// 128x128 tile of C per workgroup and row slice; each 32-row stage of A and B is
+// loaded row-coalesced (lane = column) and stored transposed, rows contiguous.
+__global__ __launch_bounds__(256) void k_gemm_tn(TnArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][GT][GLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][GT][GLD];
+    __shared__ float red[256];
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64;
+    const int o0 = blockIdx.y * GT, i0 = blockIdx.x * GT;
+    const long r_begin = (long)blockIdx.z * g.rows_per_slice;
+    const long r_end = r_begin + g.rows_per_slice < g.M ? r_begin + g.rows_per_slice : g.M;
+    const int wm = (wave / 2) * 64, wn = (wave % 2) * 64;
+    const int c = t % GT, h = t / GT;  // this thread's column and half (rows 16h .. 16h + 15 of a stage)
+    const bool oc = o0 + c < g.O, ic = i0 + c < g.I;
+    const bool sums = blockIdx.x == 0;
+    s16x8 va[2], vb[2];
+    float bsum = 0.f;
+    auto load = [&](long r) {
+        const long r1 = r + 16 * h;
+        const int rows = (int)(r_end - r1 < 16 ? (r_end - r1 > 0 ? r_end - r1 : 0) : 16);
+        const uint16_t* pa = g.A + r1 * g.O + o0 + c;
+        const uint16_t* pb = g.B + r1 * g.I + i0 + c;
+        if (rows == 16 && oc && ic) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                va[q / 8][q % 8] = (short)*pa;
+                vb[q / 8][q % 8] = (short)*pb;
+                pa += g.O;
+                pb += g.I;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) {
+                va[q / 8][q % 8] = (short)(q < rows && oc ? *pa : 0);
+                vb[q / 8][q % 8] = (short)(q < rows && ic ? *pb : 0);
+                pa += g.O;
+                pb += g.I;
+            }
+        }
+    };
+    auto store = [&](int buf) {
+        *reinterpret_cast<s16x8*>(&As[buf][c][16 * h]) = va[0];
+        *reinterpret_cast<s16x8*>(&As[buf][c][16 * h + 8]) = va[1];
+        *reinterpret_cast<s16x8*>(&Bs[buf][c][16 * h]) = vb[0];
+        *reinterpret_cast<s16x8*>(&Bs[buf][c][16 * h + 8]) = vb[1];
+        if (sums) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) bsum += bf2f((uint16_t)va[q / 8][q % 8]);
+        }
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const long nk = r_end > r_begin ? (r_end - r_begin + GK - 1) / GK : 0;
+    if (nk > 0) {
+        load(r_begin);
+        store(0);
+    }
+    __syncthreads();
+    for (long s = 0; s < nk; ++s) {
+        const int buf = (int)(s & 1);
+        if (s + 1 < nk) load(r_begin + (s + 1) * GK);
+        s16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = *reinterpret_cast<const s16x8*>(&As[buf][wm + 16 * i + lane % 16][8 * (lane / 16)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = *reinterpret_cast<const s16x8*>(&Bs[buf][wn + 16 * j + lane % 16][8 * (lane / 16)]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = mfma_bf16_k32(b[j], a[i], acc[i][j]);
+        if (s + 1 < nk) store(buf ^ 1);
+        __syncthreads();
+    }
+    float* C = g.part + (long)blockIdx.z * g.O * g.I;
+    // lane: row o0 + wm + 16 i + lane % 16 of C, columns i0 + wn + 16 j + 4 (lane / 16) + (0..3)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int o = o0 + wm + 16 * i + lane % 16;
+            const int col = i0 + wn + 16 * j + 4 * (lane / 16);
+            if (o < g.O && col < g.I) *reinterpret_cast<f32x4*>(C + (long)o * g.I + col) = acc[i][j];
+        }
+    if (sums) {
+        red[t] = bsum;
+        __syncthreads();
+        if (t < GT && oc) g.part_bias[(long)blockIdx.z * g.O + o0 + t] = red[t] + red[t + GT];
+    }
+}
+
+// Whole-network inference, one kernel: 128 rows per workgroup, each wave owning
+// 32 rows whose activations stay in LDS from layer to layer (bf16, as the
+// layer-by-layer forward rounds them), the layer's weights streamed through LDS
+// in 32-deep K slices shared by the four waves.  Every product and epilogue is
+// the one k_gemm_nt and k_head compute (same instruction, same K order), so the
+// outputs equal the layer-by-layer forward bit for bit.
+constexpr int IR = 128;   // rows per workgroup
+constexpr int ILD = 264;  // activation row stride in bf16 (528 B)
+constexpr int IBK = 64;   // K depth of one weight slice
+constexpr int IWLD = 72;  // weight-slice row stride in bf16 (144 B)
+
+struct InferArgs {
+    const float* X;          // [n][64]
+    long n;
+    const uint16_t* W[4];    // bf16 [256][in_l]
+    const float* b[4];       // [256]
+    const float* w4;         // head weights [256] (fp32; rounded to bf16 as k_head does)
+    const float* b4;         // [1]
+    float* y;                // [n]
+};
+
+__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t H[IR][ILD];
+    __shared__ __attribute__((aligned(16))) uint16_t Ws[2][256][IWLD];
+    __shared__ float w4s[256];  // head weights as k_head rounds them
+    const int t = threadIdx.x, wave = t / 64, lane = t % 64;
+    const long r0 = (long)blockIdx.x * IR;
+    w4s[t] = bf2f(f2bf(g.w4[t]));
+    for (int q = t; q < IR * MLP_FEATURES; q += 256) {
+        const int row = q / MLP_FEATURES, c = q % MLP_FEATURES;
+        H[row][c] = r0 + row < g.n ? f2bf(g.X[(r0 + row) * MLP_FEATURES + c]) : (uint16_t)0;
+    }
+    const int wr = wave * 32;
+    s16x8 rw[8];
+    for (int l = 0; l < 4; ++l) {
+        const int K = l == 0 ? MLP_FEATURES : 256;
+        const int nk = K / IBK;
+        const uint16_t* W = g.W[l];
+        // thread t moves 16-byte pieces t + 256 i (i < 8) of each 256 x 64 weight slice
+        auto load_w = [&](int k) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int p = t + 256 * i, row = p / 8, part = p % 8;
+                rw[i] = *reinterpret_cast<const s16x8*>(W + (long)row * K + k + 8 * part);
+            }
+        };
+        auto store_w = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int p = t + 256 * i, row = p / 8, part = p % 8;
+                *reinterpret_cast<s16x8*>(&Ws[buf][row][8 * part]) = rw[i];
+            }
+        };
+        load_w(0);
+        store_w(0);
+        __syncthreads();
+        f32x4 acc[2][16];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int s = 0; s < nk; ++s) {
+            const int buf = s & 1;
+            if (s + 1 < nk) load_w((s + 1) * IBK);
+#pragma unroll
+            for (int kk = 0; kk < IBK / GK; ++kk) {
+                const int kc = IBK * s + GK * kk + 8 * (lane / 16);
+                s16x8 a0 = *reinterpret_cast<const s16x8*>(&H[wr + lane % 16][kc]);
+                s16x8 a1 = *reinterpret_cast<const s16x8*>(&H[wr + 16 + lane % 16][kc]);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    s16x8 b = *reinterpret_cast<const s16x8*>(&Ws[buf][16 * j + lane % 16][GK * kk + 8 * (lane / 16)]);
+                    acc[0][j] = mfma_bf16_k32(b, a0, acc[0][j]);
+                    acc[1][j] = mfma_bf16_k32(b, a1, acc[1][j]);
+                }
+            }
+            if (s + 1 < nk) store_w(buf ^ 1);
+            __syncthreads();
+        }
+        // this wave's rows: bias and GELU, back into the activations (read only by this wave)
+        const float* bias = g.b[l];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const int row = wr + 16 * i + lane % 16, col = 16 * j + 4 * (lane / 16);
+                uint16_t h[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) h[v] = f2bf(gelu(acc[i][j][v] + bias[col + v]));
+                *reinterpret_cast<uint2*>(&H[row][col]) = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+            }
+    }
+    __syncthreads();
+    if (t < IR && r0 + t < g.n) {
+        float s = 0.f;
+        for (int k = 0; k < 256; ++k) s += bf2f(H[t][k]) * w4s[k];
+        s += g.b4[0];
+        g.y[r0 + t] = s;
+    }
 }
 
 // out[i] = sum over slices s of part[s][i] (ascending s), i < n
@@ -106,29 +394,21 @@ __global__ void k_sum_slices(const float* part, int slices, long n, float* out) 
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = 0.f;
-    for (int q = 0; q < slices; ++q) s += part[(long)q * n + i];
+    int q = 0;
+    for (; q + 8 <= slices; q += 8) {  // eight loads in flight, added in slice order
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[(long)(q + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    for (; q < slices; ++q) s += part[(long)q * n + i];
     out[i] = s;
 }
 
 __global__ void k_to_bf16(const float* x, long n, uint16_t* y) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = f2bf(x[i]);
-}
-
-// bf16 [rows][cols] -> [cols][ld] (rows <= ld; columns rows..ld-1 zero)
-__global__ void k_transpose_bf16(const uint16_t* x, int rows, int cols, int ld, uint16_t* y) {
-    __shared__ uint16_t t[32][33];
-    int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 256 threads: 32 x 8
-    for (int k = ty; k < 32; k += 8) {
-        int rr = r0 + k, cc = c0 + tx;
-        t[k][tx] = (rr < rows && cc < cols) ? x[(long)rr * cols + cc] : (uint16_t)0;
-    }
-    __syncthreads();
-    for (int k = ty; k < 32; k += 8) {
-        int cc = c0 + k, rr = r0 + tx;
-        if (cc < cols && rr < ld) y[(long)cc * ld + rr] = t[tx][k];
-    }
 }
 
 // Output layer y = Hb3 . w4 + b4 and its loss gradient dy = 2 (y - t) / n.
@@ -156,23 +436,6 @@ __global__ void k_head_back(const float* dy, const float* w, const float* Z, lon
     int k = (int)(q % 256);
     float g = dy[i] * bf2f(f2bf(w[k]));
     dZb[q] = f2bf(g * gelu_grad(Z[q]));
-}
-
-// dZ = dH * gelu'(Z) -> bf16
-__global__ void k_gelu_back(const float* dH, const float* Z, long n, uint16_t* dZb) {
-    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) dZb[q] = f2bf(dH[q] * gelu_grad(Z[q]));
-}
-
-// Column sums of a bf16 [rows][cols] matrix (bias gradients), two passes over
-// fixed row chunks so the order is deterministic.
-__global__ void k_colsum_bf16(const uint16_t* x, long rows, int cols, int chunk, float* part) {
-    int c = threadIdx.x + blockIdx.y * blockDim.x;
-    if (c >= cols) return;
-    long r0 = (long)blockIdx.x * chunk, r1 = r0 + chunk < rows ? r0 + chunk : rows;
-    float s = 0.f;
-    for (long r = r0; r < r1; ++r) s += bf2f(x[r * cols + c]);
-    part[(long)blockIdx.x * cols + c] = s;
 }
 
 // head gradients: dw4[k] = sum_i dy_i H3[i][k], db4 = sum_i dy_i (chunked partials)
@@ -254,13 +517,13 @@ struct sb_mlp {
     float* Z[4] = {};                // pre-activations [cap][256]
     uint16_t* Hb[4] = {};            // bf16 activations [cap][256]
     float *y = nullptr, *dy = nullptr, *sq = nullptr, *loss = nullptr;
-    uint16_t *dZb = nullptr, *T0 = nullptr, *T1 = nullptr;  // dZ (bf16), transposes
-    float *dH = nullptr, *part = nullptr;
+    uint16_t *dZb = nullptr, *dZb2 = nullptr;  // back-propagated dZ (bf16), two layers in turn
+    float* part = nullptr;
     long part_cap = 0;
 
     ~sb_mlp() {
         (void)hipSetDevice(device);
-        void* bufs[] = {p, g, m, v, Xb, y, dy, sq, loss, dZb, T0, T1, dH, part};
+        void* bufs[] = {p, g, m, v, Xb, y, dy, sq, loss, dZb, dZb2, part};
         for (void* q : bufs)
             if (q) (void)hipFree(q);
         for (int l = 0; l < kLayers; ++l) {
@@ -321,7 +584,7 @@ struct sb_mlp {
     void ensure(long n) {
         if (n <= cap) return;
         long c = ((n + 1023) / 1024) * 1024;
-        void* bufs[] = {Xb, y, dy, sq, dZb, T0, T1, dH};
+        void* bufs[] = {Xb, y, dy, sq, dZb, dZb2};
         for (void* q : bufs)
             if (q) (void)hipFree(q);
         for (int l = 0; l < 4; ++l) {
@@ -338,9 +601,7 @@ struct sb_mlp {
         sq = dalloc<float>(c);
         if (!loss) loss = dalloc<float>(1);
         dZb = dalloc<uint16_t>(c * 256);
-        T0 = dalloc<uint16_t>(c * 256);
-        T1 = dalloc<uint16_t>(c * 256);
-        dH = dalloc<float>(c * 256);
+        dZb2 = dalloc<uint16_t>(c * 256);
         cap = c;
     }
 
@@ -351,32 +612,54 @@ struct sb_mlp {
         part_cap = n;
     }
 
-    void gemm(const uint16_t* A, const uint16_t* Bt, int M, int N, int K, float* C, const float* bias, uint16_t* Hbo,
-              int slices) {
-        GemmArgs a{A, Bt, M, N, K, K, C, bias, Hbo};
-        if (slices > 1) {
-            int ks = ((K / slices + 15) / 16) * 16;
-            slices = (K + ks - 1) / ks;
-            a.k_split = ks;
-            ensure_part((long)slices * M * N);
-            a.C = part;
-            a.bias = nullptr;
-            a.Hb = nullptr;
-        }
-        hipLaunchKernelGGL(k_gemm_bf16, dim3(nblk(N, 64), nblk(M, 64), slices), dim3(256), 0, stream, a);
-        if (slices > 1)
-            hipLaunchKernelGGL(k_sum_slices, dim3(nblk((long)M * N, 256)), dim3(256), 0, stream, part, slices,
-                               (long)M * N, C);
+    // C[M][N] = A[M][K] Bt[N][K]^T with the epilogue `epi` (k_gemm_nt)
+    void gemm_nt(const uint16_t* A, const uint16_t* Bt, int M, int N, int K, int epi, const float* bias, float* Zo,
+                 uint16_t* Hbo, const float* Zin) {
+        NtArgs a{A, Bt, M, N, K, epi, bias, Zo, Hbo, Zin};
+        hipLaunchKernelGGL(k_gemm_nt, dim3(nblk(N, GT), nblk(M, GT)), dim3(256), 0, stream, a);
         check(hipGetLastError(), "gemm");
     }
 
-    // forward for n rows of fp32 features (device); y (device) gets the outputs
+    // W-gradient C[O][I] = A[M][O]^T B[M][I] and the bias gradient (column sums of A),
+    // over at most 128 row slices of 512 or more rows, summed in slice order (k_gemm_tn)
+    void gemm_tn(const uint16_t* A, const uint16_t* B, long M, int O, int I, float* C, float* bias_grad) {
+        const int slices = (int)std::min<long>(128, std::max<long>(1, (M + 511) / 512));
+        const int rows = (int)(((M + slices - 1) / slices + GK - 1) / GK * GK);
+        ensure_part((long)slices * O * I + (long)slices * O);
+        TnArgs a{A, B, (int)M, O, I, rows, part, part + (long)slices * O * I};
+        hipLaunchKernelGGL(k_gemm_tn, dim3(nblk(I, GT), nblk(O, GT), slices), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(k_sum_slices, dim3(nblk((long)O * I, 256)), dim3(256), 0, stream, part, slices,
+                           (long)O * I, C);
+        hipLaunchKernelGGL(k_sum_slices, dim3(nblk(O, 256)), dim3(256), 0, stream, a.part_bias, slices, (long)O,
+                           bias_grad);
+        check(hipGetLastError(), "gemm");
+    }
+
+    // inference: y (device) for n rows of fp32 features (device), one kernel
+    void infer(const float* X, long n, float* out) {
+        InferArgs a{};
+        a.X = X;
+        a.n = n;
+        for (int l = 0; l < 4; ++l) {
+            a.W[l] = Wb[l];
+            a.b[l] = p + param_offset(l) + (long)kOut[l] * kIn[l];
+        }
+        a.w4 = p + param_offset(4);
+        a.b4 = a.w4 + 256;
+        a.y = out;
+        hipLaunchKernelGGL(k_mlp_infer, dim3(nblk(n, IR)), dim3(256), 0, stream, a);
+        check(hipGetLastError(), "inference");
+    }
+
+    // forward for n rows of fp32 features (device), keeping what the backward pass needs;
+    // y (device) gets the outputs
     void forward(const float* X, long n, const float* target) {
         ensure(n);
         hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n * MLP_FEATURES, 256)), dim3(256), 0, stream, X, n * MLP_FEATURES, Xb);
         const uint16_t* in = Xb;
         for (int l = 0; l < 4; ++l) {
-            gemm(in, Wb[l], (int)n, 256, kIn[l], Z[l], p + param_offset(l) + (long)kOut[l] * kIn[l], Hb[l], 1);
+            gemm_nt(in, Wb[l], (int)n, 256, kIn[l], EPI_FWD, p + param_offset(l) + (long)kOut[l] * kIn[l], Z[l], Hb[l],
+                    nullptr);
             in = Hb[l];
         }
         const float* w4 = p + param_offset(4);
@@ -389,9 +672,8 @@ struct sb_mlp {
     float gradients(const float* X, const float* target, long n) {
         forward(X, n, target);
         hipLaunchKernelGGL(k_reduce_sq, dim3(1), dim3(256), 0, stream, sq, n, loss);
-        const int chunk = 1024;
+        const int chunk = 128;
         long nch = (n + chunk - 1) / chunk;
-        const long n16 = ((n + 15) / 16) * 16;
         // head
         ensure_part(nch * 257);
         hipLaunchKernelGGL(k_head_grad, dim3((unsigned)nch), dim3(257), 0, stream, dy, Hb[3], n, chunk, part);
@@ -399,27 +681,16 @@ struct sb_mlp {
                            g + param_offset(4));
         const float* w4 = p + param_offset(4);
         hipLaunchKernelGGL(k_head_back, dim3(nblk(n * 256, 256)), dim3(256), 0, stream, dy, w4, Z[3], n, dZb);
+        uint16_t *dz = dZb, *dz_next = dZb2;
         for (int l = 3; l >= 0; --l) {
             const int out = kOut[l], in = kIn[l];
             const uint16_t* Hin = l > 0 ? Hb[l - 1] : Xb;
-            // dW_l = dZ_l^T . H_{l-1}: A = dZ^T [out][n16], Bt = H^T [in][n16]
-            hipLaunchKernelGGL(k_transpose_bf16, dim3(nblk(out, 32), nblk(n16, 32)), dim3(256), 0, stream, dZb, (int)n,
-                               out, (int)n16, T0);
-            hipLaunchKernelGGL(k_transpose_bf16, dim3(nblk(in, 32), nblk(n16, 32)), dim3(256), 0, stream, Hin, (int)n,
-                               in, (int)n16, T1);
-            int slices = (int)std::min<long>(64, std::max<long>(1, n16 / 1024));
-            gemm(T0, T1, out, in, (int)n16, g + param_offset(l), nullptr, nullptr, slices);
-            // db_l = column sums of dZ_l
-            ensure_part(nch * out);
-            hipLaunchKernelGGL(k_colsum_bf16, dim3((unsigned)nch, nblk(out, 256)), dim3(256), 0, stream, dZb, n, out,
-                               chunk, part);
-            hipLaunchKernelGGL(k_sum_slices, dim3(nblk(out, 256)), dim3(256), 0, stream, part, (int)nch, (long)out,
-                               g + param_offset(l) + (long)out * in);
+            // dW_l = dZ_l^T H_{l-1} and db_l = column sums of dZ_l
+            gemm_tn(dz, Hin, n, out, in, g + param_offset(l), g + param_offset(l) + (long)out * in);
             if (l > 0) {
-                // dH_{l-1} = dZ_l . W_l: A = dZ [n][out], Bt = W^T [in][out]
-                gemm(dZb, WTb[l], (int)n, in, out, dH, nullptr, nullptr, 1);
-                hipLaunchKernelGGL(k_gelu_back, dim3(nblk(n * in, 256)), dim3(256), 0, stream, dH, Z[l - 1], n * in,
-                                   dZb);
+                // dZ_{l-1} = (dZ_l W_l) * gelu'(Z_{l-1}): A = dZ [n][out], Bt = W^T [in][out]
+                gemm_nt(dz, WTb[l], (int)n, in, out, EPI_BACK, nullptr, nullptr, dz_next, Z[l - 1]);
+                std::swap(dz, dz_next);
             }
         }
         check(hipGetLastError(), "backward");
@@ -449,15 +720,13 @@ hipError_t mlp_forward_on(sb_mlp* m, const float* X, long n, float* y, hipStream
     m->stream = st;
     hipError_t e = hipSuccess;
     try {
-        m->forward(X, n, nullptr);
-        e = hipMemcpyAsync(y, m->y, sizeof(float) * n, hipMemcpyDeviceToDevice, st);
+        m->infer(X, n, y);
     } catch (const HipError& err) {
         e = (hipError_t)err.code;
     }
     m->stream = keep;
     return e;
 }
-void mlp_reserve(sb_mlp* m, long n) { m->ensure(n); }
 }  // namespace sb
 
 extern "C" {
@@ -493,8 +762,7 @@ int sb_mlp_forward(sb_mlp* m, const float* x_dev, long n, float* y_dev) {
     return guarded([&] {
         check(hipSetDevice(m->device), "hipSetDevice");
         wait_for_null_stream(m->stream, m->ev_input);
-        m->forward(x_dev, n, nullptr);
-        check(hipMemcpyAsync(y_dev, m->y, sizeof(float) * n, hipMemcpyDeviceToDevice, m->stream), "copy");
+        m->infer(x_dev, n, y_dev);
         check(hipStreamSynchronize(m->stream), "forward");
     });
 }

@@ -1,7 +1,8 @@
 """CPU restatement (numpy, float64) of the warm-start network for checking the GPU kernels.
 
 Same architecture and parameter layout as sb_mlp (64 -> 256 -> 256 -> 256 -> 256 -> 1,
-GELU with erf), and the same rounding points: the GEMM operands are rounded to bf16
+GELU with erf; the GPU evaluates erf by Abramowitz & Stegun 7.1.26, |error| <= 1.5e-7,
+far below the bf16 rounding that follows), and the same rounding points: the GEMM operands are rounded to bf16
 (inputs, weights, activations, back-propagated gradients) and everything else is
 computed here in float64.  Test infrastructure only.
 """

@@ -164,3 +164,31 @@ def test_first_step_does_not_depend_on_wake_length():
     for i in range(6):
         compare_results(rs[i], rl[i], exact=True, where=f"instance {i}")
     assert np.array_equal(xl, xs) and np.array_equal(sl, ss)
+
+
+def test_fused_inference_equals_layer_by_layer_forward():
+    """Inference runs the whole network in one kernel; training runs it layer by layer,
+    keeping what the backward pass needs.  Both use the same instruction in the same K
+    order and the same rounding points: the training loss (k_head's squared errors summed
+    by k_reduce_sq: 256 strided partial sums, then those in order) equals the same sum
+    restated in float32 over the fused outputs, bit for bit."""
+    rng = np.random.default_rng(11)
+    n = 1000
+    X = rng.standard_normal((n, 64)).astype(np.float32)
+    t = rng.standard_normal(n).astype(np.float32)
+    m = sb.Mlp(seed=3)
+    xd, td = dev(X), dev(t)
+    yd = torch.zeros(n, dtype=torch.float32, device="cuda")
+    m.forward(xd.data_ptr(), n, yd.data_ptr())
+    torch.cuda.synchronize()
+    y = yd.cpu().numpy()
+    _, loss = m.gradients(xd.data_ptr(), td.data_ptr(), n)
+    e = (y - t).astype(np.float32)
+    sq = (e * e).astype(np.float32)
+    total = np.float32(0.0)
+    for lane in range(256):
+        s = np.float32(0.0)
+        for i in range(lane, n, 256):
+            s = np.float32(s + sq[i])
+        total = np.float32(total + s)
+    assert np.float32(total / np.float32(n)) == np.float32(loss)
