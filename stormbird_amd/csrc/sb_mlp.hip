@@ -41,12 +41,14 @@ typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __host__ inline uint16_t f2bf(float f) {  // round to nearest even
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+// fp32 -> bf16, round to nearest even (v_cvt_pk_bf16_f32)
+__device__ inline uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+// two of them in one 32-bit word, a in the low half
+__device__ inline uint32_t f2bf2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2));
 }
 __device__ __host__ inline float bf2f(uint16_t h) {
     uint32_t u = (uint32_t)h << 16;
@@ -58,26 +60,31 @@ __device__ __host__ inline float bf2f(uint16_t h) {
 // e^x on the hardware's base-2 exponential (relative error ~1e-7; 0 below e^-87)
 __device__ inline float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
-// erf(x) by Abramowitz & Stegun 7.1.26 (Hastings' rational approximation in
-// t = 1 / (1 + p |x|) times e^(-x^2)), |error| <= 1.5e-7: a dozen instructions
-// and no branch, against ~10^2 for a correctly rounded erff; the GELU values it
-// feeds are rounded to bf16 (relative step 2^-8) right after.
-__device__ inline float erf_as(float x) {
-    const float ax = fabsf(x);
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, ax, 1.0f));
-    float p = fmaf(1.061405429f, t, -1.453152027f);
-    p = fmaf(p, t, 1.421413741f);
-    p = fmaf(p, t, -0.284496736f);
-    p = fmaf(p, t, 0.254829592f);
-    const float r = 1.0f - p * t * exp_hw(-ax * ax);
-    return copysignf(r, x);
+// GELU(x) = x Phi(x) with the normal tail q = (1 - Phi(|x|)) = erfc(|x| / sqrt 2) / 2 by
+// Abramowitz & Stegun 7.1.26 (Hastings' rational approximation): q = P(t) e^(-x^2/2) / 2,
+// t = 1 / (1 + p |x| / sqrt 2), P(t) = a1 t + ... + a5 t^5, |error in erf| <= 1.5e-7 --
+// a dozen instructions and no branch, against ~10^2 for a correctly rounded erff; the
+// values it feeds are rounded to bf16 (relative step 2^-8) right after.
+struct NormalTail {
+    float q;  // 1 - Phi(|x|)
+    float e;  // e^(-x^2/2)
+};
+__device__ inline NormalTail normal_tail(float x) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(x), 1.0f));
+    float p = fmaf(0.5f * 1.061405429f, t, 0.5f * -1.453152027f);
+    p = fmaf(p, t, 0.5f * 1.421413741f);
+    p = fmaf(p, t, 0.5f * -0.284496736f);
+    p = fmaf(p, t, 0.5f * 0.254829592f);
+    const float e = exp_hw(-0.5f * x * x);
+    return {p * t * e, e};
 }
-
-__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f)); }
-__device__ inline float gelu_grad(float x) {
-    float cdf = 0.5f * (1.0f + erf_as(x * 0.70710678118654752f));
-    float pdf = 0.39894228040143268f * exp_hw(-0.5f * x * x);
-    return cdf + x * pdf;
+__device__ inline float gelu(float x) {
+    const NormalTail n = normal_tail(x);
+    return x * (x >= 0.0f ? 1.0f - n.q : n.q);
+}
+__device__ inline float gelu_grad(float x) {  // Phi(x) + x phi(x)
+    const NormalTail n = normal_tail(x);
+    return (x >= 0.0f ? 1.0f - n.q : n.q) + x * (0.39894228040143268f * n.e);
 }
 
 __device__ inline f32x4 mfma_bf16_k32(s16x8 a, s16x8 b, f32x4 c) {
@@ -97,7 +104,7 @@ struct NtArgs {
     int M, N, K;         // K a multiple of 32, N a multiple of 16
     int epi;
     const float* bias;   // [N] (EPI_FWD)
-    float* Z;            // fp32 [M][N]: the product (EPI_PLAIN) or product + bias (EPI_FWD)
+    float* Z;            // fp32 [M][N]: the product (EPI_PLAIN) or product + bias (EPI_FWD; null: not kept)
     uint16_t* Hb;        // bf16 [M][N]: gelu(product + bias) (EPI_FWD), product * gelu'(Zin) (EPI_BACK)
     const float* Zin;    // [M][N] (EPI_BACK)
 };
@@ -166,23 +173,17 @@ __global__ __launch_bounds__(256) void k_gemm_nt(NtArgs g) {
             f32x4 z = acc[i][j];
             if (g.epi == EPI_BACK) {
                 const f32x4 zi = *reinterpret_cast<const f32x4*>(g.Zin + q);
-                uint16_t h[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) h[v] = f2bf(z[v] * gelu_grad(zi[v]));
-                *reinterpret_cast<uint2*>(g.Hb + q) = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+                *reinterpret_cast<uint2*>(g.Hb + q) = make_uint2(f2bf2(z[0] * gelu_grad(zi[0]), z[1] * gelu_grad(zi[1])),
+                                                                 f2bf2(z[2] * gelu_grad(zi[2]), z[3] * gelu_grad(zi[3])));
             } else {
                 if (g.epi == EPI_FWD) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) z[v] += g.bias[col + v];
                 }
-                *reinterpret_cast<f32x4*>(g.Z + q) = z;
-                if (g.epi == EPI_FWD) {
-                    uint16_t h[4];
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) h[v] = f2bf(gelu(z[v]));
+                if (g.Z) *reinterpret_cast<f32x4*>(g.Z + q) = z;
+                if (g.epi == EPI_FWD)
                     *reinterpret_cast<uint2*>(g.Hb + q) =
-                        make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
-                }
+                        make_uint2(f2bf2(gelu(z[0]), gelu(z[1])), f2bf2(gelu(z[2]), gelu(z[3])));
             }
         }
 }
@@ -316,9 +317,22 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
     const int t = threadIdx.x, wave = t / 64, lane = t % 64;
     const long r0 = (long)blockIdx.x * IR;
     w4s[t] = bf2f(f2bf(g.w4[t]));
-    for (int q = t; q < IR * MLP_FEATURES; q += 256) {
-        const int row = q / MLP_FEATURES, c = q % MLP_FEATURES;
-        H[row][c] = r0 + row < g.n ? f2bf(g.X[(r0 + row) * MLP_FEATURES + c]) : (uint16_t)0;
+    {
+        // the features: 128 rows x 64 fp32 = 2048 pieces of 4 values, 8 per thread, all
+        // loads in flight before the first store
+        constexpr int PER = IR * MLP_FEATURES / 4 / 256;
+        float4 f[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int q = t + 256 * i, row = q / (MLP_FEATURES / 4), c4 = q % (MLP_FEATURES / 4);
+            f[i] = r0 + row < g.n ? *reinterpret_cast<const float4*>(g.X + (r0 + row) * MLP_FEATURES + 4 * c4)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int q = t + 256 * i, row = q / (MLP_FEATURES / 4), c4 = q % (MLP_FEATURES / 4);
+            *reinterpret_cast<uint2*>(&H[row][4 * c4]) = make_uint2(f2bf2(f[i].x, f[i].y), f2bf2(f[i].z, f[i].w));
+        }
     }
     const int wr = wave * 32;
     s16x8 rw[8];
@@ -374,10 +388,10 @@ __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const int row = wr + 16 * i + lane % 16, col = 16 * j + 4 * (lane / 16);
-                uint16_t h[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) h[v] = f2bf(gelu(acc[i][j][v] + bias[col + v]));
-                *reinterpret_cast<uint2*>(&H[row][col]) = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+                const f32x4 z = acc[i][j];
+                *reinterpret_cast<uint2*>(&H[row][col]) =
+                    make_uint2(f2bf2(gelu(z[0] + bias[col]), gelu(z[1] + bias[col + 1])),
+                               f2bf2(gelu(z[2] + bias[col + 2]), gelu(z[3] + bias[col + 3])));
             }
     }
     __syncthreads();
