@@ -494,7 +494,7 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
         "kernel_variants": dict(zip(("fixed_per", "targets_tile"), route.batch.get_tuning())),
     }
     if mlp is not None:
-        route.batch.set_warm_start(mlp, 3)
+        route.batch.set_warm_start(mlp, 3, residual=True)
         res = route.step(want_result=True)
         out["judged_warm_start"] = {"iterations_mean": float(np.mean([r.iterations for r in res])),
                                     "taken_fraction": route.batch.last_warm_taken() / n_ships,
@@ -506,9 +506,9 @@ def route_simulation(n_ships=64, seed=5, steps=1, mlp=None):
 def route_warm_start(n_ships=64, train_seed=11, eval_seed=5, nr_panels=60, warmup=40, record=120, eval_steps=40,
                      train_steps=1500, minibatch=65536):
     """The judged per-step warm start of the route, trained on route steps: the features
-    include the solver's default initial guess (the previous step's circulation), so the
-    network predicts this step's circulation from the previous one and this step's inflow
-    (gusts, roll).  Training data: `record` steps of a route of other wind conditions and
+    include the solver's default initial guess (the previous step's circulation), and the
+    network predicts the change of the circulation from it over this step (gusts, roll), in
+    units of the scale, so that a zero prediction is the default guess.  Training data: `record` steps of a route of other wind conditions and
     turbulence (seed train_seed, 8 conditions) with short wakes (`nr_panels` rows), each
     solved tight (max |Gamma_est - Gamma| < 1e-9 |u| c / 2 or 400 iterations).  Evaluation:
     the route's own conditions (seed eval_seed) with short wakes and the default solver
@@ -532,7 +532,8 @@ def route_warm_start(n_ships=64, train_seed=11, eval_seed=5, nr_panels=60, warmu
         x, sc = tr.batch.get_features()
         g = np.concatenate([r.force_input.circulation_strength for r in res])
         X.append(x)
-        T.append((g / sc).astype(np.float32))
+        # the change from the previous circulation (feature 53: the strip's own, over its scale)
+        T.append((g / sc - x[:, 53]).astype(np.float32))
     X, T = np.concatenate(X), np.concatenate(T)
     del tr
     t_data = time.perf_counter() - t0
@@ -560,8 +561,11 @@ def route_warm_start(n_ships=64, train_seed=11, eval_seed=5, nr_panels=60, warmu
     saved = [(s.get_wake_state(), s.previous_circulation_strength.copy()) for s in ev.batch.instances]
     k0 = ev.k
 
-    def run(warm):
-        ev.batch.set_warm_start(mlp if warm else None, 3 if warm else 0)
+    def run(warm, max_iterations=40):
+        for s in ev.batch.instances:
+            s.set_solver_settings(max_iterations=max_iterations, damping=0.05, residual_tolerance=1e-4,
+                                  strength_tolerance=1e-6)
+        ev.batch.set_warm_start(mlp if warm else None, 3 if warm else 0, residual=True)
         for s, (st, prev) in zip(ev.batch.instances, saved):
             s.set_wake_state(st["points"], st["velocities"], st["strengths"], st["steps_completed"])
             s.previous_circulation_strength = prev
@@ -578,6 +582,10 @@ def route_warm_start(n_ships=64, train_seed=11, eval_seed=5, nr_panels=60, warmu
 
     it_cold, res_cold, _ = run(False)
     it_warm, res_warm, taken = run(True)
+    # the same steps with the solver allowed to converge (the default tolerances, at most 1000
+    # iterations): what the guess saves in iterations
+    itc_cold, _, _ = run(False, 1000)
+    itc_warm, _, taken_c = run(True, 1000)
     return {
         "workload": (f"route steps with short wakes ({nr_panels} rows), {n_ships} ships; trained on {record} steps of "
                      f"another route (seed {train_seed}, 8 wind conditions), evaluated on {eval_steps} steps of the "
@@ -588,6 +596,11 @@ def route_warm_start(n_ships=64, train_seed=11, eval_seed=5, nr_panels=60, warmu
         "iterations_mean_without": float(it_cold.mean()), "iterations_mean_judged": float(it_warm.mean()),
         "residual_mean_without": float(res_cold.mean()), "residual_mean_judged": float(res_warm.mean()),
         "guess_taken_fraction": taken / float(n_ships * eval_steps),
+        "converging_solver": {"max_iterations": 1000, "iterations_mean_without": float(itc_cold.mean()),
+                              "iterations_mean_judged": float(itc_warm.mean()),
+                              "iterations_max_without": float(itc_cold.max()),
+                              "iterations_max_judged": float(itc_warm.max()),
+                              "guess_taken_fraction": taken_c / float(n_ships * eval_steps)},
     }, mlp
 
 

@@ -358,7 +358,9 @@ int sb_mlp_train_step(sb_mlp* m, const float* x, const float* t, long n, float l
 /* mode 0: off; 1: at the first step; 2: at every step; 3: at every step,
  * judged: per instance the solver's first evaluation keeps whichever of the
  * default guess (previous circulation) and the prediction has the smaller
- * max |Gamma_est - Gamma|.  The network must outlive its use by the batch. */
+ * max |Gamma_est - Gamma|.  Plus 4 (modes 5, 6, 7): the network predicts the
+ * change from the previous circulation (guess = previous + prediction x scale)
+ * instead of the circulation.  The network must outlive its use by the batch. */
 int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode);
 /* Record the per-strip features at every step (also when no network is set). */
 int sb_batch_set_record_features(sb_batch* b, int on);

@@ -599,11 +599,13 @@ class Batch:
             _err("sb_batch_autotune", rc)
         return bool(rc == 1), self.get_tuning(), t
 
-    def set_warm_start(self, mlp, mode: int = 1):
+    def set_warm_start(self, mlp, mode: int = 1, residual: bool = False):
         """mode 0 off, 1 first step, 2 every step, 3 every step judged by the solver's first
-        evaluation (keeps a reference to the network)."""
+        evaluation (keeps a reference to the network); residual: the network predicts the
+        change from the previous circulation, in units of the scale."""
         self._mlp = mlp if mode else None
-        rc = lib().sb_batch_set_warm_start(self._h, mlp._h if mlp is not None else None, int(mode))
+        m = int(mode) | (4 if residual and mode else 0)
+        rc = lib().sb_batch_set_warm_start(self._h, mlp._h if mlp is not None else None, m)
         if rc != 0:
             _err("sb_batch_set_warm_start", rc)
 

@@ -97,6 +97,7 @@ struct sb_batch {
     // learned warm start
     sb_mlp* mlp = nullptr;
     int warm_mode = 0;          // 0 off, 1 first step only, 2 every step, 3 every step judged
+                                // (plus 4: the prediction is the change from the previous circulation)
     bool record_features = false;
     long n_strips = 0;
     std::vector<long> h_strip_off;  // [n + 1]
@@ -411,10 +412,11 @@ struct sb_batch {
                   "upload freestream");
         }
         check(hipEventRecord(ev[0], stream), "event");
-        const bool apply = mlp && (warm_mode >= 2 || (warm_mode == 1 && first));
+        const int when = warm_mode & 3;
+        const bool apply = mlp && (when >= 2 || (when == 1 && first));
         const bool feats = apply || record_features;
         last_applied = apply;
-        long sig[3] = {first ? 1 : 0, bd.n_fixed_jobs, bd.n_target_jobs * 4 + (apply ? 2 : 0) + (feats ? 1 : 0)};
+        long sig[3] = {first ? 1 : 0, bd.n_fixed_jobs, (bd.n_target_jobs * 8 + (apply ? warm_mode : 0)) * 2 + (feats ? 1 : 0)};
         if (first || !use_graph) {
             enqueue(first, apply, feats);
         } else {
@@ -661,7 +663,7 @@ struct sb_batch {
         BatchDev b = bd;
         b.inst = d_inst;
         check(launch_batch_prepare(b, first ? 1 : 0, stream), "prepare");
-        const bool judged = apply && warm_mode == 3;
+        const bool judged = apply && (warm_mode & 3) == 3;
         if (feats) check(launch_batch_warm_start(b, mlp, apply ? (judged ? 2 : 1) : 0, stream), "warm start");
         if (first) check(launch_batch_init_wake(b, stream), "init wake");
         check(launch_batch_wake_step(b, stream), "wake step");
@@ -883,7 +885,7 @@ int sb_batch_reset(sb_batch* b) {
 
 int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode) {
     if (!b) return -1;
-    if (mode < 0 || mode > 3 || (mode > 0 && !m)) return -4;
+    if (mode < 0 || mode > 7 || mode == 4 || (mode > 0 && !m)) return -4;
     if (b->quasi && mode > 0) {
         g_error = "the learned warm start runs on dynamic batches";
         return -4;
@@ -891,6 +893,7 @@ int sb_batch_set_warm_start(sb_batch* b, sb_mlp* m, int mode) {
     return guarded([&] {
         b->mlp = mode > 0 ? m : nullptr;
         b->warm_mode = mode;
+        b->bd.warm_residual = (mode & 4) ? 1 : 0;
     });
 }
 
@@ -926,7 +929,7 @@ long sb_batch_last_fallbacks(const sb_batch* b) {
 long sb_batch_last_warm_taken(const sb_batch* b) {
     if (!b) return -1;
     if (!b->last_applied) return 0;
-    return b->warm_mode == 3 ? (long)b->h_fallbacks[1] : (long)b->n - (long)b->h_fallbacks[0];
+    return (b->warm_mode & 3) == 3 ? (long)b->h_fallbacks[1] : (long)b->n - (long)b->h_fallbacks[0];
 }
 
 }  // extern "C"

@@ -219,6 +219,7 @@ struct BatchDev {
     float* feat = nullptr;
     float* scale = nullptr;
     float* pred = nullptr;
+    int warm_residual = 0;            // the prediction is the change from the previous circulation
     unsigned* fallbacks = nullptr;    // device [2]: instances that kept their default guess (invalid
                                       // prediction); instances started from the prediction (judged mode)
 };

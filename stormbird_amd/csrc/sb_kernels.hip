@@ -2673,8 +2673,10 @@ __global__ void kb_features(const Inst* I, const long* strip_off, float* X, floa
 // prev_gamma = prediction * scale for every strip of the instance, unless a
 // prediction is not finite or exceeds 10 in magnitude (|CL| > 10): then the
 // instance keeps its default initial guess and counts as a fallback.
+// residual: the prediction is the change from the default guess (the previous
+// circulation), in units of the scale.
 __global__ void kb_warm_apply(const Inst* I, const long* strip_off, const float* pred, const float* scale,
-                              unsigned* fallbacks) {
+                              unsigned* fallbacks, int residual) {
     __shared__ int bad;
     const Inst d = I[blockIdx.x];
     long o = strip_off[blockIdx.x];
@@ -2689,7 +2691,10 @@ __global__ void kb_warm_apply(const Inst* I, const long* strip_off, const float*
         if (threadIdx.x == 0) __hip_atomic_fetch_add(fallbacks, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    for (int i = threadIdx.x; i < d.N; i += blockDim.x) d.prev_gamma[i] = (double)pred[o + i] * (double)scale[o + i];
+    for (int i = threadIdx.x; i < d.N; i += blockDim.x) {
+        const double g = (double)pred[o + i] * (double)scale[o + i];
+        d.prev_gamma[i] = residual ? d.prev_gamma[i] + g : g;
+    }
 }
 
 // Judged warm start: the solver's own first evaluation decides between the
@@ -2699,7 +2704,8 @@ __global__ void kb_warm_apply(const Inst* I, const long* strip_off, const float*
 // the fixed velocities and factors, before the solve; `taken` counts the
 // instances that start from the network's guess.
 __global__ __launch_bounds__(1024) void kb_warm_judge(const Inst* I, const long* strip_off, const float* pred,
-                                                      const float* scale, unsigned* taken, unsigned* fallbacks) {
+                                                      const float* scale, unsigned* taken, unsigned* fallbacks,
+                                                      int residual) {
     extern __shared__ double sh[];
     const Inst d = I[blockIdx.x];
     const int N = d.N;
@@ -2720,7 +2726,8 @@ __global__ __launch_bounds__(1024) void kb_warm_judge(const Inst* I, const long*
         return;
     }
     Strip s = load_strip(d, ic);
-    double cand[2] = {d.prev_gamma[ic], (double)pred[o + ic] * (double)scale[o + ic]};
+    const double g = (double)pred[o + ic] * (double)scale[o + ic];
+    double cand[2] = {d.prev_gamma[ic], residual ? d.prev_gamma[ic] + g : g};
     for (int c = 0; c < 2; ++c) {
         if (act) G[i] = cand[c];
         __syncthreads();
@@ -3073,14 +3080,14 @@ hipError_t launch_batch_warm_start(const BatchDev& b, sb_mlp* mlp, int apply, hi
     if (e != hipSuccess) return e;
     if (apply == 1)
         hipLaunchKernelGGL(kb_warm_apply, dim3(b.n), dim3(256), 0, st, b.inst, b.strip_off, b.pred, b.scale,
-                           b.fallbacks);
+                           b.fallbacks, b.warm_residual);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_warm_judge(const BatchDev& b, hipStream_t st) {
     size_t shm = sizeof(double) * ((size_t)b.max_threads + 16);
     hipLaunchKernelGGL(kb_warm_judge, dim3(b.n), dim3(b.max_threads), shm, st, b.inst, b.strip_off, b.pred, b.scale,
-                       b.fallbacks + 1, b.fallbacks);
+                       b.fallbacks + 1, b.fallbacks, b.warm_residual);
     return hipGetLastError();
 }
 

@@ -229,3 +229,22 @@ def test_autotuner_and_every_variant_keep_results(tmp_path, symmetric):
     fresh = sb.Batch(setups)
     with pytest.raises(RuntimeError, match="take a step"):
         fresh.autotune(None)
+
+
+def test_residual_warm_start_with_a_zero_network_is_the_default_guess():
+    """In residual mode the network predicts the change from the previous circulation: a
+    network whose every parameter is zero predicts no change, and the steps then equal the
+    steps without a warm start bit for bit."""
+    from stormbird_amd.route import RouteSimulation
+
+    a = RouteSimulation(n_ships=3, n_conditions=2, seed=6, nr_strips=8, nr_panels=12, nr_freqs=50)
+    b = RouteSimulation(n_ships=3, n_conditions=2, seed=6, nr_strips=8, nr_panels=12, nr_freqs=50)
+    net = sb.Mlp(seed=2)
+    net.set_params(np.zeros(net.nr_params, dtype=np.float32))
+    b.batch.set_warm_start(net, 2, residual=True)
+    for k in range(6):
+        print(f"residual warm start step {k}", flush=True)
+        ra, rb = a.step(), b.step()
+        for i in range(3):
+            compare_results(rb[i], ra[i], exact=True, where=f"step {k} ship {i}")
+    assert b.batch.last_fallbacks() == 0
