@@ -1,5 +1,5 @@
 // Learned warm start for the circulation solve: a per-strip MLP
-// (64 -> 256 -> 256 -> 256 -> 256 -> 1, GELU) whose training and inference
+// (64 -> 256 -> 256 -> 256 -> 256 -> 1, GELU in its sigmoid form) whose training and inference
 // GEMMs run on the gfx950 matrix cores in bf16 with fp32 accumulation and fp32
 // master weights (Adam).  No counterpart in the reference: the reference's
 // solver starts from the previous circulation (or zero at the first step,
@@ -60,31 +60,14 @@ __device__ __host__ inline float bf2f(uint16_t h) {
 // e^x on the hardware's base-2 exponential (relative error ~1e-7; 0 below e^-87)
 __device__ inline float exp_hw(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
 
-// GELU(x) = x Phi(x) with the normal tail q = (1 - Phi(|x|)) = erfc(|x| / sqrt 2) / 2 by
-// Abramowitz & Stegun 7.1.26 (Hastings' rational approximation): q = P(t) e^(-x^2/2) / 2,
-// t = 1 / (1 + p |x| / sqrt 2), P(t) = a1 t + ... + a5 t^5, |error in erf| <= 1.5e-7 --
-// a dozen instructions and no branch, against ~10^2 for a correctly rounded erff; the
-// values it feeds are rounded to bf16 (relative step 2^-8) right after.
-struct NormalTail {
-    float q;  // 1 - Phi(|x|)
-    float e;  // e^(-x^2/2)
-};
-__device__ inline NormalTail normal_tail(float x) {
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, fabsf(x), 1.0f));
-    float p = fmaf(0.5f * 1.061405429f, t, 0.5f * -1.453152027f);
-    p = fmaf(p, t, 0.5f * 1.421413741f);
-    p = fmaf(p, t, 0.5f * -0.284496736f);
-    p = fmaf(p, t, 0.5f * 0.254829592f);
-    const float e = exp_hw(-0.5f * x * x);
-    return {p * t * e, e};
-}
-__device__ inline float gelu(float x) {
-    const NormalTail n = normal_tail(x);
-    return x * (x >= 0.0f ? 1.0f - n.q : n.q);
-}
-__device__ inline float gelu_grad(float x) {  // Phi(x) + x phi(x)
-    const NormalTail n = normal_tail(x);
-    return (x >= 0.0f ? 1.0f - n.q : n.q) + x * (0.39894228040143268f * n.e);
+// GELU in the sigmoid form of Hendrycks & Gimpel (2016), x s(1.702 x) with s the
+// logistic function (within 0.02 of x Phi(x)): an exponential, a reciprocal and three
+// more operations, no branch; the values it feeds are rounded to bf16 right after.
+__device__ inline float gelu_s(float x) { return __builtin_amdgcn_rcpf(1.0f + exp_hw(-1.702f * x)); }
+__device__ inline float gelu(float x) { return x * gelu_s(x); }
+__device__ inline float gelu_grad(float x) {  // s + 1.702 x s (1 - s)
+    const float sg = gelu_s(x);
+    return sg + 1.702f * x * (sg * (1.0f - sg));
 }
 
 __device__ inline f32x4 mfma_bf16_k32(s16x8 a, s16x8 b, f32x4 c) {
@@ -104,7 +87,7 @@ struct NtArgs {
     int M, N, K;         // K a multiple of 32, N a multiple of 16
     int epi;
     const float* bias;   // [N] (EPI_FWD)
-    float* Z;            // fp32 [M][N]: the product (EPI_PLAIN) or product + bias (EPI_FWD; null: not kept)
+    float* Z;            // fp32 [M][N]: the product (EPI_PLAIN) or bias + product (EPI_FWD; null: not kept)
     uint16_t* Hb;        // bf16 [M][N]: gelu(product + bias) (EPI_FWD), product * gelu'(Zin) (EPI_BACK)
     const float* Zin;    // [M][N] (EPI_BACK)
 };
@@ -137,11 +120,16 @@ __global__ __launch_bounds__(256) void k_gemm_nt(NtArgs g) {
             *reinterpret_cast<s16x8*>(&Bs[buf][row][8 * part]) = rb[i];
         }
     };
+    // the accumulators start at the bias (EPI_FWD; zero otherwise): z = b + sum of products
     f32x4 acc[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+        const int col = n0 + wn + 16 * j + 4 * (lane / 16);
+        const f32x4 b0 = g.epi == EPI_FWD && col < g.N ? *reinterpret_cast<const f32x4*>(g.bias + col)
+                                                        : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int i = 0; i < 4; ++i) acc[i][j] = b0;
+    }
     const int nk = g.K / GK;
     load(0);
     store(0);
@@ -176,10 +164,6 @@ __global__ __launch_bounds__(256) void k_gemm_nt(NtArgs g) {
                 *reinterpret_cast<uint2*>(g.Hb + q) = make_uint2(f2bf2(z[0] * gelu_grad(zi[0]), z[1] * gelu_grad(zi[1])),
                                                                  f2bf2(z[2] * gelu_grad(zi[2]), z[3] * gelu_grad(zi[3])));
             } else {
-                if (g.epi == EPI_FWD) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) z[v] += g.bias[col + v];
-                }
                 if (g.Z) *reinterpret_cast<f32x4*>(g.Z + q) = z;
                 if (g.epi == EPI_FWD)
                     *reinterpret_cast<uint2*>(g.Hb + q) =
@@ -289,16 +273,17 @@ __global__ __launch_bounds__(256) void k_gemm_tn(TnArgs g) {
     }
 }
 
-// Whole-network inference, one kernel: 128 rows per workgroup, each wave owning
-// 32 rows whose activations stay in LDS from layer to layer (bf16, as the
-// layer-by-layer forward rounds them), the layer's weights streamed through LDS
-// in 32-deep K slices shared by the four waves.  Every product and epilogue is
+// Whole-network inference, one kernel: 128 rows per workgroup, each of its WAVES
+// waves owning 128 / WAVES rows whose activations stay in LDS from layer to layer
+// (bf16, as the layer-by-layer forward rounds them), the layer's weights streamed
+// through LDS in 64-deep K slices shared by the waves.  Every product and epilogue is
 // the one k_gemm_nt and k_head compute (same instruction, same K order), so the
 // outputs equal the layer-by-layer forward bit for bit.
 constexpr int IR = 128;   // rows per workgroup
 constexpr int ILD = 264;  // activation row stride in bf16 (528 B)
 constexpr int IBK = 64;   // K depth of one weight slice
 constexpr int IWLD = 72;  // weight-slice row stride in bf16 (144 B)
+constexpr int kInferWaves = 4;  // waves per inference workgroup (32 rows each; eight waves of 16 rows measured no faster)
 
 struct InferArgs {
     const float* X;          // [n][64]
@@ -310,88 +295,96 @@ struct InferArgs {
     float* y;                // [n]
 };
 
-__global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_infer(InferArgs g) {
+    constexpr int NT = 64 * WAVES;       // threads
+    constexpr int RT = IR / 16 / WAVES;  // 16-row MFMA tiles per wave
     __shared__ __attribute__((aligned(16))) uint16_t H[IR][ILD];
     __shared__ __attribute__((aligned(16))) uint16_t Ws[2][256][IWLD];
     __shared__ float w4s[256];  // head weights as k_head rounds them
     const int t = threadIdx.x, wave = t / 64, lane = t % 64;
     const long r0 = (long)blockIdx.x * IR;
-    w4s[t] = bf2f(f2bf(g.w4[t]));
+    if (t < 256) w4s[t] = bf2f(f2bf(g.w4[t]));
     {
-        // the features: 128 rows x 64 fp32 = 2048 pieces of 4 values, 8 per thread, all
-        // loads in flight before the first store
-        constexpr int PER = IR * MLP_FEATURES / 4 / 256;
+        // the features: 128 rows x 64 fp32 = 2048 pieces of 4 values, all loads in
+        // flight before the first store
+        constexpr int PER = IR * MLP_FEATURES / 4 / NT;
         float4 f[PER];
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int q = t + 256 * i, row = q / (MLP_FEATURES / 4), c4 = q % (MLP_FEATURES / 4);
+            const int q = t + NT * i, row = q / (MLP_FEATURES / 4), c4 = q % (MLP_FEATURES / 4);
             f[i] = r0 + row < g.n ? *reinterpret_cast<const float4*>(g.X + (r0 + row) * MLP_FEATURES + 4 * c4)
                                   : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int q = t + 256 * i, row = q / (MLP_FEATURES / 4), c4 = q % (MLP_FEATURES / 4);
+            const int q = t + NT * i, row = q / (MLP_FEATURES / 4), c4 = q % (MLP_FEATURES / 4);
             *reinterpret_cast<uint2*>(&H[row][4 * c4]) = make_uint2(f2bf2(f[i].x, f[i].y), f2bf2(f[i].z, f[i].w));
         }
     }
-    const int wr = wave * 32;
-    s16x8 rw[8];
+    const int wr = wave * 16 * RT;
+    constexpr int WP = 256 * IBK / 8 / NT;  // 16-byte pieces of a weight slice per thread
+    s16x8 rw[WP];
+    // thread t moves 16-byte pieces t + NT i of each 256 x 64 weight slice
+    auto load_w = [&](const uint16_t* W, int K, int k) {
+#pragma unroll
+        for (int i = 0; i < WP; ++i) {
+            const int p = t + NT * i, row = p / 8, part = p % 8;
+            rw[i] = *reinterpret_cast<const s16x8*>(W + (long)row * K + k + 8 * part);
+        }
+    };
+    auto store_w = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < WP; ++i) {
+            const int p = t + NT * i, row = p / 8, part = p % 8;
+            *reinterpret_cast<s16x8*>(&Ws[buf][row][8 * part]) = rw[i];
+        }
+    };
+    load_w(g.W[0], MLP_FEATURES, 0);
     for (int l = 0; l < 4; ++l) {
         const int K = l == 0 ? MLP_FEATURES : 256;
         const int nk = K / IBK;
         const uint16_t* W = g.W[l];
-        // thread t moves 16-byte pieces t + 256 i (i < 8) of each 256 x 64 weight slice
-        auto load_w = [&](int k) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int p = t + 256 * i, row = p / 8, part = p % 8;
-                rw[i] = *reinterpret_cast<const s16x8*>(W + (long)row * K + k + 8 * part);
-            }
-        };
-        auto store_w = [&](int buf) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int p = t + 256 * i, row = p / 8, part = p % 8;
-                *reinterpret_cast<s16x8*>(&Ws[buf][row][8 * part]) = rw[i];
-            }
-        };
-        load_w(0);
-        store_w(0);
+        store_w(0);  // the layer's first slice, loaded before the previous layer's epilogue
         __syncthreads();
-        f32x4 acc[2][16];
+        // the accumulators start at the bias: z = b + sum of products
+        f32x4 acc[RT][16];
+        const float* bias = g.b[l];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 16; ++j) {
+            const f32x4 b0 = *reinterpret_cast<const f32x4*>(bias + 16 * j + 4 * (lane / 16));
 #pragma unroll
-            for (int j = 0; j < 16; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int i = 0; i < RT; ++i) acc[i][j] = b0;
+        }
         for (int s = 0; s < nk; ++s) {
             const int buf = s & 1;
-            if (s + 1 < nk) load_w((s + 1) * IBK);
+            if (s + 1 < nk) load_w(W, K, (s + 1) * IBK);
 #pragma unroll
             for (int kk = 0; kk < IBK / GK; ++kk) {
                 const int kc = IBK * s + GK * kk + 8 * (lane / 16);
-                s16x8 a0 = *reinterpret_cast<const s16x8*>(&H[wr + lane % 16][kc]);
-                s16x8 a1 = *reinterpret_cast<const s16x8*>(&H[wr + 16 + lane % 16][kc]);
+                s16x8 a[RT];
+#pragma unroll
+                for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const s16x8*>(&H[wr + 16 * i + lane % 16][kc]);
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
                     s16x8 b = *reinterpret_cast<const s16x8*>(&Ws[buf][16 * j + lane % 16][GK * kk + 8 * (lane / 16)]);
-                    acc[0][j] = mfma_bf16_k32(b, a0, acc[0][j]);
-                    acc[1][j] = mfma_bf16_k32(b, a1, acc[1][j]);
+#pragma unroll
+                    for (int i = 0; i < RT; ++i) acc[i][j] = mfma_bf16_k32(b, a[i], acc[i][j]);
                 }
             }
             if (s + 1 < nk) store_w(buf ^ 1);
             __syncthreads();
         }
-        // this wave's rows: bias and GELU, back into the activations (read only by this wave)
-        const float* bias = g.b[l];
+        if (l < 3) load_w(g.W[l + 1], 256, 0);  // in flight during the epilogue
+        // this wave's rows: GELU, back into the activations (read only by this wave)
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < RT; ++i)
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
                 const int row = wr + 16 * i + lane % 16, col = 16 * j + 4 * (lane / 16);
                 const f32x4 z = acc[i][j];
                 *reinterpret_cast<uint2*>(&H[row][col]) =
-                    make_uint2(f2bf2(gelu(z[0] + bias[col]), gelu(z[1] + bias[col + 1])),
-                               f2bf2(gelu(z[2] + bias[col + 2]), gelu(z[3] + bias[col + 3])));
+                    make_uint2(f2bf2(gelu(z[0]), gelu(z[1])), f2bf2(gelu(z[2]), gelu(z[3])));
             }
     }
     __syncthreads();
@@ -661,7 +654,7 @@ struct sb_mlp {
         a.w4 = p + param_offset(4);
         a.b4 = a.w4 + 256;
         a.y = out;
-        hipLaunchKernelGGL(k_mlp_infer, dim3(nblk(n, IR)), dim3(256), 0, stream, a);
+        hipLaunchKernelGGL(k_mlp_infer<kInferWaves>, dim3(nblk(n, IR)), dim3(64 * kInferWaves), 0, stream, a);
         check(hipGetLastError(), "inference");
     }
 

@@ -1,13 +1,12 @@
 """CPU restatement (numpy, float64) of the warm-start network for checking the GPU kernels.
 
 Same architecture and parameter layout as sb_mlp (64 -> 256 -> 256 -> 256 -> 256 -> 1,
-GELU with erf; the GPU evaluates erf by Abramowitz & Stegun 7.1.26, |error| <= 1.5e-7,
-far below the bf16 rounding that follows), and the same rounding points: the GEMM operands are rounded to bf16
+GELU in the sigmoid form x s(1.702 x) of Hendrycks & Gimpel, s the logistic function),
+and the same rounding points: the GEMM operands are rounded to bf16
 (inputs, weights, activations, back-propagated gradients) and everything else is
 computed here in float64.  Test infrastructure only.
 """
 import numpy as np
-from scipy.special import erf
 
 LAYERS = [(64, 256), (256, 256), (256, 256), (256, 256), (256, 1)]  # (in, out)
 
@@ -31,12 +30,17 @@ def unpack(p):
     return out
 
 
+def _s(z):
+    return 1.0 / (1.0 + np.exp(-1.702 * z))
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    return z * _s(z)
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    s = _s(z)
+    return s + 1.702 * z * s * (1.0 - s)
 
 
 def forward(p, X):
