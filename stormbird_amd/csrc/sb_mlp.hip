@@ -365,12 +365,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_infer(InferArgs g) {
                 s16x8 a[RT];
 #pragma unroll
                 for (int i = 0; i < RT; ++i) a[i] = *reinterpret_cast<const s16x8*>(&H[wr + 16 * i + lane % 16][kc]);
+                // all sixteen weight fragments of this K step in flight before the first product
+                s16x8 bf[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    bf[j] = *reinterpret_cast<const s16x8*>(&Ws[buf][16 * j + lane % 16][GK * kk + 8 * (lane / 16)]);
 #pragma unroll
                 for (int j = 0; j < 16; ++j) {
-                    s16x8 b = *reinterpret_cast<const s16x8*>(&Ws[buf][16 * j + lane % 16][GK * kk + 8 * (lane / 16)]);
 #pragma unroll
-                    for (int i = 0; i < RT; ++i) acc[i][j] = mfma_bf16_k32(b, a[i], acc[i][j]);
+                    for (int i = 0; i < RT; ++i) acc[i][j] = mfma_bf16_k32(bf[j], a[i], acc[i][j]);
                 }
+                // schedule: six LDS reads ahead, then two products per further read, so that
+                // each read's latency hides behind a dozen products
+                static_assert(RT == 2, "the schedule below counts two row tiles");
+                __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+#pragma unroll
+                for (int q = 0; q < 12; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, 8, 0);
             }
             if (s + 1 < nk) store_w(buf ^ 1);
             __syncthreads();
