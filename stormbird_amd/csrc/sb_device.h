@@ -153,6 +153,8 @@ hipError_t launch_horseshoe_factors(const Inst& d, hipStream_t st);
 // the default Linearized settings (start of a SimpleIterative solve).  Needs d.lin
 // unless linearized_in_lds(d.N).
 hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st);
+// The one-workgroup Gaussian elimination (variant 1-3) on a dense row-major system.
+hipError_t launch_gauss_dense(const double* A, const double* b, int N, double* x, int* status, int variant, hipStream_t st);
 bool linearized_in_lds(int N);
 long linearized_global_doubles(int N);
 hipError_t launch_solve(const Inst& d, hipStream_t st);

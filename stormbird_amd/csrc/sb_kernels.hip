@@ -1692,18 +1692,23 @@ __device__ inline void lin_finish(const Inst& d, const Strip& s, double* G, doub
 // of row r in the columns j = q + c Q, c < MAXC (the right-hand side is column
 // N) in registers for the whole elimination.  Row exchanges are not carried
 // out: every wave keeps the logical position of every row (lanes hold rows l,
-// l + 64, ...), and the rows already used as pivots.  Per pivot k:
-//   - every wave finds the pivot row from column k (published in shared memory
-//     by its owners): the largest magnitude, ties to the lower logical position,
-//     NaN never chosen unless the row at position k holds NaN (then that row);
-//   - the pivot row's threads publish their entries (prow);
-//   - every other row still in play subtracts f * prow[j], f = a[r][k] / pivot,
-//     in blocks of 8 entries; blocks entirely left of column k + 1 are skipped,
-//     and the other entries left of it (never read again) take harmless updates;
-//     the owners of column k + 1 publish the new column.
-// Two barriers per pivot.  The upper triangle is then written row-major in
-// logical order for the back substitution.
+// l + 64, l + 128) and which rows have been pivots.  Per pivot k:
+//   - column k of every row is read from shared memory (cst: per row, the block
+//     of 8 register entries that holds the column, published at step k - 1);
+//   - every wave reduces it on its own: the largest magnitude among the rows not
+//     yet used, NaN excluded, ties to the lower logical position; the row at
+//     position k when that row holds NaN (linalg.rs:58-64 scans positions
+//     k + 1.. with a strict >, so a NaN at position k is never replaced);
+//   - each row's factor f = a[r][k] / pivot, formed before the first barrier;
+//   - the pivot row's threads publish their entries (prow, thread layout, only
+//     the blocks of 8 entries that still hold a column > k); barrier;
+//   - every other row subtracts f * prow[j] entry by entry in those blocks
+//     (entries left of column k + 1 in them take harmless updates: they are
+//     never read again), and the owners of column k + 1 publish its block; barrier.
+// The upper triangle is then written row-major in logical order, and wave 0
+// back-substitutes.
 constexpr int LINR_MAXS = 3;  // rows per lane in the pivot search (N <= 192)
+constexpr int LINR_CST = 9;   // doubles per row in cst (odd: column reads spread over the banks)
 
 template <int T>
 SB_HD inline int linr_q(int N) { return T / N < 16 ? T / N : 16; }
@@ -1713,122 +1718,175 @@ SB_HD inline bool linr_fits(int N) {
     const int q = linr_q<T>(N);
     return q >= 1 && (N + 1 + q - 1) / q <= MAXC;
 }
-// Shared-memory doubles: U [N][N + 1] (+8 pad), G (+8 pad), R, two column
-// buffers, position map, control, prow [max(N + 1, Q MAXC)].
+// Shared memory (doubles): U [N][N + 1] (+8), G [N] (+8), R [N] (also the position
+// map), cst (the column blocks; later the back substitution's two product rows),
+// prow [Q][MAXC + 2].  Every region starts on a 16-byte boundary.
+struct LinrLayout {
+    long G, R, cst, prow, total;
+    int pstride;  // product row stride of the back substitution
+};
+SB_HD inline long linr_even(long v) { return (v + 1) & ~1L; }
 template <int T, int MAXC>
-SB_HD inline long linr_shared_doubles(int N) {
+SB_HD inline LinrLayout linr_layout(int N) {
+    LinrLayout L;
     const long q = linr_q<T>(N);
-    const long pr = q * MAXC > N + 1 ? q * MAXC : N + 1;
-    return (long)N * (N + 1) + 8 + (N + 8) + N + 2L * N + N + 8 + pr;
+    L.pstride = (int)linr_even(N + 8);
+    L.G = (long)N * (N + 1) + 8;
+    L.R = L.G + linr_even(N + 8);
+    L.cst = L.R + linr_even(N);
+    const long cst = (long)LINR_CST * N > 2L * L.pstride + 16 ? (long)LINR_CST * N : 2L * L.pstride + 16;
+    L.prow = L.cst + linr_even(cst);
+    L.total = L.prow + q * (MAXC + 2);
+    return L;
 }
-
-// linalg.rs:92-101 on one lane over a row-major upper triangle.  U and x are
-// padded by 8 doubles so that every batch of 8 terms loads unconditionally;
-// terms past the row end are replaced by +0.0, which leaves the sum unchanged
-// (it starts at +0.0 and cannot become -0.0).  The next batch is loaded before
-// the current one is summed, in ascending j.
-__device__ inline void back_substitute_rows(const double* U, int N, double* x) {
-    const int ld = N + 1;
-    for (int ii = N - 1; ii >= 0; --ii) {
-        const double* row = U + (long)ii * ld;
-        double sum = 0.0;
-        int j = ii + 1;
-        double ua[8], xa[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            ua[u] = row[j + u];
-            xa[u] = x[j + u];
-        }
-        while (j < N) {
-            const int jn = j + 8;
-            double ub[8], xb[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                ub[u] = row[jn + u];
-                xb[u] = x[jn + u];
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double p = ua[u] * xa[u];
-                sum += (j + u < N) ? p : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                ua[u] = ub[u];
-                xa[u] = xb[u];
-            }
-            j = jn;
-        }
-        x[ii] = (row[N] - sum) / row[ii];
-    }
-}
-
-// One wave's argmax of (magnitude, key): larger magnitude wins, then the smaller
-// key.  Four DPP exchange levels inside each row of 16 lanes, then the four row
-// results by readlane; every lane returns the same pair.
-template <int CTRL>
-__device__ inline void argmax_level(double& bm, int& bkey) {
-    const int lo = __double2loint(bm), hi = __double2hiint(bm);
-    const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
-    const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
-    const int ok = __builtin_amdgcn_update_dpp(bkey, bkey, CTRL, 0xf, 0xf, false);
-    const double om = __hiloint2double(ohi, olo);
-    if (om > bm || (om == bm && ok < bkey)) {
-        bm = om;
-        bkey = ok;
-    }
-}
-__device__ inline void wave_argmax(double& bm, int& bkey) {
-    argmax_level<0xB1>(bm, bkey);   // quad_perm [1,0,3,2]
-    argmax_level<0x4E>(bm, bkey);   // quad_perm [2,3,0,1]
-    argmax_level<0x141>(bm, bkey);  // row_half_mirror
-    argmax_level<0x140>(bm, bkey);  // row_mirror
-    const int lo = __double2loint(bm), hi = __double2hiint(bm);
-    double m = bm;
-    int key = bkey;
-#pragma unroll
-    for (int rw = 0; rw < 4; ++rw) {
-        const double om = __hiloint2double(__builtin_amdgcn_readlane(hi, 16 * rw), __builtin_amdgcn_readlane(lo, 16 * rw));
-        const int ok = __builtin_amdgcn_readlane(bkey, 16 * rw);
-        if (rw == 0 || om > m || (om == m && ok < key)) {
-            m = om;
-            key = ok;
-        }
-    }
-    bm = m;
-    bkey = key;
-}
-
 template <int T, int MAXC>
-__device__ inline void linearized_reg_body(const Inst& d, double* sh, int as_start) {
+SB_HD inline long linr_shared_doubles(int N) { return linr_layout<T, MAXC>(N).total; }
+
+__device__ inline double readlane_f64(double v, int lane) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+    return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ inline double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(v), __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(v), __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+// Largest value over the wave (fmax: a NaN operand loses), the same on every lane.
+// Four DPP exchange levels inside each row of 16 lanes, then the four row results.
+__device__ inline double wave_max_f64(double v) {
+    v = fmax(v, dpp_f64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_f64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_f64<0x141>(v));  // row_half_mirror
+    v = fmax(v, dpp_f64<0x140>(v));  // row_mirror
+    return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)), fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
+}
+__device__ inline int wave_min_i32(int v) {
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xf, 0xf, false));
+    return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+               min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// linalg.rs:92-101 by one wave over the row-major upper triangle U (logical
+// rows, leading dimension N + 1).  Every lane runs the same ordered sum
+// sum = ((0 + u[i][i+1] x[i+1]) + u[i][i+2] x[i+2]) + ..., reading the products
+// u[i][j] x[j], j >= i + 2, from shared memory (pb, two rows of pstride): lane t
+// forms the products of row i - 1 while row i is summed (its x[j], j > i, are
+// known by then), padded with +0.0 to a multiple of 8 terms.  The padding leaves
+// the sum unchanged: it starts at +0.0 and cannot become -0.0.
+__device__ inline void back_substitute_wave(const double* U, int N, double* x, double* pb, int pstride) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const long ld = N + 1;
+    double xprev = (U[(N - 1) * ld + N] - 0.0) / U[(N - 1) * ld + N - 1];
+    x[N - 1] = xprev;
+    for (int i = N - 2; i >= 0; --i) {
+        const double* row = U + i * ld;
+        const double di = row[i], bi = row[N], u1 = row[i + 1];
+        // products of row i - 1: u[i-1][i+1+t] x[i+1+t], t < lenp
+        const int lenp = i >= 1 ? N - i - 1 : 0;
+        double pu[LINR_MAXS], px[LINR_MAXS];
+#pragma unroll
+        for (int s = 0; s < LINR_MAXS; ++s) {
+            const int t = lane + WAVE * s;
+            pu[s] = 0.0;
+            px[s] = 0.0;
+            if (t < lenp) {
+                pu[s] = row[-ld + i + 1 + t];
+                px[s] = x[i + 1 + t];
+            }
+        }
+        double sum = 0.0 + u1 * xprev;
+        const int len = N - i - 2;
+        const double2* p2 = (const double2*)(pb + (long)(i & 1) * pstride);
+        if (len > 0) {
+            double2 c0 = p2[0], c1 = p2[1], c2 = p2[2], c3 = p2[3];
+            for (int g = 0; g < len; g += 8) {
+                const double2* nx = p2 + (g >> 1) + 4;
+                const double2 n0 = nx[0], n1 = nx[1], n2 = nx[2], n3 = nx[3];
+                sum += c0.x;
+                sum += c0.y;
+                sum += c1.x;
+                sum += c1.y;
+                sum += c2.x;
+                sum += c2.y;
+                sum += c3.x;
+                sum += c3.y;
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+            }
+        }
+        if (lenp > 0) {
+            double* po = pb + (long)((i - 1) & 1) * pstride;
+            const int pad = (lenp + 7) & ~7;
+#pragma unroll
+            for (int s = 0; s < LINR_MAXS; ++s) {
+                const int t = lane + WAVE * s;
+                if (t < pad) po[t] = t < lenp ? pu[s] * px[s] : 0.0;
+            }
+        }
+        xprev = (bi - sum) / di;
+        x[i] = xprev;
+    }
+}
+
+// Where the one-workgroup elimination takes its augmented matrix from: row(r)
+// returns what every entry of row r needs, entry(row, r, j) the entry in column
+// j (j = N: the right-hand side).
+struct LinSystemSrc {  // linearized.rs:75-131
+    const Inst* d;
+    __device__ LinRow row(int r) const { return lin_row(load_strip(*d, r)); }
+    __device__ double entry(const LinRow& q, int r, int j) const { return j < d->N ? lin_entry(*d, q, r, j) : -q.cll; }
+};
+struct DenseSrc {  // a row-major N x N matrix and its right-hand side
+    const double* A;
+    const double* b;
+    int N;
+    struct Row {};
+    __device__ Row row(int) const { return Row{}; }
+    __device__ double entry(Row, int r, int j) const { return j < N ? A[(long)r * N + j] : b[r]; }
+};
+
+// Solves the system on one workgroup of T threads; the solution goes to G of
+// the layout.  Returns 0, or k + 1 when the pivot of step k is below 1e-12 in
+// magnitude (linalg.rs:67-71; G is then not written).  Ends with a barrier.
+template <int T, int MAXC, class Src>
+__device__ inline int gauss_regs(int N, const Src& src, double* sh) {
     static_assert(MAXC % 8 == 0, "MAXC in blocks of 8");
-    const int N = d.N;
+    constexpr int NB = MAXC / 8;
+    constexpr bool PREFETCH = MAXC <= 48;  // all live blocks of the pivot row in flight at once
+    const LinrLayout L = linr_layout<T, MAXC>(N);
     const int t = threadIdx.x;
     const int lane = t % WAVE;
     const int Q = linr_q<T>(N);
     const int r = t / Q, q = t - r * Q;
     const bool own = r < N;
-    double* U = sh;                           // [N][N + 1] row-major, logical rows (+8 pad)
-    double* G = U + (long)N * (N + 1) + 8;    // [N] (+8 pad)
-    double* R = G + N + 8;                    // [N]
-    double* cbuf = R + N;                     // [2][N]
-    double* posd = cbuf + 2 * N;              // [N] logical position of each row
-    double* ctl = posd + N;                   // [8]
-    double* prow = ctl + 8;                   // [max(N + 1, Q MAXC)]
+    const int nbu = ((N + 1 + Q - 1) / Q + 7) / 8;  // blocks that hold a column <= N
+    double* U = sh;
+    double* G = sh + L.G;
+    double* posd = sh + L.R;
+    double* cst = sh + L.cst;
+    double* prow = sh + L.prow;
+    double* prq = prow + (long)q * (MAXC + 2);
 
     double a[MAXC];
     {
-        const LinRow qr = lin_row(load_strip(d, own ? r : N - 1));
+        const auto qr = src.row(own ? r : N - 1);
 #pragma unroll
         for (int c = 0; c < MAXC; ++c) {
             const int j = q + c * Q;
-            double v = 0.0;
-            if (own && j < N) v = lin_entry(d, qr, r, j);
-            else if (own && j == N) v = -qr.cll;
-            a[c] = v;
+            a[c] = own && j <= N ? src.entry(qr, r, j) : 0.0;
         }
     }
-    if (own && q == 0) cbuf[r] = a[0];
+    if (own && q == 0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cst[r * LINR_CST + u] = a[u];
+    }
     int wpos[LINR_MAXS];
     bool wdone[LINR_MAXS];
 #pragma unroll
@@ -1839,46 +1897,61 @@ __device__ inline void linearized_reg_body(const Inst& d, double* sh, int as_sta
     bool done = false;
     int singular = 0;
     __syncthreads();
+    int kq = 0, kr = 0;  // k / Q, k % Q
     for (int k = 0; k < N; ++k) {
-        const double* cb = cbuf + (k & 1) * N;
-        double bm = -1.0;
-        int bkey = 0x7fffffff;  // logical position * 1024 + row
-        int rk = -1;            // row at logical position k
-        bool nan_k = false;
+        kq = __builtin_amdgcn_readfirstlane(kq);
+        kr = __builtin_amdgcn_readfirstlane(kr);
+        const int uk = kq & 7;
+        // the next column: its owner q and the block of its register entries
+        const int k1r = kr + 1 == Q ? 0 : kr + 1;
+        const int b1 = (kr + 1 == Q ? kq + 1 : kq) >> 3;
+        // blocks still in play: some column of the block (over all q) is > k
+        const int b0 = ((k + 2 + Q - 1) / Q + 7) / 8 - 1;
+        double v[LINR_MAXS];
 #pragma unroll
         for (int sl = 0; sl < LINR_MAXS; ++sl) {
             const int row = lane + WAVE * sl;
-            if (row < N && !wdone[sl]) {
-                const double v = cb[row];
-                if (wpos[sl] == k) {
-                    rk = row;
-                    nan_k = v != v;
-                }
-                const double m = fabs(v);
-                const int key = wpos[sl] * 1024 + row;
-                if (m > bm || (m == bm && key < bkey)) {  // NaN never wins
-                    bm = m;
-                    bkey = key;
-                }
+            v[sl] = cst[(row < N ? row : N - 1) * LINR_CST + uk];
+        }
+        const double fa = cst[(own ? r : 0) * LINR_CST + uk];
+        double mm = -1.0;
+#pragma unroll
+        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+            const bool cand = lane + WAVE * sl < N && !wdone[sl];
+            mm = fmax(mm, cand ? fabs(v[sl]) : -1.0);
+        }
+        const double M = wave_max_f64(mm);
+        int key = 0x7fffffff;  // logical position * 1024 + row
+        int rk = -1;           // row at logical position k
+        bool nank = false;
+#pragma unroll
+        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+            const int row = lane + WAVE * sl;
+            const bool cand = row < N && !wdone[sl];
+            const int kk = wpos[sl] * 1024 + row;
+            if (cand && fabs(v[sl]) == M && kk < key) key = kk;
+            if (cand && wpos[sl] == k) {
+                rk = row;
+                nank = v[sl] != v[sl];
             }
         }
-        wave_argmax(bm, bkey);
+        const int K = wave_min_i32(key);
         {
             const unsigned long long hit = __ballot(rk >= 0);
-            const int src = (int)__builtin_ctzll(hit);
-            rk = __builtin_amdgcn_readlane(rk, src);
-            nan_k = __builtin_amdgcn_readlane((int)nan_k, src) != 0;
+            const int src_lane = (int)__builtin_ctzll(hit);
+            rk = __builtin_amdgcn_readlane(rk, src_lane);
+            nank = __builtin_amdgcn_readlane((int)nank, src_lane) != 0;
         }
-        int pr = bkey & 1023, ppos = bkey >> 10;
-        if (nan_k || bkey == 0x7fffffff) {
+        int pr = K & 1023, ppos = K >> 10;
+        if (nank) {
             pr = rk;
             ppos = k;
         }
-        const double piv = cb[pr];
-        if (fabs(piv) < 1e-12) {
-            singular = k + 1;
-            break;
-        }
+        const int ps = pr >> 6;
+        const double piv = readlane_f64(ps == 0 ? v[0] : (ps == 1 ? v[1] : v[2]), pr & (WAVE - 1));
+        // a singular pivot is recorded and the remaining steps run on (their results
+        // are not used): the loop keeps one uniform exit
+        if (fabs(piv) < 1e-12 && !singular) singular = k + 1;
 #pragma unroll
         for (int sl = 0; sl < LINR_MAXS; ++sl) {
             const int row = lane + WAVE * sl;
@@ -1889,73 +1962,135 @@ __device__ inline void linearized_reg_body(const Inst& d, double* sh, int as_sta
                 wpos[sl] = ppos;
             }
         }
+        const double f = fa / piv;
+        asm volatile("" ::"v"(f));  // formed before the barrier, while the pivot row is published
         if (own && r == pr) {
             done = true;
 #pragma unroll
-            for (int c = 0; c < MAXC; ++c) prow[q + c * Q] = a[c];
+            for (int b = 0; b < NB; ++b) {
+                if (b >= b0 && b < nbu) {
+                    double2* o = (double2*)(prq + 8 * b);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) o[u] = make_double2(a[8 * b + 2 * u], a[8 * b + 2 * u + 1]);
+                }
+            }
         }
         __syncthreads();
         if (own && !done) {
-            const double f = cb[r] / piv;
+            if constexpr (PREFETCH) {
+                double2 w[MAXC / 2];
 #pragma unroll
-            for (int b8 = 0; b8 < MAXC / 8; ++b8) {
-                if (q + (8 * b8 + 7) * Q > k && q + 8 * b8 * Q <= N) {
-                    double w[8];
+                for (int b = 0; b < NB; ++b) {
+                    if (b >= b0 && b < nbu) {
+                        const double2* s2 = (const double2*)(prq + 8 * b);
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) w[u] = prow[q + (8 * b8 + u) * Q];
+                        for (int u = 0; u < 4; ++u) w[4 * b + u] = s2[u];
+                    } else {  // defined, so that w is not carried around the loop
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) a[8 * b8 + u] = a[8 * b8 + u] - f * w[u];
-                }
-            }
-            const int k1 = k + 1;
-            if (k1 < N && q == k1 % Q) {
-                const int c1 = k1 / Q;
-                double v = 0.0;
-#pragma unroll
-                for (int b8 = 0; b8 < MAXC / 8; ++b8) {
-                    if ((c1 >> 3) == b8) {
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            if ((c1 & 7) == u) v = a[8 * b8 + u];
+                        for (int u = 0; u < 4; ++u) w[4 * b + u] = make_double2(0.0, 0.0);
                     }
                 }
-                cbuf[(k1 & 1) * N + r] = v;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if (b >= b0 && b < nbu) {
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            a[8 * b + 2 * u] = a[8 * b + 2 * u] - f * w[4 * b + u].x;
+                            a[8 * b + 2 * u + 1] = a[8 * b + 2 * u + 1] - f * w[4 * b + u].y;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if (b >= b0 && b < nbu) {
+                        const double2* s2 = (const double2*)(prq + 8 * b);
+                        double2 w[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) w[u] = s2[u];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            a[8 * b + 2 * u] = a[8 * b + 2 * u] - f * w[u].x;
+                            a[8 * b + 2 * u + 1] = a[8 * b + 2 * u + 1] - f * w[u].y;
+                        }
+                    }
+                }
+            }
+            if (k + 1 < N && q == k1r) {
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if (b == b1) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) cst[r * LINR_CST + u] = a[8 * b + u];
+                    }
+                }
             }
         }
         __syncthreads();
+        if (++kr == Q) {
+            kr = 0;
+            ++kq;
+        }
     }
+    if (singular) return singular;
+    if (t < WAVE) {
+#pragma unroll
+        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+            const int row = lane + WAVE * sl;
+            if (row < N) posd[row] = (double)wpos[sl];
+        }
+    }
+    __syncthreads();
+    if (own) {
+        double* urow = U + (long)posd[r] * (N + 1);
+        // a fresh column index (opaque to the compiler), so that the column indices
+        // of the assembly are not kept in registers through the elimination
+        int j = q;
+        asm volatile("" : "+v"(j));
+#pragma unroll
+        for (int c = 0; c < MAXC; ++c) {
+            if (j <= N) urow[j] = a[c];
+            j += Q;
+        }
+    }
+    __syncthreads();
+    if (t < WAVE) back_substitute_wave(U, N, G, cst, L.pstride);
+    __syncthreads();
+    return 0;
+}
+
+template <int T, int MAXC>
+__device__ inline void linearized_reg_body(const Inst& d, double* sh, int as_start) {
+    const int N = d.N;
+    const int t = threadIdx.x;
+    const LinrLayout L = linr_layout<T, MAXC>(N);
+    double* G = sh + L.G;
+    double* R = sh + L.R;
+    const int singular = gauss_regs<T, MAXC>(N, LinSystemSrc{&d}, sh);
     if (singular) {
         if (t == 0) {
             for (int qq = 0; qq < N; ++qq) G[qq] = 0.0;
             d.scal[4] = (double)singular;
         }
-    } else {
-        if (t < WAVE) {
-#pragma unroll
-            for (int sl = 0; sl < LINR_MAXS; ++sl) {
-                const int row = lane + WAVE * sl;
-                if (row < N) posd[row] = (double)wpos[sl];
-            }
-        }
         __syncthreads();
-        if (own) {
-            double* urow = U + (long)posd[r] * (N + 1);
-#pragma unroll
-            for (int c = 0; c < MAXC; ++c) {
-                const int j = q + c * Q;
-                if (j <= N) urow[j] = a[c];
-            }
-        }
-        __syncthreads();
-        if (t == 0) back_substitute_rows(U, N, G);
     }
-    __syncthreads();
     lin_finish(d, load_strip(d, t < N ? t : N - 1), G, R, as_start);
 }
 template <int T, int MAXC>
 __global__ __launch_bounds__(T) void k_linearized(Inst d, int as_start) {
     extern __shared__ double sh[];
     linearized_reg_body<T, MAXC>(d, sh, as_start);
+}
+
+// The one-workgroup solve on a given system (checks of the elimination itself):
+// status = 0, or the singular step + 1 (x is then zero).
+template <int T, int MAXC>
+__global__ __launch_bounds__(T) void k_gauss_dense(const double* A, const double* b, int N, double* x, int* status) {
+    extern __shared__ double sh[];
+    const int s = gauss_regs<T, MAXC>(N, DenseSrc{A, b, N}, sh);
+    const double* G = sh + linr_layout<T, MAXC>(N).G;
+    if (threadIdx.x == 0) *status = s;
+    for (int i = threadIdx.x; i < N; i += T) x[i] = s ? 0.0 : G[i];
 }
 
 // ---- blocked form (matrix in global memory, d.lin) -------------------------
@@ -2851,10 +2986,29 @@ static long lds_limit_bytes() {
 }
 
 // One-workgroup variants: 256 threads x 72 entries (N <= 128), 512 x 48.
+// One-workgroup variants <threads, entries per thread>: 1 = <256, 72>, 2 = <512, 48>,
+// 3 = <512, 40>.  A variant applies when a row's N + 1 entries fit its threads'
+// registers and its shared memory fits the device.
+template <int T, int MAXC>
+static bool linr_ok(int N) {
+    return linr_fits<T, MAXC>(N) && (long)sizeof(double) * linr_shared_doubles<T, MAXC>(N) <= lds_limit_bytes();
+}
+static bool linr_variant_ok(int v, int N) {
+    switch (v) {
+        case 1: return linr_ok<256, 72>(N);
+        case 2: return linr_ok<512, 48>(N);
+        case 3: return linr_ok<512, 40>(N);
+        default: return false;
+    }
+}
+static size_t linr_variant_shared(int v, int N) {
+    long n = v == 1 ? linr_shared_doubles<256, 72>(N) : v == 2 ? linr_shared_doubles<512, 48>(N) : linr_shared_doubles<512, 40>(N);
+    return sizeof(double) * (size_t)n;
+}
+static int linr_variant_threads(int v) { return v == 1 ? 256 : 512; }
 static int linr_variant(int N) {
-    const long lim = lds_limit_bytes();
-    if (linr_fits<256, 72>(N) && N <= 128 && (long)sizeof(double) * linr_shared_doubles<256, 72>(N) <= lim) return 1;
-    if (linr_fits<512, 48>(N) && (long)sizeof(double) * linr_shared_doubles<512, 48>(N) <= lim) return 2;
+    for (int v : {3, 2, 1})
+        if (linr_variant_ok(v, N)) return v;
     return 0;
 }
 
@@ -2876,18 +3030,14 @@ hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
     const int N = d.N;
     if (N < 1 || N > 1024) return hipErrorInvalidValue;
     const int v = linr_variant(N);
-    if (v == 1) {
-        size_t shm = sizeof(double) * (size_t)linr_shared_doubles<256, 72>(N);
-        hipError_t e = allow_shared((const void*)k_linearized<256, 72>, shm);
+    if (v) {
+        const size_t shm = linr_variant_shared(v, N);
+        const void* fn = v == 1 ? (const void*)k_linearized<256, 72> : v == 2 ? (const void*)k_linearized<512, 48> : (const void*)k_linearized<512, 40>;
+        hipError_t e = allow_shared(fn, shm);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_linearized<256, 72>), dim3(1), dim3(256), shm, st, d, as_start);
-        return hipGetLastError();
-    }
-    if (v == 2) {
-        size_t shm = sizeof(double) * (size_t)linr_shared_doubles<512, 48>(N);
-        hipError_t e = allow_shared((const void*)k_linearized<512, 48>, shm);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((k_linearized<512, 48>), dim3(1), dim3(512), shm, st, d, as_start);
+        if (v == 1) hipLaunchKernelGGL((k_linearized<256, 72>), dim3(1), dim3(256), shm, st, d, as_start);
+        if (v == 2) hipLaunchKernelGGL((k_linearized<512, 48>), dim3(1), dim3(512), shm, st, d, as_start);
+        if (v == 3) hipLaunchKernelGGL((k_linearized<512, 40>), dim3(1), dim3(512), shm, st, d, as_start);
         return hipGetLastError();
     }
     if (!d.lin) return hipErrorInvalidValue;
@@ -2906,6 +3056,19 @@ hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
         hipLaunchKernelGGL(k_lin_trailing, dim3(cols), dim3(LIN_TRAIL_THREADS), 0, st, d, k0, bb);
     }
     hipLaunchKernelGGL(k_lin_finish, dim3(1), dim3(1024), sizeof(double) * 2 * (size_t)N, st, d, as_start);
+    return hipGetLastError();
+}
+
+hipError_t launch_gauss_dense(const double* A, const double* b, int N, double* x, int* status, int variant, hipStream_t st) {
+    if (!linr_variant_ok(variant, N)) return hipErrorInvalidValue;
+    const size_t shm = linr_variant_shared(variant, N);
+    const void* fn = variant == 1 ? (const void*)k_gauss_dense<256, 72> : variant == 2 ? (const void*)k_gauss_dense<512, 48> : (const void*)k_gauss_dense<512, 40>;
+    hipError_t e = allow_shared(fn, shm);
+    if (e != hipSuccess) return e;
+    const dim3 blk(linr_variant_threads(variant));
+    if (variant == 1) hipLaunchKernelGGL((k_gauss_dense<256, 72>), dim3(1), blk, shm, st, A, b, N, x, status);
+    if (variant == 2) hipLaunchKernelGGL((k_gauss_dense<512, 48>), dim3(1), blk, shm, st, A, b, N, x, status);
+    if (variant == 3) hipLaunchKernelGGL((k_gauss_dense<512, 40>), dim3(1), blk, shm, st, A, b, N, x, status);
     return hipGetLastError();
 }
 
@@ -3158,28 +3321,24 @@ int batch_linearized_variant(int max_n) {
 }
 
 size_t batch_linearized_shared(const int* ns, int n, int variant) {
-    long m = 0;
+    size_t m = 0;
     for (int i = 0; i < n; ++i) {
-        long x = variant == 1 ? linr_shared_doubles<256, 72>(ns[i]) : linr_shared_doubles<512, 48>(ns[i]);
-        bool fits = variant == 1 ? linr_fits<256, 72>(ns[i]) : linr_fits<512, 48>(ns[i]);
-        if (!fits) return 0;
+        if (!linr_variant_ok(variant, ns[i])) return 0;
+        const size_t x = linr_variant_shared(variant, ns[i]);
         m = x > m ? x : m;
     }
-    return sizeof(double) * (size_t)m;
+    return m;
 }
 
 hipError_t launch_batch_linearized(const BatchDev& b, int variant, size_t shm, int as_start, hipStream_t st) {
-    if (variant == 1) {
-        hipError_t e = allow_shared((const void*)kb_linearized<256, 72>, shm);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((kb_linearized<256, 72>), dim3(b.n), dim3(256), shm, st, b.inst, as_start);
-    } else if (variant == 2) {
-        hipError_t e = allow_shared((const void*)kb_linearized<512, 48>, shm);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((kb_linearized<512, 48>), dim3(b.n), dim3(512), shm, st, b.inst, as_start);
-    } else {
-        return hipErrorInvalidValue;
-    }
+    if (variant < 1 || variant > 3) return hipErrorInvalidValue;
+    const void* fn = variant == 1 ? (const void*)kb_linearized<256, 72> : variant == 2 ? (const void*)kb_linearized<512, 48> : (const void*)kb_linearized<512, 40>;
+    hipError_t e = allow_shared(fn, shm);
+    if (e != hipSuccess) return e;
+    const dim3 blk(linr_variant_threads(variant));
+    if (variant == 1) hipLaunchKernelGGL((kb_linearized<256, 72>), dim3(b.n), blk, shm, st, b.inst, as_start);
+    if (variant == 2) hipLaunchKernelGGL((kb_linearized<512, 48>), dim3(b.n), blk, shm, st, b.inst, as_start);
+    if (variant == 3) hipLaunchKernelGGL((kb_linearized<512, 40>), dim3(b.n), blk, shm, st, b.inst, as_start);
     return hipGetLastError();
 }
 

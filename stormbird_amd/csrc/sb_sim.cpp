@@ -384,6 +384,33 @@ int sb_debug_linearized_in_shared_memory(long n_strips) {
     return linearized_in_lds((int)n_strips) ? 1 : 0;
 }
 
+int sb_debug_gauss_device(const double* A, const double* b, size_t n, double* x, int variant) {
+    if (!A || !b || !x || n < 1 || n > 1024) return -1;
+    double* d = nullptr;
+    int rc = 0;
+    try {
+        d = dalloc<double>(n * n + 2 * n + 1);
+        check(hipMemcpy(d, A, sizeof(double) * n * n, hipMemcpyHostToDevice), "upload");
+        check(hipMemcpy(d + n * n, b, sizeof(double) * n, hipMemcpyHostToDevice), "upload");
+        int* status = (int*)(d + n * n + 2 * n);
+        hipError_t e = launch_gauss_dense(d, d + n * n, (int)n, d + n * n + n, status, variant, nullptr);
+        if (e == hipErrorInvalidValue) {
+            rc = -1;
+        } else {
+            check(e, "gauss");
+            int st = 0;
+            check(hipMemcpy(&st, status, sizeof(int), hipMemcpyDeviceToHost), "download");
+            check(hipMemcpy(x, d + n * n + n, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
+            rc = st;
+        }
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        rc = -2;
+    }
+    if (d) (void)hipFree(d);
+    return rc;
+}
+
 int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n) {
     if (!x || !out || (fn == 4 && !y)) return -1;
     for (size_t i = 0; i < n; ++i) {
