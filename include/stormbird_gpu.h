@@ -253,8 +253,8 @@ int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, si
 int sb_debug_linearized_in_shared_memory(long n_strips);
 /* The one-workgroup Gaussian elimination with partial pivoting that the
  * Linearized solve runs (stormath matrix/linalg.rs:41-104), on a given system:
- * A row-major n x n, b [n], x [n]; variant 1 (256 threads, 72 entries each),
- * 2 (512, 48) or 3 (512, 40).  Returns 0, k + 1 when the pivot of step k is
+ * A row-major n x n, b [n], x [n]; variant 1 (256 threads, n <= 64), 2 (256,
+ * n <= 127), 3 (256, n <= 128) or 4 (512, n <= 139).  Returns 0, k + 1 when the pivot of step k is
  * below 1e-12 in magnitude (where linalg.rs:67-71 returns its error; x is then
  * zero), -1 when n does not fit the variant, -2 on a device error. */
 int sb_debug_gauss_device(const double* A, const double* b, size_t n, double* x, int variant);

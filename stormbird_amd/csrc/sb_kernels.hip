@@ -1688,59 +1688,70 @@ __device__ inline void lin_finish(const Inst& d, const Strip& s, double* G, doub
 }
 
 // ---- one-workgroup form (small N) -------------------------------------------
-// T threads; Q = min(16, T / N) threads per row, thread (r, q) holds the entries
-// of row r in the columns j = q + c Q, c < MAXC (the right-hand side is column
-// N) in registers for the whole elimination.  Row exchanges are not carried
-// out: every wave keeps the logical position of every row (lanes hold rows l,
-// l + 64, l + 128) and which rows have been pivots.  Per pivot k:
-//   - column k of every row is read from shared memory (cst: per row, the block
-//     of 8 register entries that holds the column, published at step k - 1);
+// T threads in 8 column groups x T/8 row groups: thread (rg, cg) holds rows
+// r = rg + s T/8 (s < R) and columns j = cg + 8 c (c < C = 4 CB; the right-hand
+// side is column N) of the augmented matrix in registers for the whole
+// elimination, so a pivot-row value read from shared memory serves R rows.  Row
+// exchanges are not carried out: every wave keeps the logical position of every
+// row (lanes hold rows l, l + 64, ...) and which rows have been pivots.  Per pivot k:
+//   - column k of every row is read from shared memory (cst, published at step k - 1);
 //   - every wave reduces it on its own: the largest magnitude among the rows not
 //     yet used, NaN excluded, ties to the lower logical position; the row at
 //     position k when that row holds NaN (linalg.rs:58-64 scans positions
 //     k + 1.. with a strict >, so a NaN at position k is never replaced);
-//   - each row's factor f = a[r][k] / pivot, formed before the first barrier;
-//   - the pivot row's threads publish their entries (prow, thread layout, only
-//     the blocks of 8 entries that still hold a column > k); barrier;
-//   - every other row subtracts f * prow[j] entry by entry in those blocks
-//     (entries left of column k + 1 in them take harmless updates: they are
-//     never read again), and the owners of column k + 1 publish its block; barrier.
-// The upper triangle is then written row-major in logical order, and wave 0
-// back-substitutes.
-constexpr int LINR_MAXS = 3;  // rows per lane in the pivot search (N <= 192)
-constexpr int LINR_CST = 9;   // doubles per row in cst (odd: column reads spread over the banks)
-
-template <int T>
-SB_HD inline int linr_q(int N) { return T / N < 16 ? T / N : 16; }
-template <int T, int MAXC>
+//   - each thread's factors f = a[r][k] / pivot, formed before the first barrier;
+//   - the pivot row's threads publish its columns >= k (prow, [8][C + 2]); barrier;
+//   - every row subtracts f * prow[j] entry by entry in the blocks of 4 register
+//     columns that hold a column >= k (the rows already used as pivots and the
+//     entries left of column k + 1 take harmless updates: they are never read
+//     again), wave 0 copies the pivot row into U (row-major, logical position k),
+//     and the owners of column k + 1 publish it; barrier.
+// Wave 0 then back-substitutes.
+template <int V>
+struct LinrV;  // <threads, rows per thread, blocks of 4 columns per thread, 64-row slices of the pivot search>
+template <>
+struct LinrV<1> {
+    static constexpr int T = 256, R = 2, CB = 3, MS = 1;  // N <= 64
+};
+template <>
+struct LinrV<2> {
+    static constexpr int T = 256, R = 4, CB = 4, MS = 2;  // N <= 127
+};
+template <>
+struct LinrV<3> {
+    static constexpr int T = 256, R = 4, CB = 5, MS = 2;  // N <= 128
+};
+template <>
+struct LinrV<4> {
+    static constexpr int T = 512, R = 3, CB = 5, MS = 3;  // N <= 159 (160 KB of shared memory: N <= 139)
+};
+template <int V>
 SB_HD inline bool linr_fits(int N) {
-    if (N < 1 || N > 64 * LINR_MAXS) return false;
-    const int q = linr_q<T>(N);
-    return q >= 1 && (N + 1 + q - 1) / q <= MAXC;
+    using P = LinrV<V>;
+    return N >= 1 && N <= P::T / 8 * P::R && N + 1 <= 32 * P::CB && N <= 64 * P::MS;
 }
-// Shared memory (doubles): U [N][N + 1] (+8), G [N] (+8), R [N] (also the position
-// map), cst (the column blocks; later the back substitution's two product rows),
-// prow [Q][MAXC + 2].  Every region starts on a 16-byte boundary.
+// Shared memory (doubles): U [N][N + 1] (+8), G [N] (+8), R [N], cst (column k of
+// every row, [N + 1] with a spare slot; later the back substitution's two product
+// rows), prow [8][C + 2].  Every region starts on a 16-byte boundary.
 struct LinrLayout {
     long G, R, cst, prow, total;
     int pstride;  // product row stride of the back substitution
 };
 SB_HD inline long linr_even(long v) { return (v + 1) & ~1L; }
-template <int T, int MAXC>
+template <int V>
 SB_HD inline LinrLayout linr_layout(int N) {
     LinrLayout L;
-    const long q = linr_q<T>(N);
     L.pstride = (int)linr_even(N + 8);
     L.G = (long)N * (N + 1) + 8;
     L.R = L.G + linr_even(N + 8);
     L.cst = L.R + linr_even(N);
-    const long cst = (long)LINR_CST * N > 2L * L.pstride + 16 ? (long)LINR_CST * N : 2L * L.pstride + 16;
+    const long cst = N + 1 > 2L * L.pstride + 16 ? N + 1 : 2L * L.pstride + 16;
     L.prow = L.cst + linr_even(cst);
-    L.total = L.prow + q * (MAXC + 2);
+    L.total = L.prow + 8L * (4 * LinrV<V>::CB + 2);
     return L;
 }
-template <int T, int MAXC>
-SB_HD inline long linr_shared_doubles(int N) { return linr_layout<T, MAXC>(N).total; }
+template <int V>
+SB_HD inline long linr_shared_doubles(int N) { return linr_layout<V>(N).total; }
 
 __device__ inline double readlane_f64(double v, int lane) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
@@ -1779,6 +1790,7 @@ __device__ inline int wave_min_i32(int v) {
 // known by then), padded with +0.0 to a multiple of 8 terms.  The padding leaves
 // the sum unchanged: it starts at +0.0 and cannot become -0.0.
 __device__ inline void back_substitute_wave(const double* U, int N, double* x, double* pb, int pstride) {
+    constexpr int PS = 3;  // product slots per lane: rows of up to 192 products (N <= 193)
     const int lane = threadIdx.x & (WAVE - 1);
     const long ld = N + 1;
     double xprev = (U[(N - 1) * ld + N] - 0.0) / U[(N - 1) * ld + N - 1];
@@ -1788,9 +1800,9 @@ __device__ inline void back_substitute_wave(const double* U, int N, double* x, d
         const double di = row[i], bi = row[N], u1 = row[i + 1];
         // products of row i - 1: u[i-1][i+1+t] x[i+1+t], t < lenp
         const int lenp = i >= 1 ? N - i - 1 : 0;
-        double pu[LINR_MAXS], px[LINR_MAXS];
+        double pu[PS], px[PS];
 #pragma unroll
-        for (int s = 0; s < LINR_MAXS; ++s) {
+        for (int s = 0; s < PS; ++s) {
             const int t = lane + WAVE * s;
             pu[s] = 0.0;
             px[s] = 0.0;
@@ -1825,7 +1837,7 @@ __device__ inline void back_substitute_wave(const double* U, int N, double* x, d
             double* po = pb + (long)((i - 1) & 1) * pstride;
             const int pad = (lenp + 7) & ~7;
 #pragma unroll
-            for (int s = 0; s < LINR_MAXS; ++s) {
+            for (int s = 0; s < PS; ++s) {
                 const int t = lane + WAVE * s;
                 if (t < pad) po[t] = t < lenp ? pu[s] * px[s] : 0.0;
             }
@@ -1852,71 +1864,66 @@ struct DenseSrc {  // a row-major N x N matrix and its right-hand side
     __device__ double entry(Row, int r, int j) const { return j < N ? A[(long)r * N + j] : b[r]; }
 };
 
-// Solves the system on one workgroup of T threads; the solution goes to G of
-// the layout.  Returns 0, or k + 1 when the pivot of step k is below 1e-12 in
+// Solves the system on one workgroup of LinrV<V>::T threads; the solution goes to
+// G of the layout.  Returns 0, or k + 1 when the pivot of step k is below 1e-12 in
 // magnitude (linalg.rs:67-71; G is then not written).  Ends with a barrier.
-template <int T, int MAXC, class Src>
+template <int V, class Src>
 __device__ inline int gauss_regs(int N, const Src& src, double* sh) {
-    static_assert(MAXC % 8 == 0, "MAXC in blocks of 8");
-    constexpr int NB = MAXC / 8;
-    constexpr bool PREFETCH = MAXC <= 48;  // all live blocks of the pivot row in flight at once
-    const LinrLayout L = linr_layout<T, MAXC>(N);
-    const int t = threadIdx.x;
-    const int lane = t % WAVE;
-    const int Q = linr_q<T>(N);
-    const int r = t / Q, q = t - r * Q;
-    const bool own = r < N;
-    const int nbu = ((N + 1 + Q - 1) / Q + 7) / 8;  // blocks that hold a column <= N
+    using P = LinrV<V>;
+    constexpr int T = P::T, R = P::R, CB = P::CB, MS = P::MS;
+    constexpr int CG = 8, RG = T / CG, C = 4 * CB;
+    constexpr int US = (8 * C + WAVE - 1) / WAVE;  // wave 0's slots for the columns <= N
+    const LinrLayout L = linr_layout<V>(N);
+    const int t = threadIdx.x, lane = t & (WAVE - 1);
+    const int rg = t >> 3, cg = t & 7;
     double* U = sh;
     double* G = sh + L.G;
-    double* posd = sh + L.R;
     double* cst = sh + L.cst;
     double* prow = sh + L.prow;
-    double* prq = prow + (long)q * (MAXC + 2);
-
-    double a[MAXC];
-    {
+    double* prq = prow + cg * (C + 2);
+    const long ld = N + 1;
+    const int nbu = (N + 1 + 31) >> 5;  // blocks of 4 register columns that hold a column <= N
+    int rs[R];                          // this thread's rows; N (a spare slot of cst) past the end
+#pragma unroll
+    for (int s = 0; s < R; ++s) rs[s] = rg + RG * s < N ? rg + RG * s : N;
+    double a[R][C];
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int r = rg + RG * s;
+        const bool own = r < N;
         const auto qr = src.row(own ? r : N - 1);
 #pragma unroll
-        for (int c = 0; c < MAXC; ++c) {
-            const int j = q + c * Q;
-            a[c] = own && j <= N ? src.entry(qr, r, j) : 0.0;
+        for (int c = 0; c < C; ++c) {
+            const int j = cg + CG * c;
+            a[s][c] = own && j <= N ? src.entry(qr, r, j) : 0.0;
         }
     }
-    if (own && q == 0) {
+    if (cg == 0) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) cst[r * LINR_CST + u] = a[u];
+        for (int s = 0; s < R; ++s) cst[rs[s]] = a[s][0];
     }
-    int wpos[LINR_MAXS];
-    bool wdone[LINR_MAXS];
+    int wpos[MS];
+    bool wdone[MS];
 #pragma unroll
-    for (int sl = 0; sl < LINR_MAXS; ++sl) {
+    for (int sl = 0; sl < MS; ++sl) {
         wpos[sl] = lane + WAVE * sl;
         wdone[sl] = false;
     }
-    bool done = false;
     int singular = 0;
     __syncthreads();
-    int kq = 0, kr = 0;  // k / Q, k % Q
     for (int k = 0; k < N; ++k) {
-        kq = __builtin_amdgcn_readfirstlane(kq);
-        kr = __builtin_amdgcn_readfirstlane(kr);
-        const int uk = kq & 7;
-        // the next column: its owner q and the block of its register entries
-        const int k1r = kr + 1 == Q ? 0 : kr + 1;
-        const int b1 = (kr + 1 == Q ? kq + 1 : kq) >> 3;
-        // blocks still in play: some column of the block (over all q) is > k
-        const int b0 = ((k + 2 + Q - 1) / Q + 7) / 8 - 1;
-        double v[LINR_MAXS];
+        double v[MS];
 #pragma unroll
-        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+        for (int sl = 0; sl < MS; ++sl) {
             const int row = lane + WAVE * sl;
-            v[sl] = cst[(row < N ? row : N - 1) * LINR_CST + uk];
+            v[sl] = cst[row < N ? row : N - 1];
         }
-        const double fa = cst[(own ? r : 0) * LINR_CST + uk];
+        double fa[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) fa[s] = cst[rs[s]];
         double mm = -1.0;
 #pragma unroll
-        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+        for (int sl = 0; sl < MS; ++sl) {
             const bool cand = lane + WAVE * sl < N && !wdone[sl];
             mm = fmax(mm, cand ? fabs(v[sl]) : -1.0);
         }
@@ -1925,7 +1932,7 @@ __device__ inline int gauss_regs(int N, const Src& src, double* sh) {
         int rk = -1;           // row at logical position k
         bool nank = false;
 #pragma unroll
-        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+        for (int sl = 0; sl < MS; ++sl) {
             const int row = lane + WAVE * sl;
             const bool cand = row < N && !wdone[sl];
             const int kk = wpos[sl] * 1024 + row;
@@ -1947,13 +1954,18 @@ __device__ inline int gauss_regs(int N, const Src& src, double* sh) {
             pr = rk;
             ppos = k;
         }
+        pr = __builtin_amdgcn_readfirstlane(pr);
         const int ps = pr >> 6;
-        const double piv = readlane_f64(ps == 0 ? v[0] : (ps == 1 ? v[1] : v[2]), pr & (WAVE - 1));
+        double vsel = v[0];
+#pragma unroll
+        for (int sl = 1; sl < MS; ++sl)
+            if (ps == sl) vsel = v[sl];
+        const double piv = readlane_f64(vsel, pr & (WAVE - 1));
         // a singular pivot is recorded and the remaining steps run on (their results
         // are not used): the loop keeps one uniform exit
         if (fabs(piv) < 1e-12 && !singular) singular = k + 1;
 #pragma unroll
-        for (int sl = 0; sl < LINR_MAXS; ++sl) {
+        for (int sl = 0; sl < MS; ++sl) {
             const int row = lane + WAVE * sl;
             if (row == pr) {
                 wpos[sl] = k;
@@ -1962,111 +1974,92 @@ __device__ inline int gauss_regs(int N, const Src& src, double* sh) {
                 wpos[sl] = ppos;
             }
         }
-        const double f = fa / piv;
-        asm volatile("" ::"v"(f));  // formed before the barrier, while the pivot row is published
-        if (own && r == pr) {
-            done = true;
+        double f[R];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                if (b >= b0 && b < nbu) {
-                    double2* o = (double2*)(prq + 8 * b);
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) o[u] = make_double2(a[8 * b + 2 * u], a[8 * b + 2 * u + 1]);
-                }
-            }
+        for (int s = 0; s < R; ++s) {
+            f[s] = fa[s] / piv;
+            asm volatile("" ::"v"(f[s]));  // formed before the barrier, while the pivot row is published
         }
-        __syncthreads();
-        if (own && !done) {
-            if constexpr (PREFETCH) {
-                double2 w[MAXC / 2];
+        const int b0 = k >> 5;  // blocks that hold a column >= k
+        if (rg == pr % RG) {    // the pivot row's threads
+            const int sp = pr / RG;
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    if (b >= b0 && b < nbu) {
-                        const double2* s2 = (const double2*)(prq + 8 * b);
+            for (int s = 0; s < R; ++s) {
+                if (s == sp) {
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) w[4 * b + u] = s2[u];
-                    } else {  // defined, so that w is not carried around the loop
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) w[4 * b + u] = make_double2(0.0, 0.0);
-                    }
-                }
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    if (b >= b0 && b < nbu) {
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            a[8 * b + 2 * u] = a[8 * b + 2 * u] - f * w[4 * b + u].x;
-                            a[8 * b + 2 * u + 1] = a[8 * b + 2 * u + 1] - f * w[4 * b + u].y;
+                    for (int b = 0; b < CB; ++b) {
+                        if (b >= b0 && b < nbu) {
+                            double2* o = (double2*)(prq + 4 * b);
+                            o[0] = make_double2(a[s][4 * b], a[s][4 * b + 1]);
+                            o[1] = make_double2(a[s][4 * b + 2], a[s][4 * b + 3]);
                         }
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    if (b >= b0 && b < nbu) {
-                        const double2* s2 = (const double2*)(prq + 8 * b);
-                        double2 w[4];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) w[u] = s2[u];
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            a[8 * b + 2 * u] = a[8 * b + 2 * u] - f * w[u].x;
-                            a[8 * b + 2 * u + 1] = a[8 * b + 2 * u + 1] - f * w[u].y;
-                        }
-                    }
-                }
-            }
-            if (k + 1 < N && q == k1r) {
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    if (b == b1) {
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) cst[r * LINR_CST + u] = a[8 * b + u];
                     }
                 }
             }
         }
         __syncthreads();
-        if (++kr == Q) {
-            kr = 0;
-            ++kq;
+        double2 w[C / 2];
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            if (b < nbu) {
+                const double2* s2 = (const double2*)(prq + 4 * b);
+                w[2 * b] = s2[0];
+                w[2 * b + 1] = s2[1];
+            }
         }
+        double uc[US];
+        if (t < WAVE) {  // wave 0 copies the pivot row into U (logical row k)
+#pragma unroll
+            for (int sl = 0; sl < US; ++sl) {
+                const int j = lane + WAVE * sl;
+                uc[sl] = j <= N ? prow[(j & 7) * (C + 2) + (j >> 3)] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < CB; ++b) {
+            if (b >= b0 && b < nbu) {
+#pragma unroll
+                for (int s = 0; s < R; ++s) {
+                    a[s][4 * b] = a[s][4 * b] - f[s] * w[2 * b].x;
+                    a[s][4 * b + 1] = a[s][4 * b + 1] - f[s] * w[2 * b].y;
+                    a[s][4 * b + 2] = a[s][4 * b + 2] - f[s] * w[2 * b + 1].x;
+                    a[s][4 * b + 3] = a[s][4 * b + 3] - f[s] * w[2 * b + 1].y;
+                }
+            }
+        }
+        if (k + 1 < N && cg == ((k + 1) & 7)) {  // publish column k + 1
+            const int c1 = (k + 1) >> 3;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                if (c == c1) {
+#pragma unroll
+                    for (int s = 0; s < R; ++s) cst[rs[s]] = a[s][c];
+                }
+            }
+        }
+        if (t < WAVE) {
+#pragma unroll
+            for (int sl = 0; sl < US; ++sl) {
+                const int j = lane + WAVE * sl;
+                if (j <= N) U[k * ld + j] = uc[sl];
+            }
+        }
+        __syncthreads();
     }
     if (singular) return singular;
-    if (t < WAVE) {
-#pragma unroll
-        for (int sl = 0; sl < LINR_MAXS; ++sl) {
-            const int row = lane + WAVE * sl;
-            if (row < N) posd[row] = (double)wpos[sl];
-        }
-    }
-    __syncthreads();
-    if (own) {
-        double* urow = U + (long)posd[r] * (N + 1);
-        // a fresh column index (opaque to the compiler), so that the column indices
-        // of the assembly are not kept in registers through the elimination
-        int j = q;
-        asm volatile("" : "+v"(j));
-#pragma unroll
-        for (int c = 0; c < MAXC; ++c) {
-            if (j <= N) urow[j] = a[c];
-            j += Q;
-        }
-    }
-    __syncthreads();
     if (t < WAVE) back_substitute_wave(U, N, G, cst, L.pstride);
     __syncthreads();
     return 0;
 }
 
-template <int T, int MAXC>
+template <int V>
 __device__ inline void linearized_reg_body(const Inst& d, double* sh, int as_start) {
     const int N = d.N;
     const int t = threadIdx.x;
-    const LinrLayout L = linr_layout<T, MAXC>(N);
+    const LinrLayout L = linr_layout<V>(N);
     double* G = sh + L.G;
     double* R = sh + L.R;
-    const int singular = gauss_regs<T, MAXC>(N, LinSystemSrc{&d}, sh);
+    const int singular = gauss_regs<V>(N, LinSystemSrc{&d}, sh);
     if (singular) {
         if (t == 0) {
             for (int qq = 0; qq < N; ++qq) G[qq] = 0.0;
@@ -2076,21 +2069,21 @@ __device__ inline void linearized_reg_body(const Inst& d, double* sh, int as_sta
     }
     lin_finish(d, load_strip(d, t < N ? t : N - 1), G, R, as_start);
 }
-template <int T, int MAXC>
-__global__ __launch_bounds__(T) void k_linearized(Inst d, int as_start) {
+template <int V>
+__global__ __launch_bounds__(LinrV<V>::T) void k_linearized(Inst d, int as_start) {
     extern __shared__ double sh[];
-    linearized_reg_body<T, MAXC>(d, sh, as_start);
+    linearized_reg_body<V>(d, sh, as_start);
 }
 
 // The one-workgroup solve on a given system (checks of the elimination itself):
 // status = 0, or the singular step + 1 (x is then zero).
-template <int T, int MAXC>
-__global__ __launch_bounds__(T) void k_gauss_dense(const double* A, const double* b, int N, double* x, int* status) {
+template <int V>
+__global__ __launch_bounds__(LinrV<V>::T) void k_gauss_dense(const double* A, const double* b, int N, double* x, int* status) {
     extern __shared__ double sh[];
-    const int s = gauss_regs<T, MAXC>(N, DenseSrc{A, b, N}, sh);
-    const double* G = sh + linr_layout<T, MAXC>(N).G;
+    const int s = gauss_regs<V>(N, DenseSrc{A, b, N}, sh);
+    const double* G = sh + linr_layout<V>(N).G;
     if (threadIdx.x == 0) *status = s;
-    for (int i = threadIdx.x; i < N; i += T) x[i] = s ? 0.0 : G[i];
+    for (int i = threadIdx.x; i < N; i += LinrV<V>::T) x[i] = s ? 0.0 : G[i];
 }
 
 // ---- blocked form (matrix in global memory, d.lin) -------------------------
@@ -2498,11 +2491,11 @@ __global__ void kb_horseshoe_factors(const Inst* I, const long* off, int n) {
 }
 
 // One workgroup per instance (the one-workgroup Linearized form).
-template <int T, int MAXC>
-__global__ __launch_bounds__(T) void kb_linearized(const Inst* I, int as_start) {
+template <int V>
+__global__ __launch_bounds__(LinrV<V>::T) void kb_linearized(const Inst* I, int as_start) {
     extern __shared__ double sh[];
     const Inst d = I[blockIdx.x];
-    linearized_reg_body<T, MAXC>(d, sh, as_start);
+    linearized_reg_body<V>(d, sh, as_start);
 }
 
 // Singular-pivot flags of every instance (scal[4]) into flags[n], then cleared.
@@ -2986,38 +2979,61 @@ static long lds_limit_bytes() {
 }
 
 // One-workgroup variants: 256 threads x 72 entries (N <= 128), 512 x 48.
-// One-workgroup variants <threads, entries per thread>: 1 = <256, 72>, 2 = <512, 48>,
-// 3 = <512, 40>.  A variant applies when a row's N + 1 entries fit its threads'
-// registers and its shared memory fits the device.
-template <int T, int MAXC>
-static bool linr_ok(int N) {
-    return linr_fits<T, MAXC>(N) && (long)sizeof(double) * linr_shared_doubles<T, MAXC>(N) <= lds_limit_bytes();
-}
-static bool linr_variant_ok(int v, int N) {
-    switch (v) {
-        case 1: return linr_ok<256, 72>(N);
-        case 2: return linr_ok<512, 48>(N);
-        case 3: return linr_ok<512, 40>(N);
-        default: return false;
-    }
-}
-static size_t linr_variant_shared(int v, int N) {
-    long n = v == 1 ? linr_shared_doubles<256, 72>(N) : v == 2 ? linr_shared_doubles<512, 48>(N) : linr_shared_doubles<512, 40>(N);
-    return sizeof(double) * (size_t)n;
-}
-static int linr_variant_threads(int v) { return v == 1 ? 256 : 512; }
-static int linr_variant(int N) {
-    for (int v : {3, 2, 1})
-        if (linr_variant_ok(v, N)) return v;
-    return 0;
-}
-
-bool linearized_in_lds(int N) { return linr_variant(N) != 0; }
-
 static hipError_t allow_shared(const void* fn, size_t bytes) {
     if (bytes <= 65536) return hipSuccess;
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
+
+// One-workgroup variants (LinrV): a variant applies when the rows and columns
+// fit its threads' registers and its shared memory fits the device.
+template <int V>
+static bool linr_ok(int N) {
+    return linr_fits<V>(N) && (long)sizeof(double) * linr_shared_doubles<V>(N) <= lds_limit_bytes();
+}
+static bool linr_variant_ok(int v, int N) {
+    switch (v) {
+        case 1: return linr_ok<1>(N);
+        case 2: return linr_ok<2>(N);
+        case 3: return linr_ok<3>(N);
+        case 4: return linr_ok<4>(N);
+        default: return false;
+    }
+}
+static size_t linr_variant_shared(int v, int N) {
+    long n = v == 1 ? linr_shared_doubles<1>(N) : v == 2 ? linr_shared_doubles<2>(N)
+           : v == 3 ? linr_shared_doubles<3>(N) : linr_shared_doubles<4>(N);
+    return sizeof(double) * (size_t)n;
+}
+static int linr_variant(int N) {
+    for (int v = 1; v <= 4; ++v)
+        if (linr_variant_ok(v, N)) return v;
+    return 0;
+}
+template <int V>
+static hipError_t launch_linr(const Inst& d, int as_start, size_t shm, hipStream_t st) {
+    hipError_t e = allow_shared((const void*)k_linearized<V>, shm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_linearized<V>), dim3(1), dim3(LinrV<V>::T), shm, st, d, as_start);
+    return hipGetLastError();
+}
+template <int V>
+static hipError_t launch_linr_batch(const BatchDev& b, int as_start, size_t shm, hipStream_t st) {
+    hipError_t e = allow_shared((const void*)kb_linearized<V>, shm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((kb_linearized<V>), dim3(b.n), dim3(LinrV<V>::T), shm, st, b.inst, as_start);
+    return hipGetLastError();
+}
+template <int V>
+static hipError_t launch_linr_dense(const double* A, const double* b, int N, double* x, int* status, size_t shm,
+                                    hipStream_t st) {
+    hipError_t e = allow_shared((const void*)k_gauss_dense<V>, shm);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_gauss_dense<V>), dim3(1), dim3(LinrV<V>::T), shm, st, A, b, N, x, status);
+    return hipGetLastError();
+}
+
+bool linearized_in_lds(int N) { return linr_variant(N) != 0; }
+
 
 // Panel width of the blocked form: up to 16 columns of N rows in shared memory.
 static int linearized_panel_width(int N) {
@@ -3032,13 +3048,12 @@ hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
     const int v = linr_variant(N);
     if (v) {
         const size_t shm = linr_variant_shared(v, N);
-        const void* fn = v == 1 ? (const void*)k_linearized<256, 72> : v == 2 ? (const void*)k_linearized<512, 48> : (const void*)k_linearized<512, 40>;
-        hipError_t e = allow_shared(fn, shm);
-        if (e != hipSuccess) return e;
-        if (v == 1) hipLaunchKernelGGL((k_linearized<256, 72>), dim3(1), dim3(256), shm, st, d, as_start);
-        if (v == 2) hipLaunchKernelGGL((k_linearized<512, 48>), dim3(1), dim3(512), shm, st, d, as_start);
-        if (v == 3) hipLaunchKernelGGL((k_linearized<512, 40>), dim3(1), dim3(512), shm, st, d, as_start);
-        return hipGetLastError();
+        switch (v) {
+            case 1: return launch_linr<1>(d, as_start, shm, st);
+            case 2: return launch_linr<2>(d, as_start, shm, st);
+            case 3: return launch_linr<3>(d, as_start, shm, st);
+            default: return launch_linr<4>(d, as_start, shm, st);
+        }
     }
     if (!d.lin) return hipErrorInvalidValue;
     const int b = linearized_panel_width(N);
@@ -3062,14 +3077,12 @@ hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
 hipError_t launch_gauss_dense(const double* A, const double* b, int N, double* x, int* status, int variant, hipStream_t st) {
     if (!linr_variant_ok(variant, N)) return hipErrorInvalidValue;
     const size_t shm = linr_variant_shared(variant, N);
-    const void* fn = variant == 1 ? (const void*)k_gauss_dense<256, 72> : variant == 2 ? (const void*)k_gauss_dense<512, 48> : (const void*)k_gauss_dense<512, 40>;
-    hipError_t e = allow_shared(fn, shm);
-    if (e != hipSuccess) return e;
-    const dim3 blk(linr_variant_threads(variant));
-    if (variant == 1) hipLaunchKernelGGL((k_gauss_dense<256, 72>), dim3(1), blk, shm, st, A, b, N, x, status);
-    if (variant == 2) hipLaunchKernelGGL((k_gauss_dense<512, 48>), dim3(1), blk, shm, st, A, b, N, x, status);
-    if (variant == 3) hipLaunchKernelGGL((k_gauss_dense<512, 40>), dim3(1), blk, shm, st, A, b, N, x, status);
-    return hipGetLastError();
+    switch (variant) {
+        case 1: return launch_linr_dense<1>(A, b, N, x, status, shm, st);
+        case 2: return launch_linr_dense<2>(A, b, N, x, status, shm, st);
+        case 3: return launch_linr_dense<3>(A, b, N, x, status, shm, st);
+        default: return launch_linr_dense<4>(A, b, N, x, status, shm, st);
+    }
 }
 
 // Doubles of d.lin for the blocked form: matrix, row data, exchange sources.
@@ -3331,15 +3344,13 @@ size_t batch_linearized_shared(const int* ns, int n, int variant) {
 }
 
 hipError_t launch_batch_linearized(const BatchDev& b, int variant, size_t shm, int as_start, hipStream_t st) {
-    if (variant < 1 || variant > 3) return hipErrorInvalidValue;
-    const void* fn = variant == 1 ? (const void*)kb_linearized<256, 72> : variant == 2 ? (const void*)kb_linearized<512, 48> : (const void*)kb_linearized<512, 40>;
-    hipError_t e = allow_shared(fn, shm);
-    if (e != hipSuccess) return e;
-    const dim3 blk(linr_variant_threads(variant));
-    if (variant == 1) hipLaunchKernelGGL((kb_linearized<256, 72>), dim3(b.n), blk, shm, st, b.inst, as_start);
-    if (variant == 2) hipLaunchKernelGGL((kb_linearized<512, 48>), dim3(b.n), blk, shm, st, b.inst, as_start);
-    if (variant == 3) hipLaunchKernelGGL((kb_linearized<512, 40>), dim3(b.n), blk, shm, st, b.inst, as_start);
-    return hipGetLastError();
+    switch (variant) {
+        case 1: return launch_linr_batch<1>(b, as_start, shm, st);
+        case 2: return launch_linr_batch<2>(b, as_start, shm, st);
+        case 3: return launch_linr_batch<3>(b, as_start, shm, st);
+        case 4: return launch_linr_batch<4>(b, as_start, shm, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_batch_take_singular(const BatchDev& b, double* flags, hipStream_t st) {
