@@ -1614,22 +1614,75 @@ __device__ inline int wave_pivot(const double* colk, int k, int N, int lane) {
     return bi;
 }
 
-// linalg.rs:92-101 on one lane: x[i] = (b[i] - sum_{j>i} a[i][j] x[j]) / a[i][i].
-__device__ inline void back_substitute(const double* aug, int N, const double* diag, long dstride, double* x) {
-    for (int ii = N - 1; ii >= 0; --ii) {
-        double sum = 0.0;
-        int j = ii + 1;
-        for (; j + 4 <= N; j += 4) {
-            double a0 = aug[(long)j * N + ii], a1 = aug[(long)(j + 1) * N + ii];
-            double a2 = aug[(long)(j + 2) * N + ii], a3 = aug[(long)(j + 3) * N + ii];
-            double x0 = x[j], x1 = x[j + 1], x2 = x[j + 2], x3 = x[j + 3];
-            sum += a0 * x0;
-            sum += a1 * x1;
-            sum += a2 * x2;
-            sum += a3 * x3;
+// linalg.rs:92-101 by one wave over the factored matrix in global memory (A
+// column-major [N + 1][N]: a[i][j] = A[j N + i], column N the right-hand side),
+// N <= 1024.  Every lane runs the same ordered sum
+// sum = ((0 + a[i][i+1] x[i+1]) + a[i][i+2] x[i+2]) + ..., reading the products
+// a[i][j] x[j], j >= i + 2, from shared memory (pb, two rows of pstride >= N + 8;
+// x in shared memory too): lane t forms the products of row i - 1 while row i is
+// summed, from matrix entries loaded one row earlier, padded with +0.0 to a
+// multiple of 8 terms (the sum starts at +0.0 and cannot become -0.0).
+__device__ inline void back_substitute_wave_cm(const double* A, int N, double* x, double* pb, int pstride) {
+    constexpr int PS = 16;  // product slots per lane: rows of up to 1024 products
+    const int lane = threadIdx.x & (WAVE - 1);
+    const long n = N;
+    double xprev = (A[n * n + N - 1] - 0.0) / A[(n - 1) * n + N - 1];
+    x[N - 1] = xprev;
+    double cu[PS];  // a[i-1][i+1+t] for the products formed at step i
+#pragma unroll
+    for (int s = 0; s < PS; ++s) cu[s] = 0.0;
+    if (N >= 3 && lane == 0) cu[0] = A[(n - 1) * n + N - 3];
+    for (int i = N - 2; i >= 0; --i) {
+        const double di = A[(long)i * n + i], bi = A[n * n + i], u1 = A[(long)(i + 1) * n + i];
+        double nu[PS];  // a[i-2][i+t], t < N - i: the products formed at step i - 1
+#pragma unroll
+        for (int s = 0; s < PS; ++s) {
+            const int t = lane + WAVE * s;
+            nu[s] = 0.0;
+            if (i >= 2 && t < N - i) nu[s] = A[(long)(i + t) * n + (i - 2)];
         }
-        for (; j < N; ++j) sum += aug[(long)j * N + ii] * x[j];
-        x[ii] = (aug[(long)N * N + ii] - sum) / diag[(long)ii * dstride];
+        const int lenp = i >= 1 ? N - i - 1 : 0;
+        double px[PS];
+#pragma unroll
+        for (int s = 0; s < PS; ++s) {
+            const int t = lane + WAVE * s;
+            px[s] = t < lenp ? x[i + 1 + t] : 0.0;
+        }
+        double sum = 0.0 + u1 * xprev;
+        const int len = N - i - 2;
+        if (len > 0) {
+            const double2* p2 = (const double2*)(pb + (long)(i & 1) * pstride);
+            double2 c0 = p2[0], c1 = p2[1], c2 = p2[2], c3 = p2[3];
+            for (int g = 0; g < len; g += 8) {
+                const double2* nx = p2 + (g >> 1) + 4;
+                const double2 n0 = nx[0], n1 = nx[1], n2 = nx[2], n3 = nx[3];
+                sum += c0.x;
+                sum += c0.y;
+                sum += c1.x;
+                sum += c1.y;
+                sum += c2.x;
+                sum += c2.y;
+                sum += c3.x;
+                sum += c3.y;
+                c0 = n0;
+                c1 = n1;
+                c2 = n2;
+                c3 = n3;
+            }
+        }
+        if (lenp > 0) {
+            double* po = pb + (long)((i - 1) & 1) * pstride;
+            const int pad = (lenp + 7) & ~7;
+#pragma unroll
+            for (int s = 0; s < PS; ++s) {
+                const int t = lane + WAVE * s;
+                if (t < pad) po[t] = t < lenp ? cu[s] * px[s] : 0.0;
+            }
+        }
+        xprev = (bi - sum) / di;
+        x[i] = xprev;
+#pragma unroll
+        for (int s = 0; s < PS; ++s) cu[s] = nu[s];
     }
 }
 
@@ -2269,18 +2322,19 @@ __global__ __launch_bounds__(LIN_TRAIL_THREADS) void k_lin_trailing(Inst d, int 
 }
 
 // Back substitution and the rest of the solve after the blocked factorization.
+// Shared memory: G [N] (+8), R [N], two product rows of linr_even(N + 8) (+16).
 __global__ __launch_bounds__(1024) void k_lin_finish(Inst d, int as_start) {
     extern __shared__ double sh[];
     const int N = d.N, t = threadIdx.x;
     double* G = sh;
-    double* R = G + N;
+    double* R = G + linr_even(N + 8);
+    double* pb = R + linr_even(N);
     const Strip s = load_strip(d, t < N ? t : N - 1);
-    if (t == 0) {
-        if (d.scal[4] != 0.0) {
+    if (d.scal[4] != 0.0) {
+        if (t == 0)
             for (int q = 0; q < N; ++q) G[q] = 0.0;
-        } else {
-            back_substitute(d.lin, N, d.lin, (long)N + 1, G);
-        }
+    } else if (t < WAVE) {
+        back_substitute_wave_cm(d.lin, N, G, pb, (int)linr_even(N + 8));
     }
     __syncthreads();
     lin_finish(d, s, G, R, as_start);
@@ -3070,7 +3124,10 @@ hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
         const unsigned cols = (unsigned)(N + 1 - k0 - bb);
         hipLaunchKernelGGL(k_lin_trailing, dim3(cols), dim3(LIN_TRAIL_THREADS), 0, st, d, k0, bb);
     }
-    hipLaunchKernelGGL(k_lin_finish, dim3(1), dim3(1024), sizeof(double) * 2 * (size_t)N, st, d, as_start);
+    const size_t fin = sizeof(double) * (size_t)(linr_even(N + 8) + linr_even(N) + 2 * linr_even(N + 8) + 16);
+    hipError_t ef = allow_shared((const void*)k_lin_finish, fin);
+    if (ef != hipSuccess) return ef;
+    hipLaunchKernelGGL(k_lin_finish, dim3(1), dim3(1024), fin, st, d, as_start);
     return hipGetLastError();
 }
 
