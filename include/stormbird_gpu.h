@@ -258,6 +258,10 @@ int sb_debug_linearized_in_shared_memory(long n_strips);
  * below 1e-12 in magnitude (where linalg.rs:67-71 returns its error; x is then
  * zero), -1 when n does not fit the variant, -2 on a device error. */
 int sb_debug_gauss_device(const double* A, const double* b, size_t n, double* x, int variant);
+/* The same elimination in the blocked form the Linearized solve takes for larger
+ * systems (16-column panels, trailing updates, one-wave back substitution), the
+ * matrix in global memory: n <= 1024.  Returns as sb_debug_gauss_device. */
+int sb_debug_gauss_blocked_device(const double* A, const double* b, size_t n, double* x);
 
 /* ---- Batched simulations ------------------------------------------------
  * Many independent simulations (each built from its own setup string, with

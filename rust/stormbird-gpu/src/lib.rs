@@ -137,6 +137,7 @@ pub mod ffi {
         pub fn sb_debug_libm_host(func: c_int, x: *const f64, y: *const f64, out: *mut f64, n: usize) -> c_int;
         pub fn sb_debug_linearized_in_shared_memory(n_strips: c_long) -> c_int;
         pub fn sb_debug_gauss_device(a: *const f64, b: *const f64, n: usize, x: *mut f64, variant: c_int) -> c_int;
+        pub fn sb_debug_gauss_blocked_device(a: *const f64, b: *const f64, n: usize, x: *mut f64) -> c_int;
         pub fn sb_batch_create(setup_jsons: *const *const c_char, n: c_long, device: c_int, mode: c_int) -> *mut sb_batch;
         pub fn sb_batch_destroy(b: *mut sb_batch);
         pub fn sb_batch_size(b: *const sb_batch) -> c_long;

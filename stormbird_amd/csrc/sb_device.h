@@ -155,6 +155,9 @@ hipError_t launch_horseshoe_factors(const Inst& d, hipStream_t st);
 hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st);
 // The one-workgroup Gaussian elimination (variant 1-3) on a dense row-major system.
 hipError_t launch_gauss_dense(const double* A, const double* b, int N, double* x, int* status, int variant, hipStream_t st);
+// The blocked factorization and back substitution on lin ([N + 1][N] column-major, then 8N doubles and N
+// ints of workspace) with scal[4] cleared.
+hipError_t launch_gauss_blocked(double* lin, double* scal, int N, double* x, int* status, hipStream_t st);
 bool linearized_in_lds(int N);
 long linearized_global_doubles(int N);
 hipError_t launch_solve(const Inst& d, hipStream_t st);

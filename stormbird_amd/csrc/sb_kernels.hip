@@ -2174,101 +2174,150 @@ __global__ void k_lin_assemble(Inst d) {
     d.lin[idx] = c < N ? lin_entry(d, q, r, c) : -q.cll;
 }
 
-// Panel factorization of columns [k0, k0 + b): one workgroup, the panel's rows
-// [k0, N) staged in shared memory.  Row exchanges cover the whole panel width;
-// the factors replace the entries below the diagonal; the source position of
-// every row after the panel's exchanges goes to lin_perm for the trailing update.
-__global__ __launch_bounds__(1024) void k_lin_panel(Inst d, int k0, int b) {
+// Panel factorization of columns [k0, k0 + b), b <= 16: one workgroup of
+// LINP_T threads, thread t holding rows k0 + t + LINP_T s (s < 4) of the panel in
+// registers.  Rows are not exchanged while the panel is factored: every row keeps
+// its logical position.  Per pivot, each wave reduces its rows (the largest
+// magnitude, NaN excluded, ties to the lower position) and its best row publishes
+// its entries with the wave's result (LINP_SLOT doubles per wave, two sets by step
+// parity), and the row at position kk publishes its entries too (the pivot when it
+// holds NaN: linalg.rs:58-64 never replaces a NaN at the pivot position); after
+// one barrier every wave reduces the wave results and takes the pivot row from its
+// slot.  The factors replace the entries below the diagonal; rows are written back
+// at their logical positions, and the source position of every row goes to
+// lin_perm for the trailing update.
+constexpr int LINP_T = 256;                // threads of the panel kernel (4 rows each, N <= 1024)
+constexpr int LINP_W = LINP_T / 64;        // its waves
+constexpr int LINP_R = 4;                  // rows per thread
+constexpr int LINP_SLOT = 20;              // a candidate row (16 entries), |v|, key, row at position kk, NaN flag
+struct LinPanelState {
+    double a[LINP_R][16];
+    int pos[LINP_R];  // logical position of each row in the panel
+    bool done[LINP_R];
+};
+// Pivot KK of the panel (a template, so that every register index is fixed).
+// Returns false at a singular pivot (recorded in scal[4]).
+template <int KK>
+__device__ __forceinline__ bool lin_panel_step(const Inst& d, int k0, int b, double* slots, double* pks, LinPanelState& st) {
+    constexpr int R = LINP_R;
+    const int t = threadIdx.x, lane = t & (WAVE - 1), w = t >> 6;
+    double m = -1.0;
+#pragma unroll
+    for (int s = 0; s < R; ++s) m = fmax(m, st.done[s] ? -1.0 : fabs(st.a[s][KK]));  // NaN |v| never wins
+    const double Mw = wave_max_f64(m);
+    int key = 0x7fffffff;
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int kk2 = st.pos[s] * 1024 + t + LINP_T * s;
+        if (!st.done[s] && fabs(st.a[s][KK]) == Mw && kk2 < key) key = kk2;
+    }
+    const int Kw = wave_min_i32(key);
+    double* sl = slots + ((KK & 1) * LINP_W + w) * LINP_SLOT;
+    double* pk = pks + (KK & 1) * LINP_SLOT;
+    if (lane == 0) {
+        sl[16] = Mw;
+        sl[17] = (double)Kw;
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int r = t + LINP_T * s;
+        if (Kw != 0x7fffffff && r == (Kw & 1023)) {
+#pragma unroll
+            for (int c = KK; c < 16; ++c)
+                if (c < b) sl[c] = st.a[s][c];
+        }
+        if (!st.done[s] && st.pos[s] == KK) {
+#pragma unroll
+            for (int c = KK; c < 16; ++c)
+                if (c < b) pk[c] = st.a[s][c];
+            pk[16] = (double)r;
+            pk[17] = st.a[s][KK] != st.a[s][KK] ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();
+    double mr = -1.0;
+    int kr = 0x7fffffff;
+    if (lane < LINP_W) {
+        mr = slots[((KK & 1) * LINP_W + lane) * LINP_SLOT + 16];
+        kr = (int)slots[((KK & 1) * LINP_W + lane) * LINP_SLOT + 17];
+    }
+    const double M = wave_max_f64(mr);
+    const int K = wave_min_i32(mr == M ? kr : 0x7fffffff);
+    int pr, ppos;
+    const double* prow;
+    if (pk[17] != 0.0) {  // NaN at the pivot position: that row
+        pr = (int)pk[16];
+        ppos = KK;
+        prow = pk;
+    } else {
+        pr = K & 1023;
+        ppos = K >> 10;
+        prow = slots + ((KK & 1) * LINP_W + ((pr % LINP_T) >> 6)) * LINP_SLOT;
+    }
+    const double piv = prow[KK];
+    if (fabs(piv) < 1e-12) {  // uniform: every thread read the same slots
+        if (t == 0) d.scal[4] = (double)(k0 + KK + 1);
+        return false;
+    }
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+        const int r = t + LINP_T * s;
+        if (r == pr) {
+            st.pos[s] = KK;
+            st.done[s] = true;
+        } else if (!st.done[s] && st.pos[s] == KK) {
+            st.pos[s] = ppos;
+        }
+        if (!st.done[s]) {
+            const double f = st.a[s][KK] / piv;
+#pragma unroll
+            for (int c = KK + 1; c < 16; ++c)
+                if (c < b) st.a[s][c] = st.a[s][c] - f * prow[c];
+            st.a[s][KK] = f;
+        }
+    }
+    return true;
+}
+template <int KK>
+struct LinPanelSteps {
+    __device__ __forceinline__ static bool run(const Inst& d, int k0, int b, double* slots, double* pks, LinPanelState& st) {
+        if (KK >= b) return true;
+        if (!lin_panel_step<KK>(d, k0, b, slots, pks, st)) return false;
+        return LinPanelSteps<KK + 1>::run(d, k0, b, slots, pks, st);
+    }
+};
+template <>
+struct LinPanelSteps<16> {
+    __device__ __forceinline__ static bool run(const Inst&, int, int, double*, double*, LinPanelState&) { return true; }
+};
+
+__global__ __launch_bounds__(LINP_T) void k_lin_panel(Inst d, int k0, int b) {
     extern __shared__ double sh[];
     if (d.scal[4] != 0.0) return;
-    const int N = d.N, T = blockDim.x, t = threadIdx.x;
+    const int N = d.N, t = threadIdx.x;
     const int nr = N - k0;
-    double* P = sh;                   // [b][nr]
-    double* rv = P + (long)b * nr;    // [16]
-    double* rix = rv + 16;            // [16]
-    double* ctl = rix + 16;           // [4]
-    int* src = (int*)(ctl + 4);       // [nr]
-    for (long idx = t; idx < (long)b * nr; idx += T) {
-        int c = (int)(idx / nr), rr = (int)(idx - (long)c * nr);
-        P[idx] = d.lin[(long)(k0 + c) * N + k0 + rr];
+    double* slots = sh;                              // [2][LINP_W][LINP_SLOT]
+    double* pks = slots + 2 * LINP_W * LINP_SLOT;    // [2][LINP_SLOT]: the row at position kk
+    LinPanelState st;
+#pragma unroll
+    for (int s = 0; s < LINP_R; ++s) {
+        const int r = t + LINP_T * s;
+        st.pos[s] = r;
+        st.done[s] = r >= nr;  // past the panel: never a candidate, never updated
+#pragma unroll
+        for (int c = 0; c < 16; ++c) st.a[s][c] = r < nr && c < b ? d.lin[(long)(k0 + c) * N + k0 + r] : 0.0;
     }
-    for (int q = t; q < nr; q += T) src[q] = q;
-    if (t == 0) ctl[1] = 0.0;
-    __syncthreads();
-    const int nw = (T + WAVE - 1) / WAVE;
-    for (int kk = 0; kk < b; ++kk) {
-        double* col = P + (long)kk * nr;
-        double bv = -1.0;
-        int bi = nr;
-        for (int r = kk + t; r < nr; r += T) {
-            double v = fabs(col[r]);
-            if (v > bv) {
-                bv = v;
-                bi = r;
-            }
+    if (!LinPanelSteps<0>::run(d, k0, b, slots, pks, st)) return;
+#pragma unroll
+    for (int s = 0; s < LINP_R; ++s) {
+        const int r = t + LINP_T * s;
+        if (r < nr) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c < b) d.lin[(long)(k0 + c) * N + k0 + st.pos[s]] = st.a[s][c];
+            lin_perm(d)[st.pos[s]] = r;
         }
-        for (int off = WAVE / 2; off > 0; off >>= 1) {
-            double ov = __shfl_xor(bv, off, WAVE);
-            int oi = __shfl_xor(bi, off, WAVE);
-            if (ov > bv || (ov == bv && oi < bi)) {
-                bv = ov;
-                bi = oi;
-            }
-        }
-        if ((t % WAVE) == 0) {
-            rv[t / WAVE] = bv;
-            rix[t / WAVE] = (double)bi;
-        }
-        __syncthreads();
-        if (t == 0) {
-            double best = rv[0];
-            int p = (int)rix[0];
-            for (int w = 1; w < nw; ++w) {
-                double ov = rv[w];
-                int oi = (int)rix[w];
-                if (ov > best || (ov == best && oi < p)) {
-                    best = ov;
-                    p = oi;
-                }
-            }
-            double akk = col[kk];
-            if (akk != akk || p >= nr) p = kk;
-            ctl[0] = (double)p;
-            if (fabs(col[p]) < 1e-12) ctl[1] = (double)(k0 + kk + 1);
-            if (p != kk) {
-                int tmp = src[kk];
-                src[kk] = src[p];
-                src[p] = tmp;
-            }
-        }
-        __syncthreads();
-        if (ctl[1] != 0.0) {
-            if (t == 0) d.scal[4] = ctl[1];
-            return;
-        }
-        const int p = (int)ctl[0];
-        if (p != kk) {
-            for (int c = t; c < b; c += T) {
-                double a0 = P[(long)c * nr + kk];
-                P[(long)c * nr + kk] = P[(long)c * nr + p];
-                P[(long)c * nr + p] = a0;
-            }
-        }
-        __syncthreads();
-        const double piv = col[kk];
-        for (int r = kk + 1 + t; r < nr; r += T) {
-            const double f = col[r] / piv;
-            col[r] = f;
-            for (int c = kk + 1; c < b; ++c) P[(long)c * nr + r] = P[(long)c * nr + r] - f * P[(long)c * nr + kk];
-        }
-        __syncthreads();
     }
-    for (long idx = t; idx < (long)b * nr; idx += T) {
-        int c = (int)(idx / nr), rr = (int)(idx - (long)c * nr);
-        d.lin[(long)(k0 + c) * N + k0 + rr] = P[idx];
-    }
-    int* gp = lin_perm(d);
-    for (int q = t; q < nr; q += T) gp[q] = src[q];
 }
 
 // Trailing update after the panel [k0, k0 + b): one workgroup per column j >=
@@ -2338,6 +2387,21 @@ __global__ __launch_bounds__(1024) void k_lin_finish(Inst d, int as_start) {
     }
     __syncthreads();
     lin_finish(d, s, G, R, as_start);
+}
+
+// The blocked solve on a given system (checks of the elimination itself): the
+// back substitution of the factored d.lin into x; status = the singular step + 1
+// or 0.  Shared memory as k_lin_finish.
+__global__ __launch_bounds__(1024) void k_gauss_blocked_finish(Inst d, double* x, int* status) {
+    extern __shared__ double sh[];
+    const int N = d.N, t = threadIdx.x;
+    double* G = sh;
+    double* pb = G + linr_even(N + 8) + linr_even(N);
+    const int s = (int)d.scal[4];
+    if (s == 0 && t < WAVE) back_substitute_wave_cm(d.lin, N, G, pb, (int)linr_even(N + 8));
+    __syncthreads();
+    if (t == 0) *status = s;
+    for (int i = t; i < N; i += 1024) x[i] = s ? 0.0 : G[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -3089,11 +3153,22 @@ static hipError_t launch_linr_dense(const double* A, const double* b, int N, dou
 bool linearized_in_lds(int N) { return linr_variant(N) != 0; }
 
 
-// Panel width of the blocked form: up to 16 columns of N rows in shared memory.
-static int linearized_panel_width(int N) {
-    long room = lds_limit_bytes() - (long)sizeof(double) * 40 - (long)sizeof(int) * N;
-    long b = room / ((long)sizeof(double) * N);
-    return (int)(b < 16 ? b : 16);
+// Panel width of the blocked form: 16 columns (the panel kernel's registers per row).
+static int linearized_panel_width(int) { return 16; }
+
+// The blocked factorization of d.lin ([N + 1][N] column-major, N <= 1024): panels
+// of 16 columns and their trailing updates.
+static hipError_t launch_lin_blocked_factor(const Inst& d, hipStream_t st) {
+    const int N = d.N;
+    const int b = linearized_panel_width(N);
+    for (int k0 = 0; k0 < N; k0 += b) {
+        const int bb = N - k0 < b ? N - k0 : b;
+        const size_t shm = sizeof(double) * (size_t)(2 * LINP_W * LINP_SLOT + 2 * LINP_SLOT);
+        hipLaunchKernelGGL(k_lin_panel, dim3(1), dim3(LINP_T), shm, st, d, k0, bb);
+        const unsigned cols = (unsigned)(N + 1 - k0 - bb);
+        hipLaunchKernelGGL(k_lin_trailing, dim3(cols), dim3(LIN_TRAIL_THREADS), 0, st, d, k0, bb);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
@@ -3110,20 +3185,10 @@ hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
         }
     }
     if (!d.lin) return hipErrorInvalidValue;
-    const int b = linearized_panel_width(N);
-    if (b < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_lin_rows, dim3(blocks_for(N, 256)), dim3(256), 0, st, d);
     hipLaunchKernelGGL(k_lin_assemble, dim3(blocks_for((long)N * (N + 1), 256)), dim3(256), 0, st, d);
-    for (int k0 = 0; k0 < N; k0 += b) {
-        const int bb = N - k0 < b ? N - k0 : b;
-        const int nr = N - k0;
-        size_t shm = sizeof(double) * ((size_t)bb * nr + 36) + sizeof(int) * (size_t)nr;
-        hipError_t e = allow_shared((const void*)k_lin_panel, shm);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k_lin_panel, dim3(1), dim3(1024), shm, st, d, k0, bb);
-        const unsigned cols = (unsigned)(N + 1 - k0 - bb);
-        hipLaunchKernelGGL(k_lin_trailing, dim3(cols), dim3(LIN_TRAIL_THREADS), 0, st, d, k0, bb);
-    }
+    hipError_t eb = launch_lin_blocked_factor(d, st);
+    if (eb != hipSuccess) return eb;
     const size_t fin = sizeof(double) * (size_t)(linr_even(N + 8) + linr_even(N) + 2 * linr_even(N + 8) + 16);
     hipError_t ef = allow_shared((const void*)k_lin_finish, fin);
     if (ef != hipSuccess) return ef;
@@ -3140,6 +3205,21 @@ hipError_t launch_gauss_dense(const double* A, const double* b, int N, double* x
         case 3: return launch_linr_dense<3>(A, b, N, x, status, shm, st);
         default: return launch_linr_dense<4>(A, b, N, x, status, shm, st);
     }
+}
+
+hipError_t launch_gauss_blocked(double* lin, double* scal, int N, double* x, int* status, hipStream_t st) {
+    if (N < 1 || N > 1024) return hipErrorInvalidValue;
+    Inst d{};
+    d.N = N;
+    d.lin = lin;
+    d.scal = scal;
+    hipError_t e = launch_lin_blocked_factor(d, st);
+    if (e != hipSuccess) return e;
+    const size_t fin = sizeof(double) * (size_t)(linr_even(N + 8) + linr_even(N) + 2 * linr_even(N + 8) + 16);
+    e = allow_shared((const void*)k_gauss_blocked_finish, fin);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_gauss_blocked_finish, dim3(1), dim3(1024), fin, st, d, x, status);
+    return hipGetLastError();
 }
 
 // Doubles of d.lin for the blocked form: matrix, row data, exchange sources.

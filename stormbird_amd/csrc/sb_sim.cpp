@@ -411,6 +411,35 @@ int sb_debug_gauss_device(const double* A, const double* b, size_t n, double* x,
     return rc;
 }
 
+int sb_debug_gauss_blocked_device(const double* A, const double* b, size_t n, double* x) {
+    if (!A || !b || !x || n < 1 || n > 1024) return -1;
+    double* d = nullptr;
+    int rc = 0;
+    try {
+        const size_t lin = n * (n + 1) + 8 * n + (n + 1) / 2 + 1;  // matrix, row data, exchange sources
+        d = dalloc<double>(lin + 8 + n + 1);
+        std::vector<double> cm(n * (n + 1));
+        for (size_t i = 0; i < n; ++i) {
+            for (size_t j = 0; j < n; ++j) cm[j * n + i] = A[i * n + j];
+            cm[n * n + i] = b[i];
+        }
+        check(hipMemcpy(d, cm.data(), sizeof(double) * cm.size(), hipMemcpyHostToDevice), "upload");
+        double* scal = d + lin;
+        check(hipMemset(scal, 0, sizeof(double) * 8), "memset");
+        int* status = (int*)(scal + 8 + n);
+        check(launch_gauss_blocked(d, scal, (int)n, scal + 8, status, nullptr), "gauss blocked");
+        int st = 0;
+        check(hipMemcpy(&st, status, sizeof(int), hipMemcpyDeviceToHost), "download");
+        check(hipMemcpy(x, scal + 8, sizeof(double) * n, hipMemcpyDeviceToHost), "download");
+        rc = st;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        rc = -2;
+    }
+    if (d) (void)hipFree(d);
+    return rc;
+}
+
 int sb_debug_libm_host(int fn, const double* x, const double* y, double* out, size_t n) {
     if (!x || !out || (fn == 4 && !y)) return -1;
     for (size_t i = 0; i < n; ++i) {

@@ -6,7 +6,8 @@ The solution must equal the oracle's bit for bit: the same pivot choice (largest
 magnitude, the first of equal ones, a NaN at the pivot position kept), the same
 a - f * p updates and the same ascending-column back substitution.  Cases: random
 systems of every size class of the four register layouts (one to three 64-row
-slices of the pivot search, two to four rows and 12-20 columns per thread), systems with many equal
+slices of the pivot search, two to four rows and 12-20 columns per thread) and of the
+blocked form (16-column panels; partial and several panels, up to 1024 rows), systems with many equal
 magnitudes (pivot ties), zero diagonals (row exchanges at every step), NaN and
 infinite entries, and singular systems, whose failing step must be the oracle's.
 """
@@ -28,12 +29,17 @@ def _p(a):
 
 
 def gpu_solve(A, b, variant):
+    """variant 1-4: the one-workgroup register layouts; "blocked": the global-memory form."""
     L = sb.lib()
-    L.sb_debug_gauss_device.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, ctypes.c_int]
-    L.sb_debug_gauss_device.restype = ctypes.c_int
     A = np.ascontiguousarray(A, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.full(b.size, 7.0)
+    if variant == "blocked":
+        L.sb_debug_gauss_blocked_device.argtypes = [_dp, _dp, ctypes.c_size_t, _dp]
+        L.sb_debug_gauss_blocked_device.restype = ctypes.c_int
+        return L.sb_debug_gauss_blocked_device(_p(A), _p(b), b.size, _p(x)), x
+    L.sb_debug_gauss_device.argtypes = [_dp, _dp, ctypes.c_size_t, _dp, ctypes.c_int]
+    L.sb_debug_gauss_device.restype = ctypes.c_int
     rc = L.sb_debug_gauss_device(_p(A), _p(b), b.size, _p(x), variant)
     return rc, x
 
@@ -65,13 +71,14 @@ def check(oracle, A, b, variant, where=""):
 SIZES = {1: [1, 2, 3, 7, 8, 9, 17, 31, 32, 33, 40, 63, 64],
          2: [1, 5, 31, 32, 33, 63, 64, 65, 96, 100, 120, 127],
          3: [2, 33, 64, 97, 120, 128],
-         4: [1, 9, 64, 65, 120, 128, 129, 137, 139]}
+         4: [1, 9, 64, 65, 120, 128, 129, 137, 139],
+         "blocked": [1, 2, 15, 16, 17, 33, 100, 150, 300, 1024]}
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, "blocked"])
 def test_random_systems(oracle, variant):
     for n in SIZES[variant]:
-        rng = np.random.default_rng(1000 * variant + n)
+        rng = np.random.default_rng((1000 * variant if variant != "blocked" else 5000) + n)
         A = rng.standard_normal((n, n)) + 0.1 * n * np.eye(n)
         b = rng.standard_normal(n)
         print(f"gauss variant {variant} n {n}", flush=True)
@@ -83,7 +90,7 @@ def test_random_systems(oracle, variant):
             check(oracle, A0, b, variant, "zero diagonal")
 
 
-@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("variant", [2, 4, "blocked"])
 def test_pivot_ties(oracle, variant):
     """Small integer entries: many equal magnitudes in the pivot column, so the
     first of them (in the exchanged order) must be taken, as linalg.rs:58-64 does."""
@@ -97,7 +104,7 @@ def test_pivot_ties(oracle, variant):
             check(oracle, -A, b, variant, f"ties negated {trial}")
 
 
-@pytest.mark.parametrize("variant", [2, 4])
+@pytest.mark.parametrize("variant", [2, 4, "blocked"])
 def test_nan_and_infinity(oracle, variant):
     for n in [3, 20, 70, 120]:
         rng = np.random.default_rng(7 + n)
@@ -117,7 +124,7 @@ def test_nan_and_infinity(oracle, variant):
         check(oracle, A, bn, variant, "NaN right-hand side")
 
 
-@pytest.mark.parametrize("variant", [1, 2, 4])
+@pytest.mark.parametrize("variant", [1, 2, 4, "blocked"])
 def test_singular_systems_stop_where_the_oracle_does(oracle, variant):
     for n in ([2, 9, 40, 64] if variant == 1 else [2, 9, 65, 120]):
         rng = np.random.default_rng(n)
@@ -136,3 +143,4 @@ def test_sizes_beyond_a_variant_are_rejected():
     assert gpu_solve(np.eye(128), np.ones(128), 2)[0] == -1
     assert gpu_solve(np.eye(129), np.ones(129), 3)[0] == -1
     assert gpu_solve(np.eye(140), np.ones(140), 4)[0] == -1  # shared memory
+    assert gpu_solve(np.eye(1025), np.ones(1025), "blocked")[0] == -1
