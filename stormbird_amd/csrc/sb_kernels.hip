@@ -1618,44 +1618,39 @@ __device__ inline int wave_pivot(const double* colk, int k, int N, int lane) {
 // column-major [N + 1][N]: a[i][j] = A[j N + i], column N the right-hand side),
 // N <= 1024.  Every lane runs the same ordered sum
 // sum = ((0 + a[i][i+1] x[i+1]) + a[i][i+2] x[i+2]) + ..., reading the products
-// a[i][j] x[j], j >= i + 2, from shared memory (pb, two rows of pstride >= N + 8;
-// x in shared memory too): lane t forms the products of row i - 1 while row i is
-// summed, from matrix entries loaded one row earlier, padded with +0.0 to a
-// multiple of 8 terms (the sum starts at +0.0 and cannot become -0.0).
+// a[i][j] x[j], j >= i + 2, from shared memory in groups of 16 (pb, two rows of
+// pstride >= N + 64; x in shared memory too): lane t forms the products of row
+// i - 1 while row i is summed, from matrix entries loaded one row earlier, every
+// slot from a valid index and +0.0 past the row's end (the sum starts at +0.0 and
+// cannot become -0.0).
 __device__ inline void back_substitute_wave_cm(const double* A, int N, double* x, double* pb, int pstride) {
     constexpr int PS = 16;  // product slots per lane: rows of up to 1024 products
     const int lane = threadIdx.x & (WAVE - 1);
+    const int nsl = (N + WAVE - 1) / WAVE;  // slots in use
     const long n = N;
     double xprev = (A[n * n + N - 1] - 0.0) / A[(n - 1) * n + N - 1];
     x[N - 1] = xprev;
     double cu[PS];  // a[i-1][i+1+t] for the products formed at step i
 #pragma unroll
-    for (int s = 0; s < PS; ++s) cu[s] = 0.0;
-    if (N >= 3 && lane == 0) cu[0] = A[(n - 1) * n + N - 3];
+    for (int s = 0; s < PS; ++s) {
+        const int r = N >= 3 ? N - 3 : 0;
+        cu[s] = s < nsl ? A[(long)min(N - 1 + lane + WAVE * s, N - 1) * n + r] : 0.0;
+    }
     for (int i = N - 2; i >= 0; --i) {
         const double di = A[(long)i * n + i], bi = A[n * n + i], u1 = A[(long)(i + 1) * n + i];
         double nu[PS];  // a[i-2][i+t], t < N - i: the products formed at step i - 1
+        const int r2 = i >= 2 ? i - 2 : 0;
 #pragma unroll
-        for (int s = 0; s < PS; ++s) {
-            const int t = lane + WAVE * s;
-            nu[s] = 0.0;
-            if (i >= 2 && t < N - i) nu[s] = A[(long)(i + t) * n + (i - 2)];
-        }
+        for (int s = 0; s < PS; ++s) nu[s] = s < nsl ? A[(long)min(i + lane + WAVE * s, N - 1) * n + r2] : 0.0;
         const int lenp = i >= 1 ? N - i - 1 : 0;
-        double px[PS];
-#pragma unroll
-        for (int s = 0; s < PS; ++s) {
-            const int t = lane + WAVE * s;
-            px[s] = t < lenp ? x[i + 1 + t] : 0.0;
-        }
         double sum = 0.0 + u1 * xprev;
         const int len = N - i - 2;
         if (len > 0) {
             const double2* p2 = (const double2*)(pb + (long)(i & 1) * pstride);
             double2 c0 = p2[0], c1 = p2[1], c2 = p2[2], c3 = p2[3];
-            for (int g = 0; g < len; g += 8) {
-                const double2* nx = p2 + (g >> 1) + 4;
-                const double2 n0 = nx[0], n1 = nx[1], n2 = nx[2], n3 = nx[3];
+            for (int g = 0; g < len; g += 16) {
+                const double2* nx = p2 + (g >> 1);
+                const double2 n0 = nx[4], n1 = nx[5], n2 = nx[6], n3 = nx[7];
                 sum += c0.x;
                 sum += c0.y;
                 sum += c1.x;
@@ -1664,19 +1659,27 @@ __device__ inline void back_substitute_wave_cm(const double* A, int N, double* x
                 sum += c2.y;
                 sum += c3.x;
                 sum += c3.y;
-                c0 = n0;
-                c1 = n1;
-                c2 = n2;
-                c3 = n3;
+                c0 = nx[8];
+                c1 = nx[9];
+                c2 = nx[10];
+                c3 = nx[11];
+                sum += n0.x;
+                sum += n0.y;
+                sum += n1.x;
+                sum += n1.y;
+                sum += n2.x;
+                sum += n2.y;
+                sum += n3.x;
+                sum += n3.y;
             }
         }
-        if (lenp > 0) {
-            double* po = pb + (long)((i - 1) & 1) * pstride;
-            const int pad = (lenp + 7) & ~7;
+        double* po = pb + (long)((i - 1) & 1) * pstride;
 #pragma unroll
-            for (int s = 0; s < PS; ++s) {
+        for (int s = 0; s < PS; ++s) {
+            if (s < nsl) {
                 const int t = lane + WAVE * s;
-                if (t < pad) po[t] = t < lenp ? cu[s] * px[s] : 0.0;
+                const double p = cu[s] * x[min(i + 1 + t, N - 1)];
+                po[t] = t < lenp ? p : 0.0;
             }
         }
         xprev = (bi - sum) / di;
@@ -1791,10 +1794,11 @@ struct LinrLayout {
     int pstride;  // product row stride of the back substitution
 };
 SB_HD inline long linr_even(long v) { return (v + 1) & ~1L; }
+constexpr int LINR_PS = 3;  // product slots per lane of the back substitution (N <= 192)
 template <int V>
 SB_HD inline LinrLayout linr_layout(int N) {
     LinrLayout L;
-    L.pstride = (int)linr_even(N + 8);
+    L.pstride = (int)(N + 40 > 64 * LINR_PS ? linr_even(N + 40) : 64 * LINR_PS);
     L.G = (long)N * (N + 1) + 8;
     L.R = L.G + linr_even(N + 8);
     L.cst = L.R + linr_even(N);
@@ -1836,14 +1840,16 @@ __device__ inline int wave_min_i32(int v) {
 }
 
 // linalg.rs:92-101 by one wave over the row-major upper triangle U (logical
-// rows, leading dimension N + 1).  Every lane runs the same ordered sum
-// sum = ((0 + u[i][i+1] x[i+1]) + u[i][i+2] x[i+2]) + ..., reading the products
-// u[i][j] x[j], j >= i + 2, from shared memory (pb, two rows of pstride): lane t
-// forms the products of row i - 1 while row i is summed (its x[j], j > i, are
-// known by then), padded with +0.0 to a multiple of 8 terms.  The padding leaves
-// the sum unchanged: it starts at +0.0 and cannot become -0.0.
+// rows, leading dimension N + 1), N <= 64 PS.  Every lane runs the same ordered
+// sum sum = ((0 + u[i][i+1] x[i+1]) + u[i][i+2] x[i+2]) + ..., reading the
+// products u[i][j] x[j], j >= i + 2, from shared memory (pb, two rows of pstride
+// >= max(N + 40, 64 PS)) in groups of 16: lane t forms the products of row i - 1
+// while row i is summed (its x[j], j > i, are known by then), every slot from a
+// valid index and +0.0 past the row's end.  The padding leaves the sum unchanged:
+// it starts at +0.0 and cannot become -0.0.  (Fewer instructions per row, not
+// fewer latencies, is what shortens this loop: one wave issues all of it.)
+template <int PS>
 __device__ inline void back_substitute_wave(const double* U, int N, double* x, double* pb, int pstride) {
-    constexpr int PS = 3;  // product slots per lane: rows of up to 192 products (N <= 193)
     const int lane = threadIdx.x & (WAVE - 1);
     const long ld = N + 1;
     double xprev = (U[(N - 1) * ld + N] - 0.0) / U[(N - 1) * ld + N - 1];
@@ -1851,27 +1857,24 @@ __device__ inline void back_substitute_wave(const double* U, int N, double* x, d
     for (int i = N - 2; i >= 0; --i) {
         const double* row = U + i * ld;
         const double di = row[i], bi = row[N], u1 = row[i + 1];
-        // products of row i - 1: u[i-1][i+1+t] x[i+1+t], t < lenp
-        const int lenp = i >= 1 ? N - i - 1 : 0;
-        double pu[PS], px[PS];
+        const int lenp = i >= 1 ? N - i - 1 : 0;  // products of row i - 1
+        const double* rowp = row - (i >= 1 ? ld : 0);
+        double pr[PS];
 #pragma unroll
         for (int s = 0; s < PS; ++s) {
             const int t = lane + WAVE * s;
-            pu[s] = 0.0;
-            px[s] = 0.0;
-            if (t < lenp) {
-                pu[s] = row[-ld + i + 1 + t];
-                px[s] = x[i + 1 + t];
-            }
+            const int j = min(i + 1 + t, N - 1);
+            const double p = rowp[j] * x[j];
+            pr[s] = t < lenp ? p : 0.0;
         }
         double sum = 0.0 + u1 * xprev;
         const int len = N - i - 2;
-        const double2* p2 = (const double2*)(pb + (long)(i & 1) * pstride);
         if (len > 0) {
+            const double2* p2 = (const double2*)(pb + (long)(i & 1) * pstride);
             double2 c0 = p2[0], c1 = p2[1], c2 = p2[2], c3 = p2[3];
-            for (int g = 0; g < len; g += 8) {
-                const double2* nx = p2 + (g >> 1) + 4;
-                const double2 n0 = nx[0], n1 = nx[1], n2 = nx[2], n3 = nx[3];
+            for (int g = 0; g < len; g += 16) {
+                const double2* nx = p2 + (g >> 1);
+                const double2 n0 = nx[4], n1 = nx[5], n2 = nx[6], n3 = nx[7];
                 sum += c0.x;
                 sum += c0.y;
                 sum += c1.x;
@@ -1880,21 +1883,23 @@ __device__ inline void back_substitute_wave(const double* U, int N, double* x, d
                 sum += c2.y;
                 sum += c3.x;
                 sum += c3.y;
-                c0 = n0;
-                c1 = n1;
-                c2 = n2;
-                c3 = n3;
+                c0 = nx[8];
+                c1 = nx[9];
+                c2 = nx[10];
+                c3 = nx[11];
+                sum += n0.x;
+                sum += n0.y;
+                sum += n1.x;
+                sum += n1.y;
+                sum += n2.x;
+                sum += n2.y;
+                sum += n3.x;
+                sum += n3.y;
             }
         }
-        if (lenp > 0) {
-            double* po = pb + (long)((i - 1) & 1) * pstride;
-            const int pad = (lenp + 7) & ~7;
+        double* po = pb + (long)((i - 1) & 1) * pstride;
 #pragma unroll
-            for (int s = 0; s < PS; ++s) {
-                const int t = lane + WAVE * s;
-                if (t < pad) po[t] = t < lenp ? pu[s] * px[s] : 0.0;
-            }
-        }
+        for (int s = 0; s < PS; ++s) po[lane + WAVE * s] = pr[s];
         xprev = (bi - sum) / di;
         x[i] = xprev;
     }
@@ -2100,7 +2105,10 @@ __device__ inline int gauss_regs(int N, const Src& src, double* sh) {
         __syncthreads();
     }
     if (singular) return singular;
-    if (t < WAVE) back_substitute_wave(U, N, G, cst, L.pstride);
+    if (t < WAVE) {
+        if (N <= 128) back_substitute_wave<2>(U, N, G, cst, L.pstride);
+        else back_substitute_wave<LINR_PS>(U, N, G, cst, L.pstride);
+    }
     __syncthreads();
     return 0;
 }
@@ -2371,21 +2379,21 @@ __global__ __launch_bounds__(LIN_TRAIL_THREADS) void k_lin_trailing(Inst d, int 
 }
 
 // Back substitution and the rest of the solve after the blocked factorization.
-// Shared memory: G [N] (+8), R [N], two product rows of linr_even(N + 8) (+16).
+// Shared memory: G [N] (+8), R [N], two product rows of linr_even(N + 64) (+16).
 __global__ __launch_bounds__(1024) void k_lin_finish(Inst d, int as_start) {
     extern __shared__ double sh[];
     const int N = d.N, t = threadIdx.x;
     double* G = sh;
     double* R = G + linr_even(N + 8);
     double* pb = R + linr_even(N);
-    const Strip s = load_strip(d, t < N ? t : N - 1);
     if (d.scal[4] != 0.0) {
         if (t == 0)
             for (int q = 0; q < N; ++q) G[q] = 0.0;
     } else if (t < WAVE) {
-        back_substitute_wave_cm(d.lin, N, G, pb, (int)linr_even(N + 8));
+        back_substitute_wave_cm(d.lin, N, G, pb, (int)linr_even(N + 64));
     }
     __syncthreads();
+    const Strip s = load_strip(d, t < N ? t : N - 1);
     lin_finish(d, s, G, R, as_start);
 }
 
@@ -2398,7 +2406,7 @@ __global__ __launch_bounds__(1024) void k_gauss_blocked_finish(Inst d, double* x
     double* G = sh;
     double* pb = G + linr_even(N + 8) + linr_even(N);
     const int s = (int)d.scal[4];
-    if (s == 0 && t < WAVE) back_substitute_wave_cm(d.lin, N, G, pb, (int)linr_even(N + 8));
+    if (s == 0 && t < WAVE) back_substitute_wave_cm(d.lin, N, G, pb, (int)linr_even(N + 64));
     __syncthreads();
     if (t == 0) *status = s;
     for (int i = t; i < N; i += 1024) x[i] = s ? 0.0 : G[i];
@@ -3189,7 +3197,7 @@ hipError_t launch_linearized(const Inst& d, int as_start, hipStream_t st) {
     hipLaunchKernelGGL(k_lin_assemble, dim3(blocks_for((long)N * (N + 1), 256)), dim3(256), 0, st, d);
     hipError_t eb = launch_lin_blocked_factor(d, st);
     if (eb != hipSuccess) return eb;
-    const size_t fin = sizeof(double) * (size_t)(linr_even(N + 8) + linr_even(N) + 2 * linr_even(N + 8) + 16);
+    const size_t fin = sizeof(double) * (size_t)(linr_even(N + 8) + linr_even(N) + 2 * linr_even(N + 64) + 16);
     hipError_t ef = allow_shared((const void*)k_lin_finish, fin);
     if (ef != hipSuccess) return ef;
     hipLaunchKernelGGL(k_lin_finish, dim3(1), dim3(1024), fin, st, d, as_start);
@@ -3215,7 +3223,7 @@ hipError_t launch_gauss_blocked(double* lin, double* scal, int N, double* x, int
     d.scal = scal;
     hipError_t e = launch_lin_blocked_factor(d, st);
     if (e != hipSuccess) return e;
-    const size_t fin = sizeof(double) * (size_t)(linr_even(N + 8) + linr_even(N) + 2 * linr_even(N + 8) + 16);
+    const size_t fin = sizeof(double) * (size_t)(linr_even(N + 8) + linr_even(N) + 2 * linr_even(N + 64) + 16);
     e = allow_shared((const void*)k_gauss_blocked_finish, fin);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_gauss_blocked_finish, dim3(1), dim3(1024), fin, st, d, x, status);
