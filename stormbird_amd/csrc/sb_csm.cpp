@@ -319,10 +319,10 @@ struct sb_csm {
                 best_i = i;
                 best_loading = x;
             }
-            kept.push_back(sim->h_out);
+            kept.emplace_back(sim->h_out.begin(), sim->h_out.end());
             x += dx;
         }
-        sim->h_out = kept[best_i];
+        sim->h_out.assign(kept[best_i].begin(), kept[best_i].end());
         sim->write_result(0.0, out);
         return best_loading;
     }

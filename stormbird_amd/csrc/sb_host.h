@@ -61,6 +61,27 @@ T* dalloc(size_t n) {
     return static_cast<T*>(p);
 }
 
+// Host memory in pinned pages (hipHostMalloc) for the per-step transfers, so that
+// they go by DMA without the runtime's staging copies.
+template <class T>
+struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <class U>
+    PinnedAlloc(const PinnedAlloc<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        check(hipHostMalloc(&p, (n ? n : 1) * sizeof(T), hipHostMallocPortable), "hipHostMalloc");
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) noexcept { (void)hipHostFree(p); }
+    template <class U>
+    bool operator==(const PinnedAlloc<U>&) const { return true; }
+    template <class U>
+    bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+using PinnedVec = std::vector<double, PinnedAlloc<double>>;
+
 inline const char* kTimerNames[] = {"wake_update", "fixed", "factors", "solve", "self_induction", "forces"};
 constexpr int kTimers = 6;
 
@@ -106,7 +127,6 @@ struct sb_sim {
     double *d_lin = nullptr, *d_lin_gamma = nullptr;  // Linearized solve scratch
     double* d_segs = nullptr;  // ring-edge geometry per panel, when the free wake is self-induced
     double hs_core = 0.0;                             // quasi-steady horseshoe core, per step
-    double h_singular = 0.0;
     size_t tmp_cap = 0;
     PanelRec* d_rec = nullptr;
     int *d_col = nullptr, *d_wing = nullptr, *d_span_src = nullptr;
@@ -116,7 +136,9 @@ struct sb_sim {
     bool balanced = true;
     int cur = 0, scur = 0;
 
-    std::vector<double> h_geo, h_out;
+    PinnedVec h_geo, h_out;
+    PinnedVec h_fs;                // staging of small host freestreams
+    PinnedVec h_flag;               // the singular-pivot flag of the Linearized solve [1]
     hipEvent_t ev[kTimers + 1] = {};
     hipEvent_t ev_input = nullptr;  // orders device inputs written on the null stream
     double pairs[3] = {0, 0, 0};
@@ -412,6 +434,7 @@ struct sb_sim {
                                  stream),
                   "upload");
         h_out.assign(OutLayout::total(N, W), 0.0);
+        h_flag.assign(1, 0.0);
         reset_state();
     }
 
@@ -451,6 +474,11 @@ struct sb_sim {
     void upload_freestream(const double* fs, size_t n, bool on_device) {
         size_t bytes = sizeof(double) * 3 * n;
         if (on_device) wait_for_null_stream(stream, ev_input);
+        if (!on_device && bytes <= (1u << 20)) {  // small: through pinned staging (the previous copy has ended)
+            h_fs.resize(3 * n);
+            std::memcpy(h_fs.data(), fs, bytes);
+            fs = h_fs.data();
+        }
         check(hipMemcpyAsync(d_fs, fs, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
               "upload freestream");
     }
@@ -560,7 +588,7 @@ struct sb_sim {
         check(hipMemcpyAsync(h_out.data(), d_out, sizeof(double) * h_out.size(), hipMemcpyDeviceToHost, stream),
               "download");
         bool lin = setup.solver.kind == 1 || setup.solver.start_linearized;
-        if (lin) check(hipMemcpyAsync(&h_singular, d_scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream), "download");
+        if (lin) check(hipMemcpyAsync(h_flag.data(), d_scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream), "download");
         check(hipStreamSynchronize(stream), "step");
         static const int order[kTimers][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
         for (int k = 0; k < kTimers; ++k) {
@@ -568,9 +596,9 @@ struct sb_sim {
             (void)hipEventElapsedTime(&ms, ev[order[k][0]], ev[order[k][1]]);
             timer_ms[k] = ms;
         }
-        if (lin && h_singular != 0.0) {  // linalg.rs:77-81, unwrapped at linearized.rs:139
-            long row = (long)h_singular - 1;
-            h_singular = 0.0;
+        if (lin && h_flag[0] != 0.0) {  // linalg.rs:77-81, unwrapped at linearized.rs:139
+            long row = (long)h_flag[0] - 1;
+            h_flag[0] = 0.0;
             check(hipMemsetAsync(d_scal + 4, 0, sizeof(double), stream), "memset");
             check(hipStreamSynchronize(stream), "step");
             throw std::runtime_error("Linearized solver: matrix is singular or nearly singular at row " +
