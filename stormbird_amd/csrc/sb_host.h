@@ -142,7 +142,19 @@ struct sb_sim {
     hipEvent_t ev[kTimers + 1] = {};
     hipEvent_t ev_input = nullptr;  // orders device inputs written on the null stream
     double pairs[3] = {0, 0, 0};
-    double timer_ms[kTimers] = {0};
+    mutable double timer_ms[kTimers] = {0};
+    mutable bool timers_stale = false;  // the step's event intervals are read when first asked for
+
+    void read_timers() const {
+        if (!timers_stale) return;
+        static const int order[kTimers][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
+        for (int k = 0; k < kTimers; ++k) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, ev[order[k][0]], ev[order[k][1]]);
+            timer_ms[k] = ms;
+        }
+        timers_stale = false;
+    }
 
     ~sb_sim() {
         (void)hipSetDevice(device);
@@ -590,12 +602,7 @@ struct sb_sim {
         bool lin = setup.solver.kind == 1 || setup.solver.start_linearized;
         if (lin) check(hipMemcpyAsync(h_flag.data(), d_scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream), "download");
         check(hipStreamSynchronize(stream), "step");
-        static const int order[kTimers][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {5, 6}};
-        for (int k = 0; k < kTimers; ++k) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, ev[order[k][0]], ev[order[k][1]]);
-            timer_ms[k] = ms;
-        }
+        timers_stale = true;
         if (lin && h_flag[0] != 0.0) {  // linalg.rs:77-81, unwrapped at linearized.rs:139
             long row = (long)h_flag[0] - 1;
             h_flag[0] = 0.0;

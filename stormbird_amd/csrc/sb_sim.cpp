@@ -290,6 +290,8 @@ int sb_sim_get_frozen_wake(const sb_sim* s, double* fixed, double* factors) {
 
 double sb_sim_last_kernel_ms(const sb_sim* s, const char* name) {
     if (!s || !name) return -1.0;
+    (void)hipSetDevice(s->device);
+    s->read_timers();
     for (int k = 0; k < kTimers; ++k)
         if (strcmp(name, kTimerNames[k]) == 0) return s->timer_ms[k];
     return -2.0;
