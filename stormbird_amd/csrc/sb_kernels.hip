@@ -1591,28 +1591,6 @@ __device__ inline double lin_entry(const Inst& d, const LinRow& q, int r, int c)
     return v;
 }
 
-// One wave's argmax of |colk[r]| over r in [k, N): larger wins, ties go to the
-// lower row, NaN never wins; every lane returns the same row (N if none).
-__device__ inline int wave_pivot(const double* colk, int k, int N, int lane) {
-    double bv = -1.0;
-    int bi = N;
-    for (int r = k + lane; r < N; r += WAVE) {
-        double v = fabs(colk[r]);
-        if (v > bv) {
-            bv = v;
-            bi = r;
-        }
-    }
-    for (int off = WAVE / 2; off > 0; off >>= 1) {
-        double ov = __shfl_xor(bv, off, WAVE);
-        int oi = __shfl_xor(bi, off, WAVE);
-        if (ov > bv || (ov == bv && oi < bi)) {
-            bv = ov;
-            bi = oi;
-        }
-    }
-    return bi;
-}
 
 // linalg.rs:92-101 by one wave over the factored matrix in global memory (A
 // column-major [N + 1][N]: a[i][j] = A[j N + i], column N the right-hand side),
