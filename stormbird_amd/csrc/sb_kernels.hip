@@ -1733,8 +1733,8 @@ __device__ inline void lin_finish(const Inst& d, const Strip& s, double* G, doub
 //     yet used, NaN excluded, ties to the lower logical position; the row at
 //     position k when that row holds NaN (linalg.rs:58-64 scans positions
 //     k + 1.. with a strict >, so a NaN at position k is never replaced);
-//   - each thread's factors f = a[r][k] / pivot, formed before the first barrier;
-//   - the pivot row's threads publish its columns >= k (prow, [8][C + 2]); barrier;
+//   - the pivot row's threads publish its columns >= k (prow, [8][C + 2]), and
+//     each thread forms its factors f = a[r][k] / pivot; barrier;
 //   - every row subtracts f * prow[j] entry by entry in the blocks of 4 register
 //     columns that hold a column >= k (the rows already used as pivots and the
 //     entries left of column k + 1 take harmless updates: they are never read
@@ -2010,12 +2010,6 @@ __device__ inline int gauss_regs(int N, const Src& src, double* sh) {
                 wpos[sl] = ppos;
             }
         }
-        double f[R];
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-            f[s] = fa[s] / piv;
-            asm volatile("" ::"v"(f[s]));  // formed before the barrier, while the pivot row is published
-        }
         const int b0 = k >> 5;  // blocks that hold a column >= k
         if (rg == pr % RG) {    // the pivot row's threads
             const int sp = pr / RG;
@@ -2032,6 +2026,13 @@ __device__ inline int gauss_regs(int N, const Src& src, double* sh) {
                     }
                 }
             }
+        }
+        // the factors, formed after the pivot row's stores are issued and before the barrier
+        double f[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            f[s] = fa[s] / piv;
+            asm volatile("" ::"v"(f[s]));
         }
         __syncthreads();
         double2 w[C / 2];
