@@ -978,7 +978,9 @@ __global__ __launch_bounds__(256) void k_fold(Inst d, const double* __restrict__
 // Few targets, many panels, in two passes over a chunk of panels
 // [p_begin, p_begin + cnt):
 //  k_terms      every (target, panel) term strength * u, one wave per 64
-//               targets x 16 panels, written to terms[c][q][t] (ld >= n_targets);
+//               targets x tp panels (tp <= 16: fewer when the chunk would give
+//               too few waves to fill the GPU), written to terms[c][q][t]
+//               (ld >= n_targets);
 //               a skipped panel (zero strength, or the target's own wing in
 //               neglect-self mode) writes +0;
 //  k_fold_terms one lane per (component, target) chain adds the chunk's terms
@@ -992,14 +994,14 @@ constexpr int TERM_PANELS = 16;
 template <int MODE>
 __global__ __launch_bounds__(256) void k_terms(Inst d, const double* __restrict__ targets, int n_targets,
                                                long p_begin, long cnt, int neglect, double* __restrict__ terms,
-                                               int ld) {
+                                               int ld, int tp) {
     int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int t = blockIdx.y * WAVE + lane;
     int tc = t < n_targets ? t : n_targets - 1;
     V3 x = load3(targets + 3 * tc);
     int wt = neglect ? d.wing_of[tc] : -1;
-    long q0 = ((long)blockIdx.x * 4 + wave) * TERM_PANELS;
-    for (int j = 0; j < TERM_PANELS; ++j) {
+    long q0 = ((long)blockIdx.x * 4 + wave) * tp;
+    for (int j = 0; j < tp; ++j) {
         long q = q0 + j;
         if (q >= cnt) break;
         int k = __builtin_amdgcn_readfirstlane((int)(p_begin + q));
@@ -3034,16 +3036,19 @@ hipError_t launch_fixed(const Inst& d, int mode, FoldBuffers* fb, hipStream_t st
         int first = 1;
         for (long b = p_begin; b < p_end; b += chunk) {
             long cnt = p_end - b < chunk ? p_end - b : chunk;
-            dim3 grid(blocks_for(cnt, 4 * TERM_PANELS), (unsigned)(ld / WAVE));
+            // panels per wave: 16, or fewer while the chunk gives less than ~16 waves per CU
+            int tp = TERM_PANELS;
+            while (tp > 1 && (cnt + tp - 1) / tp * (ld / WAVE) < 4096) tp >>= 1;
+            dim3 grid(blocks_for(cnt, 4 * tp), (unsigned)(ld / WAVE));
             if (mode == MODE_FAST)
                 hipLaunchKernelGGL(k_terms<MODE_FAST>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt, d.neglect_self,
-                                   fb->terms, ld);
+                                   fb->terms, ld, tp);
             else if (mode == MODE_FP32_KAHAN)
                 hipLaunchKernelGGL(k_terms<MODE_FP32_KAHAN>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt,
-                                   d.neglect_self, fb->terms, ld);
+                                   d.neglect_self, fb->terms, ld, tp);
             else
                 hipLaunchKernelGGL(k_terms<MODE_EXACT>, grid, dim3(256), 0, st, d, ctrl, d.N, b, cnt, d.neglect_self,
-                                   fb->terms, ld);
+                                   fb->terms, ld, tp);
             bool last = b + cnt >= p_end;
             hipLaunchKernelGGL(k_fold_terms, dim3(3 * ((d.N + FOLD_CH - 1) / FOLD_CH)), dim3(FOLD_THREADS), 0, st,
                                fb->terms, ld, d.N, cnt, first, fb->acc, last ? d.fixed : nullptr);
