@@ -62,6 +62,7 @@ def synthesize_developed_state(sim, case, seed=1):
 def time_gpu(sim, fs_dev_ptr, n_points, dt, steps, warmup, t0_step):
     import torch
 
+    sim.set_kernel_timers(True)  # seven event records per step, ~25 us of a 590 ms step
     for k in range(warmup):
         sim.do_step_device(time=(t0_step + k) * dt, time_step=dt, freestream_ptr=fs_dev_ptr, n_points=n_points)
     torch.cuda.synchronize()
@@ -753,12 +754,17 @@ def quasi_steady(steps=50):
         fs = S.uniform_freestream(g.nr_freestream_points(), u)
         for _ in range(2):
             g.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
-        dev = []
         t0 = time.perf_counter()
         for _ in range(n):
             r = g.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
-            dev.append(sum(g.last_kernel_ms(k) for k in ("factors", "solve", "forces")))
         t_gpu = (time.perf_counter() - t0) / n
+        # the kernel split in further steps with the interval timers on (their event
+        # records would add ~25 us to each host-to-host step above)
+        g.set_kernel_timers(True)
+        dev = []
+        for _ in range(n):
+            g.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
+            dev.append(sum(g.last_kernel_ms(k) for k in ("factors", "solve", "forces")))
         o = oracle_py.Simulation(js, "glibc")
         o.do_step(time=0.0, time_step=1.0, freestream_velocity=fs)
         n_cpu = max(1, min(n, 5)) if g.nr_ctrl_points > 500 else n
@@ -913,6 +919,7 @@ def full_run(steps=2000, nr_strips=40, dump=None):
     js, case = S.four_sail_free_wake(nr_panels=steps, nr_strips=nr_strips)
     dt = case["time_step"]
     sim = sb.Simulation(js, mode=sb.MODE_EXACT)
+    sim.set_kernel_timers(True)
     fs = S.uniform_freestream(sim.nr_freestream_points(), case["velocity"])
     fs_dev = torch.from_numpy(np.ascontiguousarray(fs)).to("cuda")
     n_pts = sim.nr_freestream_points()

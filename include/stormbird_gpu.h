@@ -183,9 +183,15 @@ int sb_sim_get_panels(const sb_sim* sim, double* records);
  * row-major (wake/frozen_wake.rs:31-47). */
 int sb_sim_get_frozen_wake(const sb_sim* sim, double* fixed, double* factors);
 
+/* Per-kernel interval timers (no counterpart in the reference): with on != 0,
+ * every later step records HIP events between its kernels on the handle's
+ * stream; off (the default) it records none, since the seven records cost about
+ * 25 us of a 0.18 ms step.  Returns 0, or -1 for a null handle. */
+int sb_sim_set_kernel_timers(sb_sim* sim, int on);
 /* Per-kernel timing of the last step (milliseconds, HIP events on the
  * handle's stream). names: "wake_update", "fixed", "factors", "solve",
- * "forces", "self_induction".  Returns <0 for an unknown name. */
+ * "forces", "self_induction".  Returns -2 for an unknown name and -3 when the
+ * last step ran with the timers off (sb_sim_set_kernel_timers). */
 double sb_sim_last_kernel_ms(const sb_sim* sim, const char* name);
 /* Pair evaluations with nonzero panel strength in the last step:
  * [0] self-induction, [1] fixed velocities, [2] unit-strength factor pairs. */

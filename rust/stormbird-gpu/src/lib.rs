@@ -124,6 +124,7 @@ pub mod ffi {
         pub fn sb_sim_set_wake_state_ex(sim: *mut sb_sim, points: *const f64, n_points: usize, velocities: *const f64, n_velocities: usize, strengths: *const f64, n_strengths: usize, steps_completed: c_long, first_step_done: c_int) -> c_int;
         pub fn sb_sim_get_panels(sim: *const sb_sim, records: *mut f64) -> c_int;
         pub fn sb_sim_get_frozen_wake(sim: *const sb_sim, fixed: *mut f64, factors: *mut f64) -> c_int;
+        pub fn sb_sim_set_kernel_timers(sim: *mut sb_sim, on: c_int) -> c_int;
         pub fn sb_sim_last_kernel_ms(sim: *const sb_sim, name: *const c_char) -> f64;
         pub fn sb_sim_last_pair_counts(sim: *const sb_sim, counts3: *mut f64) -> c_int;
         pub fn sb_sim_set_target_tiling(sim: *mut sb_sim, span_points_per_wave: c_int) -> c_int;

@@ -71,6 +71,8 @@ def lib():
                                                ctypes.c_size_t, ctypes.c_long, ctypes.c_int]
         L.sb_sim_get_panels.argtypes = [vp, _dp]
         L.sb_sim_get_frozen_wake.argtypes = [vp, _dp, _dp]
+        L.sb_sim_set_kernel_timers.restype = ctypes.c_int
+        L.sb_sim_set_kernel_timers.argtypes = [vp, ctypes.c_int]
         L.sb_sim_last_kernel_ms.restype = ctypes.c_double
         L.sb_sim_last_kernel_ms.argtypes = [vp, ctypes.c_char_p]
         L.sb_sim_last_pair_counts.argtypes = [vp, _dp]
@@ -366,8 +368,23 @@ class Simulation:
         if rc != 0:
             _err("set_solver_settings", rc)
 
+    def set_kernel_timers(self, on: bool = True):
+        """Per-kernel interval timers of later steps on or off (off by default: the
+        event records cost about 25 us per step)."""
+        rc = lib().sb_sim_set_kernel_timers(self._h, 1 if on else 0)
+        if rc != 0:
+            _err("set_kernel_timers", rc)
+
     def last_kernel_ms(self, name: str) -> float:
-        return lib().sb_sim_last_kernel_ms(self._h, name.encode())
+        """Milliseconds of one kernel interval of the last step ("wake_update", "fixed",
+        "factors", "solve", "forces", "self_induction"); the last step must have run with
+        the timers on."""
+        v = lib().sb_sim_last_kernel_ms(self._h, name.encode())
+        if v == -2.0:
+            raise ValueError(f"unknown kernel interval {name!r}")
+        if v == -3.0:
+            raise RuntimeError("the last step ran without kernel timers (set_kernel_timers(True))")
+        return v
 
     def count_self_induction_pairs(self):
         """(near, total) pairs with nonzero strength of the self-induction sum at the current state."""

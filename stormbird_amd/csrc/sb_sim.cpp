@@ -288,8 +288,18 @@ int sb_sim_get_frozen_wake(const sb_sim* s, double* fixed, double* factors) {
     });
 }
 
+int sb_sim_set_kernel_timers(sb_sim* s, int on) {
+    if (!s) return -1;
+    s->timers_on = on != 0;
+    return 0;
+}
+
 double sb_sim_last_kernel_ms(const sb_sim* s, const char* name) {
     if (!s || !name) return -1.0;
+    bool known = false;
+    for (int k = 0; k < kTimers; ++k) known = known || strcmp(name, kTimerNames[k]) == 0;
+    if (!known) return -2.0;
+    if (!s->last_step_timed) return -3.0;
     (void)hipSetDevice(s->device);
     s->read_timers();
     for (int k = 0; k < kTimers; ++k)

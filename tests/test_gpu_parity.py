@@ -103,6 +103,31 @@ def test_single_wing_sail_200_steps_bitwise(oracle):
     step_both(gpu, orc, fs, case["steps"], case["time_step"], wake_check_every=50)
 
 
+def test_kernel_timers_are_opt_in_and_change_no_bit(oracle):
+    """Per-kernel interval timers are off by default (last_kernel_ms then raises) and change
+    no bit when on: a four-sail free wake of 40 rows stepped with them off, on, and off
+    again, against the oracle."""
+    js, case = S.four_sail_free_wake(nr_panels=40, nr_strips=10)
+    dt = case["time_step"]
+    gpu = sb.Simulation(js)
+    orc = oracle.Simulation(js, "pm")
+    fs = S.uniform_freestream(gpu.nr_freestream_points(), case["velocity"])
+    step_both(gpu, orc, fs, 5, dt)
+    with pytest.raises(RuntimeError):
+        gpu.last_kernel_ms("solve")
+    with pytest.raises(ValueError):
+        gpu.last_kernel_ms("no such interval")
+    gpu.set_kernel_timers(True)
+    step_both(gpu, orc, fs, 5, dt, t0=5, wake_check_every=1)
+    ms = {k: gpu.last_kernel_ms(k) for k in ("wake_update", "fixed", "factors", "solve", "self_induction", "forces")}
+    assert all(v >= 0.0 for v in ms.values()), ms
+    assert ms["solve"] > 0.0 and ms["self_induction"] > 0.0, ms
+    gpu.set_kernel_timers(False)
+    step_both(gpu, orc, fs, 2, dt, t0=10, wake_check_every=1)
+    with pytest.raises(RuntimeError):
+        gpu.last_kernel_ms("solve")
+
+
 def test_single_wing_sail_vs_host_libm_oracle(oracle):
     """Against the oracle with the host libm (the Rust build's sin/exp/acos).
 
