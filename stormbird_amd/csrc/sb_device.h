@@ -164,6 +164,8 @@ hipError_t launch_solve(const Inst& d, hipStream_t st);
 size_t solve_shared_bytes(int N, int W);
 hipError_t launch_set_row0(const Inst& d, hipStream_t st);
 hipError_t launch_velocity_copy(const Inst& d, hipStream_t st);
+// dst[0, 3n) = src[0, 3n); src may be pinned host memory (read over the bus)
+hipError_t launch_copy3(const double* src, double* dst, long n, hipStream_t st);
 hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStream_t st);
 // Induced velocity at `targets` from panels [0, n_panels): out (+)= sum.
 hipError_t launch_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode, double* out,

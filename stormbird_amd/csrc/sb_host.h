@@ -137,7 +137,8 @@ struct sb_sim {
     int cur = 0, scur = 0;
 
     PinnedVec h_geo, h_out;
-    PinnedVec h_fs;                // staging of small host freestreams
+    PinnedVec h_fs;                // host freestreams up to 1 MiB, read by the upload kernel
+    bool fs_copy_pending = false;  // an upload kernel may still read h_fs (cleared once the step has ended)
     PinnedVec h_flag;               // the singular-pivot flag of the Linearized solve [1]
     hipEvent_t ev[kTimers + 1] = {};
     hipEvent_t ev_input = nullptr;  // orders device inputs written on the null stream
@@ -494,13 +495,20 @@ struct sb_sim {
     void upload_freestream(const double* fs, size_t n, bool on_device) {
         size_t bytes = sizeof(double) * 3 * n;
         if (on_device) wait_for_null_stream(stream, ev_input);
-        // small freestreams go through pinned staging (the previous copy has ended);
-        // larger ones directly: measured, a 101 KB pinned copy per step cost ~0.2 ms
-        // more than the runtime's own staged copy of the pageable array
-        if (!on_device && bytes <= (16u << 10)) {
+        // Host freestreams up to 1 MiB are copied into pinned memory and read over the
+        // bus by a copy kernel on the step's stream: for the 101 KB of a single wing sail
+        // this measured faster than the runtime's copy of the pageable array, and a DMA
+        // copy from pinned memory measured ~0.2 ms slower.  Larger arrays take the
+        // runtime's copy.  The pinned buffer is reused once the previous step has ended.
+        if (!on_device && bytes <= (1u << 20)) {
+            if (fs_copy_pending) check(hipStreamSynchronize(stream), "previous step");
             h_fs.resize(3 * n);
             std::memcpy(h_fs.data(), fs, bytes);
-            fs = h_fs.data();
+            void* dp = nullptr;
+            check(hipHostGetDevicePointer(&dp, h_fs.data(), 0), "pinned freestream");
+            fs_copy_pending = true;
+            check(launch_copy3((const double*)dp, d_fs, (long)n, stream), "upload freestream");
+            return;
         }
         check(hipMemcpyAsync(d_fs, fs, bytes, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, stream),
               "upload freestream");
@@ -613,6 +621,7 @@ struct sb_sim {
         bool lin = setup.solver.kind == 1 || setup.solver.start_linearized;
         if (lin) check(hipMemcpyAsync(h_flag.data(), d_scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream), "download");
         check(hipStreamSynchronize(stream), "step");
+        fs_copy_pending = false;
         timers_stale = timers_on;
         last_step_timed = timers_on;
         if (lin && h_flag[0] != 0.0) {  // linalg.rs:77-81, unwrapped at linearized.rs:139

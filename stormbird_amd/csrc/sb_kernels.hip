@@ -3238,6 +3238,12 @@ hipError_t launch_solve(const Inst& d, hipStream_t st) {
 
 hipError_t launch_set_row0(const Inst&, hipStream_t) { return hipSuccess; }
 
+hipError_t launch_copy3(const double* src, double* dst, long n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy3, dim3(blocks_for(3 * n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_velocity_copy(const Inst& d, hipStream_t st) {
     long n = (long)d.P * (d.M + 1);
     hipLaunchKernelGGL(k_copy3, dim3(blocks_for(3 * n, 256)), dim3(256), 0, st, d.fs + 3L * d.N, d.vel, n);
