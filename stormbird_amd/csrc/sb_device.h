@@ -106,7 +106,8 @@ struct OutLayout {
     SB_HD static long integ_m(int N, int W) { return 29L * N + 18L * W; }
     SB_HD static long iters(int N, int W) { return 29L * N + 36L * W; }
     SB_HD static long residual(int N, int W) { return 29L * N + 36L * W + 1; }
-    SB_HD static long total(int N, int W) { return 29L * N + 36L * W + 2; }
+    SB_HD static long singular(int N, int W) { return 29L * N + 36L * W + 2; }  // Linearized: failing step + 1, or 0
+    SB_HD static long total(int N, int W) { return 29L * N + 36L * W + 3; }
 };
 
 enum Mode { MODE_EXACT = 0, MODE_FAST = 1, MODE_FP32_KAHAN = 2 };
@@ -166,6 +167,8 @@ hipError_t launch_set_row0(const Inst& d, hipStream_t st);
 hipError_t launch_velocity_copy(const Inst& d, hipStream_t st);
 // dst[0, 3n) = src[0, 3n); src may be pinned host memory (read over the bus)
 hipError_t launch_copy3(const double* src, double* dst, long n, hipStream_t st);
+// dst[0, count) = src[0, count); src may be pinned host memory
+hipError_t launch_copy_doubles(const double* src, double* dst, long count, hipStream_t st);
 hipError_t launch_self_induction(const Inst& d, int mode, McBuffers* mc, hipStream_t st);
 // Induced velocity at `targets` from panels [0, n_panels): out (+)= sum.
 hipError_t launch_targets(const Inst& d, const double* targets, long n_targets, long n_panels, int mode, double* out,

@@ -61,8 +61,9 @@ T* dalloc(size_t n) {
     return static_cast<T*>(p);
 }
 
-// Host memory in pinned pages (hipHostMalloc) for the per-step transfers, so that
-// they go by DMA without the runtime's staging copies.
+// Host memory in pinned pages (hipHostMalloc) for the per-step transfers: the
+// results go back by DMA without the runtime's staging copies, and the geometry and
+// small freestreams are read from it by copy kernels.
 template <class T>
 struct PinnedAlloc {
     using value_type = T;
@@ -139,7 +140,7 @@ struct sb_sim {
     PinnedVec h_geo, h_out;
     PinnedVec h_fs;                // host freestreams up to 1 MiB, read by the upload kernel
     bool fs_copy_pending = false;  // an upload kernel may still read h_fs (cleared once the step has ended)
-    PinnedVec h_flag;               // the singular-pivot flag of the Linearized solve [1]
+    bool geo_copy_pending = false;  // the same for h_geo
     hipEvent_t ev[kTimers + 1] = {};
     hipEvent_t ev_input = nullptr;  // orders device inputs written on the null stream
     double pairs[3] = {0, 0, 0};
@@ -340,7 +341,10 @@ struct sb_sim {
     // uploads it to d_geo when dst is null.
     void pack_geometry(double* dst = nullptr) {
         const LineModel& m = setup.lfm;
-        if (!dst) h_geo.assign(gl.total, 0.0);
+        if (!dst) {
+            if (geo_copy_pending) check(hipStreamSynchronize(stream), "previous geometry upload");
+            h_geo.assign(gl.total, 0.0);
+        }
         double* g = dst ? dst : h_geo.data();
         if (dst) std::fill(dst, dst + gl.total, 0.0);
         auto put = [](double* p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; };
@@ -373,9 +377,12 @@ struct sb_sim {
         put(g + gl.body_axes, m.rbm.rotate(v3(1, 0, 0)));
         put(g + gl.body_axes + 3, m.rbm.rotate(v3(0, 1, 0)));
         put(g + gl.body_axes + 6, m.rbm.rotate(v3(0, 0, 1)));
-        if (!dst)
-            check(hipMemcpyAsync(d_geo, g, sizeof(double) * gl.total, hipMemcpyHostToDevice, stream),
-                  "upload geometry");
+        if (!dst) {  // read over the bus by a copy kernel, as the host freestream
+            void* dp = nullptr;
+            check(hipHostGetDevicePointer(&dp, h_geo.data(), 0), "pinned geometry");
+            geo_copy_pending = true;
+            check(launch_copy_doubles((const double*)dp, d_geo, (long)gl.total, stream), "upload geometry");
+        }
     }
 
     void create(const char* json, int dev, int md) { create(parse_setup(json), dev, md); }
@@ -455,7 +462,6 @@ struct sb_sim {
                                  stream),
                   "upload");
         h_out.assign(OutLayout::total(N, W), 0.0);
-        h_flag.assign(1, 0.0);
         reset_state();
     }
 
@@ -618,15 +624,13 @@ struct sb_sim {
     void finish_step(double time, sb_step_result* out) {
         check(hipMemcpyAsync(h_out.data(), d_out, sizeof(double) * h_out.size(), hipMemcpyDeviceToHost, stream),
               "download");
-        bool lin = setup.solver.kind == 1 || setup.solver.start_linearized;
-        if (lin) check(hipMemcpyAsync(h_flag.data(), d_scal + 4, sizeof(double), hipMemcpyDeviceToHost, stream), "download");
         check(hipStreamSynchronize(stream), "step");
-        fs_copy_pending = false;
+        fs_copy_pending = geo_copy_pending = false;
         timers_stale = timers_on;
         last_step_timed = timers_on;
-        if (lin && h_flag[0] != 0.0) {  // linalg.rs:77-81, unwrapped at linearized.rs:139
-            long row = (long)h_flag[0] - 1;
-            h_flag[0] = 0.0;
+        const double singular = h_out[OutLayout::singular(N, W)];  // the Linearized solve's flag
+        if (singular != 0.0) {  // linalg.rs:77-81, unwrapped at linearized.rs:139
+            long row = (long)singular - 1;
             check(hipMemsetAsync(d_scal + 4, 0, sizeof(double), stream), "memset");
             check(hipStreamSynchronize(stream), "step");
             throw std::runtime_error("Linearized solver: matrix is singular or nearly singular at row " +

@@ -2489,7 +2489,10 @@ __device__ inline void forces_body(const Inst& d, double dt) {
         store3(o + OutLayout::integ_f(N) + 3 * ((long)k * W + w), fs);
         store3(o + OutLayout::integ_m(N, W) + 3 * ((long)k * W + w), ms);
     }
-    if (i == 0) d.scal[3] = count + 1.0;
+    if (i == 0) {
+        d.scal[3] = count + 1.0;
+        d.out[OutLayout::singular(d.N, d.W)] = d.scal[4];  // with the results, one download
+    }
 }
 __global__ __launch_bounds__(1024) void k_forces(Inst d, double dt) { forces_body(d, dt); }
 
@@ -3241,6 +3244,17 @@ hipError_t launch_set_row0(const Inst&, hipStream_t) { return hipSuccess; }
 hipError_t launch_copy3(const double* src, double* dst, long n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_copy3, dim3(blocks_for(3 * n, 256)), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError();
+}
+
+__global__ void k_copy_doubles(const double* __restrict__ src, double* __restrict__ dst, long count) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < count) dst[q] = src[q];
+}
+
+hipError_t launch_copy_doubles(const double* src, double* dst, long count, hipStream_t st) {
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_copy_doubles, dim3(blocks_for(count, 256)), dim3(256), 0, st, src, dst, count);
     return hipGetLastError();
 }
 

@@ -132,3 +132,29 @@ def test_quasi_steady_has_no_wake_api(oracle):
     assert g.nr_freestream_points() == g.nr_ctrl_points == 24
     with pytest.raises(RuntimeError, match="quasi-steady"):
         g.induced_velocities(np.zeros((2, 3)))
+
+
+@pytest.mark.parametrize("nr_strips,speed", [(40, 1e13), (340, 1e14)])
+def test_singular_linearized_step_fails_at_the_oracles_row(oracle, nr_strips, speed):
+    """A freestream of 1e13-1e14 m/s makes the entries of the Linearized system smaller than
+    the 1e-12 pivot limit (linalg.rs:67-71; the oracle stops at rows 40 and 340 of the two
+    setups): the step must fail with the row the oracle reports
+    (the GPU's flag comes back with the step's results), in the one-workgroup form (N = 120)
+    and the global-memory form (N = 1020).  The next step, at 1.2 m/s, must run and give the
+    oracle's circulation (the Linearized solve depends on the step's inputs only; the
+    reference panics at a singular system, so nothing else of the state after it is fixed)."""
+    import re
+
+    js = S.wing_sails_quasi_steady(nr_strips=nr_strips)
+    g = sb.Simulation(js)
+    n = g.nr_freestream_points()
+    big = S.uniform_freestream(n, [speed, 0.0, 0.0])
+    with pytest.raises(RuntimeError) as eo:
+        oracle.Simulation(js, "pm").do_step(time=0.0, time_step=1.0, freestream_velocity=big)
+    row = re.search(r"at row (\d+)", str(eo.value)).group(1)
+    with pytest.raises(RuntimeError, match=rf"singular.* at row {row}\b"):
+        g.do_step(time=0.0, time_step=1.0, freestream_velocity=big)
+    fs = S.uniform_freestream(n, [1.2, 0.0, 0.0])
+    rg = g.do_step(time=1.0, time_step=1.0, freestream_velocity=fs)
+    ro = oracle.Simulation(js, "pm").do_step(time=1.0, time_step=1.0, freestream_velocity=fs)
+    assert same(rg.as_arrays()["circulation_strength"], ro.as_arrays()["circulation_strength"])
