@@ -29,6 +29,7 @@
 
 #include "sb_host.h"
 #include "sb_json.h"
+#include "sb_pool.h"
 
 namespace {
 
@@ -42,33 +43,7 @@ T* halloc(size_t n) {
 
 constexpr int kStages = 4;  // before the wake step, after it, after the strength shift, after the solve
 
-// fn(i) for i in [0, n) over up to 16 host threads (contiguous chunks); the
-// first exception is rethrown here.  Used for per-instance host work, which
-// touches only that instance.
-template <class F>
-void parallel_for(int n, F&& fn) {
-    unsigned hw = std::thread::hardware_concurrency();
-    int nt = (int)std::min<unsigned>(hw ? hw : 1, 16u);
-    nt = std::min(nt, std::max(1, n / 32));
-    if (nt <= 1) {
-        for (int i = 0; i < n; ++i) fn(i);
-        return;
-    }
-    std::vector<std::thread> pool;
-    std::vector<std::exception_ptr> err(nt);
-    for (int w = 0; w < nt; ++w) {
-        pool.emplace_back([&, w] {
-            try {
-                for (int i = (int)((long)n * w / nt); i < (int)((long)n * (w + 1) / nt); ++i) fn(i);
-            } catch (...) {
-                err[w] = std::current_exception();
-            }
-        });
-    }
-    for (auto& th : pool) th.join();
-    for (auto& e : err)
-        if (e) std::rethrow_exception(e);
-}
+using sb_pool::parallel_for;  // per-instance host work, which touches only that instance
 
 }  // namespace
 
