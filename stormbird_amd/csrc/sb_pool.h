@@ -23,16 +23,11 @@ namespace sb_pool {
 
 class Workers {
 public:
-    explicit Workers(int n) : pid_(getpid()) {
-        for (int w = 1; w <= n; ++w) threads_.emplace_back([this, w] { loop(w); });
-    }
-    ~Workers() {
-        {
-            std::lock_guard<std::mutex> lk(m_);
-            stop_ = true;
-        }
-        work_.notify_all();
-        for (auto& t : threads_) t.join();
+    // The threads wait on a condition variable between calls and end with the
+    // process: the object is never destroyed (workers()), so neither the exit of this
+    // process nor that of a process forked from it joins threads.
+    explicit Workers(int n) : pid_(getpid()), n_threads_(n) {
+        for (int w = 1; w <= n; ++w) std::thread([this, w] { loop(w); }).detach();
     }
     Workers(const Workers&) = delete;
     Workers& operator=(const Workers&) = delete;
@@ -41,7 +36,7 @@ public:
     // threads.  Returns false (nothing run) when the kept threads are busy, too few,
     // or belong to another process.  fn must not throw (callers catch inside).
     bool run(int nt, const std::function<void(int)>& fn) {
-        if (getpid() != pid_ || nt - 1 > (int)threads_.size()) return false;
+        if (getpid() != pid_ || nt - 1 > n_threads_) return false;
         std::unique_lock<std::mutex> busy(run_m_, std::try_to_lock);
         if (!busy.owns_lock()) return false;
         {
@@ -66,8 +61,7 @@ private:
             const std::function<void(int)>* job = nullptr;
             {
                 std::unique_lock<std::mutex> lk(m_);
-                work_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
+                work_.wait(lk, [&] { return gen_ != seen; });
                 seen = gen_;
                 if (w > n_job_) continue;  // not needed for this call
                 job = job_;
@@ -79,14 +73,13 @@ private:
     }
 
     pid_t pid_;
-    std::vector<std::thread> threads_;
+    int n_threads_;
     std::mutex run_m_;  // held for a whole call
     std::mutex m_;
     std::condition_variable work_, done_;
     const std::function<void(int)>* job_ = nullptr;
     long gen_ = 0;
     int n_job_ = 0, pending_ = 0;
-    bool stop_ = false;
 };
 
 inline int max_threads() {
@@ -95,8 +88,8 @@ inline int max_threads() {
 }
 
 inline Workers& workers() {
-    static Workers w(max_threads() - 1);
-    return w;
+    static Workers* w = new Workers(max_threads() - 1);  // never destroyed
+    return *w;
 }
 
 template <class F>

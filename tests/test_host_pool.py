@@ -3,20 +3,23 @@
 parallel_for(n, fn) must call fn(i) exactly once for every i in [0, n) whatever n,
 rethrow an exception thrown by any call, and stay correct when it is called from two
 host threads at once or from inside fn (the kept threads are then busy and the call
-starts threads of its own).  A C++ program built with g++ checks each, and a planted
+starts threads of its own), and in a process forked from the one that started the kept
+threads.  A C++ program built with g++ checks each, and a planted
 mistake (a chunk boundary off by one) must fail the same checks.
 """
 import os
 import subprocess
 
-import pytest
-
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 POOL = os.path.join(ROOT, "stormbird_amd", "csrc", "sb_pool.h")
 
 PROGRAM = r"""
+#include <sys/wait.h>
+#include <unistd.h>
+
 #include <atomic>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <thread>
 #include <vector>
@@ -60,6 +63,22 @@ int main() {
     const long want = 2L * 50 * 256 * (63 * 64 / 2);
     if (sum != want) {
         printf("nested %ld vs %ld\n", (long)sum, want);
+        return 1;
+    }
+    // a process forked from this one has none of the kept threads: its calls start their own
+    fflush(stdout);
+    const pid_t child = fork();
+    if (child == 0) {
+        std::vector<int> h(1024);
+        parallel_for(1024, [&](int i) { h[i] = i; });
+        for (int i = 0; i < 1024; ++i)
+            if (h[i] != i) _exit(3);
+        exit(0);  // runs the child's exit handlers
+    }
+    int status = 0;
+    waitpid(child, &status, 0);
+    if (!WIFEXITED(status) || WEXITSTATUS(status) != 0) {
+        printf("forked child status %d\n", status);
         return 1;
     }
     printf("ok\n");
